@@ -1,0 +1,272 @@
+"""Python host mirror of the 2d-treecodes interface over libtreecodes2d.so.
+
+The reference exposes two C functions (README.md:5-25):
+    treecode_potential(theta, xs, ys, vs, n, xt, yt, m, out)
+    treecode_force_mrag(theta, xs, ys, vs, n, x0s, y0s, hs, nblocks, xr, yr)
+`treecode_potential` / `treecode_force_mrag` below call exactly those C
+symbols (host numpy arrays in, results out).  `Context` exposes the
+extensions: precision, order and leaf options, device-resident buffers
+(torch tensors on the GPU), phase timers, interaction counters and the
+resident tree.
+
+There is no CPU fallback: importing works without a GPU, but every call
+raises TreecodeError when the HIP library or a device is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtreecodes2d.so")
+
+OPT_PRECISION, OPT_POT_ORDER, OPT_POT_LEAF, OPT_FORCE_ORDER, OPT_FORCE_LEAF, OPT_PROFILE, OPT_COUNTERS = range(7)
+PH_BUILD, PH_TSORT, PH_EVAL, PH_TOTAL = range(4)
+CNT_P2P, CNT_E2P, CNT_E2L, CNT_L2P = range(4)
+FP64, FP32 = 0, 1
+
+EXPORTED = [
+    "treecode_potential", "treecode_force_mrag", "treecode_last_error", "tc_context_create",
+    "tc_context_destroy", "tc_set_option", "tc_context_stream", "tc_potential", "tc_force_mrag",
+    "tc_synchronize", "tc_potential_resident", "tc_force_mrag_resident", "tc_get_stats", "tc_tree_export",
+    "tc_morton_sort_device", "tc_device_alloc", "tc_device_free", "tc_copy_to_device", "tc_copy_to_host",
+]
+
+
+class TreecodeError(RuntimeError):
+    pass
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_P = ctypes.c_void_p
+
+
+def lib():
+    """Load libtreecodes2d.so (built by build_lib.build()); fail loudly if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TreecodeError("libtreecodes2d.so is not built (run __graft_entry__.build()); "
+                            "there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    dbl, it = ctypes.c_double, ctypes.c_int
+    L.treecode_potential.argtypes = [dbl, _D, _D, _D, it, _D, _D, it, _D]
+    L.treecode_potential.restype = None
+    L.treecode_force_mrag.argtypes = [dbl, _D, _D, _D, it, _D, _D, _D, it, _D, _D]
+    L.treecode_force_mrag.restype = None
+    L.treecode_last_error.restype = ctypes.c_char_p
+    L.tc_context_create.argtypes = [it]
+    L.tc_context_create.restype = _P
+    L.tc_context_destroy.argtypes = [_P]
+    L.tc_set_option.argtypes = [_P, it, it]
+    L.tc_context_stream.argtypes = [_P]
+    L.tc_context_stream.restype = _P
+    L.tc_potential.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P, it]
+    L.tc_force_mrag.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, _P, it, _P, _P, it]
+    L.tc_potential_resident.argtypes = [_P, dbl, _P, _P, it, _P, it]
+    L.tc_force_mrag_resident.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it]
+    L.tc_synchronize.argtypes = [_P]
+    L.tc_get_stats.argtypes = [_P, _D, ctypes.POINTER(ctypes.c_ulonglong), _I]
+    L.tc_tree_export.argtypes = [_P, _I, _I, _D, _D, _I, _D]
+    L.tc_morton_sort_device.argtypes = [_P, _P, _P, it, _P, _P, _P]
+    L.tc_device_alloc.argtypes = [_P, ctypes.c_size_t]
+    L.tc_device_alloc.restype = _P
+    L.tc_device_free.argtypes = [_P, _P]
+    L.tc_copy_to_device.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    L.tc_copy_to_host.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().treecode_last_error().decode()
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def treecode_potential(theta, xs, ys, vs, xt, yt):
+    """The reference's treecode_potential (drivers/treecode-potential.h:21-29) on host arrays."""
+    xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
+    out = np.empty(len(xt))
+    lib().treecode_potential(float(theta), _dp(xs), _dp(ys), _dp(vs), len(xs), _dp(xt), _dp(yt), len(xt), _dp(out))
+    if len(out) and np.isnan(out[0]) and last_error():
+        raise TreecodeError(last_error())
+    return out
+
+
+def treecode_force_mrag(theta, xs, ys, vs, x0s, y0s, hs):
+    """The reference's treecode_force_mrag (drivers/treecode-force.h:21-31) on host arrays;
+    returns (xresults, yresults) with point (ix, iy) of block b at ix + 32*(iy + 32*b)."""
+    xs, ys, vs, x0s, y0s, hs = map(_f64, (xs, ys, vs, x0s, y0s, hs))
+    nb = len(x0s)
+    xr = np.empty(nb * 1024)
+    yr = np.empty(nb * 1024)
+    lib().treecode_force_mrag(float(theta), _dp(xs), _dp(ys), _dp(vs), len(xs), _dp(x0s), _dp(y0s), _dp(hs), nb,
+                              _dp(xr), _dp(yr))
+    if nb and np.isnan(xr[0]) and last_error():
+        raise TreecodeError(last_error())
+    return xr, yr
+
+
+def _ptr(a):
+    """Pointer of a numpy array (host) or a DeviceArray (device)."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.ptr
+
+
+class DeviceArray:
+    """A 1-D buffer in the HBM of a context's GPU (tc_device_alloc)."""
+
+    def __init__(self, ctx, n, dtype):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = lib().tc_device_alloc(ctx.h, self.nbytes)
+        if not self.ptr:
+            raise TreecodeError(last_error())
+
+    def __len__(self):
+        return self.n
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        self.ctx._check(lib().tc_copy_to_device(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes))
+        return self
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        self.ctx._check(lib().tc_copy_to_host(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().tc_device_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP stream + resident tree + grow-only HBM buffers (tc_context)."""
+
+    def __init__(self, device=0, precision=FP64, pot_order=12, pot_leaf=512, force_order=24, force_leaf=192,
+                 profile=False, counters=False):
+        L = lib()
+        self.h = L.tc_context_create(device)
+        if not self.h:
+            raise TreecodeError(last_error())
+        self.set(OPT_PRECISION, precision)
+        self.set(OPT_POT_ORDER, pot_order)
+        self.set(OPT_POT_LEAF, pot_leaf)
+        self.set(OPT_FORCE_ORDER, force_order)
+        self.set(OPT_FORCE_LEAF, force_leaf)
+        self.set(OPT_PROFILE, int(profile))
+        self.set(OPT_COUNTERS, int(counters))
+
+    def set(self, key, value):
+        if lib().tc_set_option(self.h, key, int(value)) != 0:
+            raise TreecodeError(last_error())
+
+    def _check(self, rc):
+        if rc != 0:
+            raise TreecodeError(last_error())
+
+    @property
+    def stream(self):
+        return lib().tc_context_stream(self.h)
+
+    def potential(self, theta, xs, ys, vs, xt, yt, out=None, device=False):
+        if not device:
+            xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
+            out = np.empty(len(xt)) if out is None else out
+        self._check(lib().tc_potential(self.h, float(theta), _ptr(xs), _ptr(ys), _ptr(vs), len(xs), _ptr(xt),
+                                       _ptr(yt), len(xt), _ptr(out), int(device)))
+        return out
+
+    def potential_resident(self, theta, xt, yt, out=None, device=False):
+        if not device:
+            xt, yt = _f64(xt), _f64(yt)
+            out = np.empty(len(xt)) if out is None else out
+        self._check(lib().tc_potential_resident(self.h, float(theta), _ptr(xt), _ptr(yt), len(xt), _ptr(out),
+                                                int(device)))
+        return out
+
+    def force(self, theta, xs, ys, vs, x0s, y0s, hs, xr=None, yr=None, device=False):
+        if not device:
+            xs, ys, vs, x0s, y0s, hs = map(_f64, (xs, ys, vs, x0s, y0s, hs))
+            xr = np.empty(len(x0s) * 1024) if xr is None else xr
+            yr = np.empty(len(x0s) * 1024) if yr is None else yr
+        self._check(lib().tc_force_mrag(self.h, float(theta), _ptr(xs), _ptr(ys), _ptr(vs), len(xs), _ptr(x0s),
+                                        _ptr(y0s), _ptr(hs), len(x0s), _ptr(xr), _ptr(yr), int(device)))
+        return xr, yr
+
+    def force_resident(self, theta, x0s, y0s, hs, xr=None, yr=None, device=False):
+        if not device:
+            x0s, y0s, hs = map(_f64, (x0s, y0s, hs))
+            xr = np.empty(len(x0s) * 1024) if xr is None else xr
+            yr = np.empty(len(x0s) * 1024) if yr is None else yr
+        self._check(lib().tc_force_mrag_resident(self.h, float(theta), _ptr(x0s), _ptr(y0s), _ptr(hs), len(x0s),
+                                                 _ptr(xr), _ptr(yr), int(device)))
+        return xr, yr
+
+    def synchronize(self):
+        self._check(lib().tc_synchronize(self.h))
+
+    def stats(self):
+        t = np.zeros(4)
+        c = (ctypes.c_ulonglong * 4)()
+        info = np.zeros(4, np.int32)
+        self._check(lib().tc_get_stats(self.h, _dp(t), c, info.ctypes.data_as(_I)))
+        return {"times_ms": {"build": t[0], "tsort": t[1], "eval": t[2], "total": t[3]},
+                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3]},
+                "n": int(info[0]), "nnodes": int(info[1]), "ninternal": int(info[2]), "order": int(info[3])}
+
+    def tree(self):
+        st = self.stats()
+        n, nn, p = st["n"], st["nnodes"], st["order"]
+        keys = np.zeros(n, np.int32)
+        perm = np.zeros(n, np.int32)
+        box = np.zeros(3)
+        geom = np.zeros(4 * nn)
+        topo = np.zeros(4 * nn, np.int32)
+        exps = np.zeros(2 * p * nn)
+        self._check(lib().tc_tree_export(self.h, keys.ctypes.data_as(_I), perm.ctypes.data_as(_I), _dp(box),
+                                         _dp(geom), topo.ctypes.data_as(_I), _dp(exps)))
+        return {"keys": keys, "perm": perm, "box": box, "geom": geom.reshape(nn, 4), "topo": topo.reshape(nn, 4),
+                "exps": exps.reshape(nn, 2, p)}
+
+    def empty(self, n, dtype=np.float64):
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        return DeviceArray(self, a.size, dtype).upload(a)
+
+    def morton_sort_device(self, x, y, box, keys, perm):
+        self._check(lib().tc_morton_sort_device(self.h, _ptr(x), _ptr(y), len(x), _ptr(box), _ptr(keys),
+                                                _ptr(perm)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tc_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,533 @@
+// tc_api.cpp -- the C ABI: drop-in treecode_potential / treecode_force_mrag
+// (drivers/treecode-potential.cpp:89-106, drivers/treecode-force.cpp:201-236)
+// and the context-based extensions declared in include/treecodes2d.h.
+//
+// Unlike the reference (globals in upward.cpp:51-59, non-reentrant), all state
+// lives in a tc_context: one HIP stream, grow-only HBM buffers and the tree
+// that stays resident after a call.  The drop-in functions share one default
+// context behind a mutex.
+#include "tc_common.h"
+
+#include "../../include/treecodes2d.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+void tc_upload_binomials_eval(hipStream_t st);
+
+enum TcKind { TC_KIND_NONE = 0, TC_KIND_POTENTIAL = 1, TC_KIND_FORCE = 2 };
+
+struct tc_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = 0, pot_order = 12, pot_leaf = 32 * 16, force_order = 24, force_leaf = 192;
+    bool profile = false, count = false;
+    int tree_kind = TC_KIND_NONE;
+    int tree_precision = 0;
+    TcTree tree;
+    TcScratch ws;
+    DevBuf io, cnt;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ev_valid = false, eval_has_tsort = false;
+    unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0};
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mutex g_init_mu;
+bool g_binom_ready[64] = {false};
+std::mutex g_default_mu;
+tc_context *g_default = nullptr;
+
+void set_error(const std::string &m) { g_last_error = m; }
+
+template <class F> int guarded(tc_context *ctx, F &&f)
+{
+    if (!ctx) {
+        set_error("treecode: null context");
+        return -1;
+    }
+    try {
+        TC_HIP(hipSetDevice(ctx->device));
+        f();
+        return 0;
+    } catch (const TcError &e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_error(std::string("treecode: ") + e.what());
+        return -9;
+    }
+}
+
+void ensure_binomials(int device, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (device < 0 || device >= 64)
+        throw TcError(-1, "treecode: device index out of range");
+    if (!g_binom_ready[device]) {
+        tc_upload_binomials(st);
+        tc_upload_binomials_eval(st);
+        g_binom_ready[device] = true;
+    }
+}
+
+void copy_in(const double *h, double *d, int n, hipStream_t st)
+{
+    if (n > 0)
+        TC_HIP(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st));
+}
+
+unsigned long long *prepare_counters(tc_context *c)
+{
+    if (!c->count)
+        return nullptr;
+    c->cnt.reserve(sizeof(unsigned long long) * TC_NCOUNTERS);
+    tc_fill_u32(c->cnt.as<uint32_t>(), 0u, 2 * TC_NCOUNTERS, c->stream);
+    return c->cnt.as<unsigned long long>();
+}
+
+void record(tc_context *c, int i)
+{
+    if (c->profile)
+        TC_HIP(hipEventRecord(c->ev[i], c->stream));
+}
+
+void build(tc_context *c, const double *xs, const double *ys, const double *vs, int n, int kind)
+{
+    const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
+    const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
+    tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, c->ws, c->stream);
+    c->tree_kind = kind;
+    c->tree_precision = c->precision;
+}
+
+void check_resident(tc_context *c, int kind)
+{
+    const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
+    const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
+    if (c->tree_kind == TC_KIND_NONE || c->tree.n <= 0)
+        throw TcError(-5, "treecode: no resident tree in this context");
+    if (c->tree.order != order || c->tree.leafcap != leaf || (c->precision == 1 && !c->tree.has_fp32))
+        throw TcError(-5, "treecode: resident tree was built with other order/leaf/precision");
+}
+
+void eval_potential(tc_context *c, double theta, const double *xt, const double *yt, int m, double *out,
+                    unsigned long long *cnt)
+{
+    tc_eval_potential(c->tree, theta, xt, yt, m, out, c->precision, cnt, c->ws, c->stream,
+                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+}
+
+int run_potential(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
+                  const double *xt, const double *yt, int m, double *out, int device_ptrs, bool rebuild)
+{
+    return guarded(ctx, [&] {
+        tc_context *c = ctx;
+        c->ev_valid = false;
+        if (m <= 0)
+            return;
+        if (rebuild && n <= 0) {
+            if (device_ptrs)
+                TC_HIP(hipMemsetAsync(out, 0, sizeof(double) * (size_t)m, c->stream));
+            else
+                std::memset(out, 0, sizeof(double) * (size_t)m);
+            c->tree_kind = TC_KIND_NONE;
+            return;
+        }
+        if (!rebuild)
+            check_resident(c, TC_KIND_POTENTIAL);
+        const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
+        double *dout = out;
+        if (!device_ptrs) {
+            for (int i = 0; i < m; ++i)
+                if (!std::isfinite(xt[i]) || !std::isfinite(yt[i]))
+                    throw TcError(-4, "treecode_potential: non-finite target coordinate");
+            const size_t ns = rebuild ? (size_t)n : 0;
+            c->io.reserve(sizeof(double) * (3 * ns + 3 * (size_t)m));
+            double *base = c->io.as<double>();
+            double *bxs = base, *bys = base + ns, *bvs = base + 2 * ns;
+            double *bxt = base + 3 * ns, *byt = bxt + m;
+            dout = byt + m;
+            if (rebuild) {
+                copy_in(xs, bxs, n, c->stream);
+                copy_in(ys, bys, n, c->stream);
+                copy_in(vs, bvs, n, c->stream);
+            }
+            copy_in(xt, bxt, m, c->stream);
+            copy_in(yt, byt, m, c->stream);
+            dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
+        }
+        unsigned long long *cnt = prepare_counters(c);
+        record(c, 0);
+        if (rebuild)
+            build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL);
+        record(c, 1);
+        eval_potential(c, theta, dxt, dyt, m, dout, cnt);
+        c->ev_valid = c->profile;
+        c->eval_has_tsort = true;
+        if (!device_ptrs) {
+            TC_HIP(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
+int run_force(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
+              const double *x0s, const double *y0s, const double *hs, int nb, double *xr, double *yr,
+              int device_ptrs, bool rebuild)
+{
+    return guarded(ctx, [&] {
+        tc_context *c = ctx;
+        c->ev_valid = false;
+        if (nb <= 0)
+            return;
+        const size_t nt = (size_t)nb * 1024;
+        if (rebuild && n <= 0) {
+            if (device_ptrs) {
+                TC_HIP(hipMemsetAsync(xr, 0, sizeof(double) * nt, c->stream));
+                TC_HIP(hipMemsetAsync(yr, 0, sizeof(double) * nt, c->stream));
+            } else {
+                std::memset(xr, 0, sizeof(double) * nt);
+                std::memset(yr, 0, sizeof(double) * nt);
+            }
+            c->tree_kind = TC_KIND_NONE;
+            return;
+        }
+        if (!rebuild)
+            check_resident(c, TC_KIND_FORCE);
+        const double *dxs = xs, *dys = ys, *dvs = vs, *dx0 = x0s, *dy0 = y0s, *dh = hs;
+        double *dxr = xr, *dyr = yr;
+        if (!device_ptrs) {
+            for (int i = 0; i < nb; ++i)
+                if (!std::isfinite(x0s[i]) || !std::isfinite(y0s[i]) || !std::isfinite(hs[i]))
+                    throw TcError(-4, "treecode_force_mrag: non-finite block origin or spacing");
+            const size_t ns = rebuild ? (size_t)n : 0;
+            c->io.reserve(sizeof(double) * (3 * ns + 3 * (size_t)nb + 2 * nt));
+            double *base = c->io.as<double>();
+            double *bxs = base, *bys = base + ns, *bvs = base + 2 * ns;
+            double *bx0 = base + 3 * ns, *by0 = bx0 + nb, *bh = by0 + nb;
+            dxr = bh + nb;
+            dyr = dxr + nt;
+            if (rebuild) {
+                copy_in(xs, bxs, n, c->stream);
+                copy_in(ys, bys, n, c->stream);
+                copy_in(vs, bvs, n, c->stream);
+            }
+            copy_in(x0s, bx0, nb, c->stream);
+            copy_in(y0s, by0, nb, c->stream);
+            copy_in(hs, bh, nb, c->stream);
+            dxs = bxs; dys = bys; dvs = bvs; dx0 = bx0; dy0 = by0; dh = bh;
+        }
+        unsigned long long *cnt = prepare_counters(c);
+        record(c, 0);
+        if (rebuild)
+            build(c, dxs, dys, dvs, n, TC_KIND_FORCE);
+        record(c, 1);
+        tc_eval_force(c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->stream,
+                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+        c->ev_valid = c->profile;
+        c->eval_has_tsort = false;
+        if (!device_ptrs) {
+            TC_HIP(hipMemcpyAsync(xr, dxr, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipMemcpyAsync(yr, dyr, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
+tc_context *default_context()
+{
+    if (!g_default)
+        g_default = tc_context_create(0);
+    return g_default;
+}
+
+void report_dropin_failure(const char *fn)
+{
+    static bool printed = false;
+    if (!printed) {
+        std::fprintf(stderr, "%s: %s\n", fn, g_last_error.c_str());
+        printed = true;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+TC_API const char *treecode_last_error(void) { return g_last_error.c_str(); }
+
+TC_API tc_context *tc_context_create(int device)
+{
+    tc_context *c = new tc_context();
+    c->device = device;
+    try {
+        int ndev = 0;
+        TC_HIP(hipGetDeviceCount(&ndev));
+        if (ndev <= 0)
+            throw TcError(-1, "treecode: no HIP device available (this library has no CPU fallback)");
+        if (device < 0 || device >= ndev)
+            throw TcError(-1, "treecode: device " + std::to_string(device) + " out of range");
+        TC_HIP(hipSetDevice(device));
+        TC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev)
+            TC_HIP(hipEventCreate(&e));
+        ensure_binomials(device, c->stream);
+    } catch (const TcError &e) {
+        set_error(e.what());
+        tc_context_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+TC_API void tc_context_destroy(tc_context *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    c->tree.release();
+    c->ws.release();
+    c->io.release();
+    c->cnt.release();
+    for (auto &e : c->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+TC_API void *tc_context_stream(tc_context *c) { return c ? (void *)c->stream : nullptr; }
+
+TC_API int tc_set_option(tc_context *c, int key, int value)
+{
+    if (!c) {
+        set_error("treecode: null context");
+        return -1;
+    }
+    auto valid_order = [](int p) { return p == 8 || p == 12 || p == 16 || p == 20 || p == 24 || p == 32; };
+    switch (key) {
+    case TC_OPT_PRECISION:
+        if (value != 0 && value != 1) break;
+        c->precision = value;
+        return 0;
+    case TC_OPT_POT_ORDER:
+        if (!valid_order(value)) break;
+        c->pot_order = value;
+        return 0;
+    case TC_OPT_POT_LEAF:
+        if (value < 1) break;
+        c->pot_leaf = value;
+        return 0;
+    case TC_OPT_FORCE_ORDER:
+        if (!valid_order(value)) break;
+        c->force_order = value;
+        return 0;
+    case TC_OPT_FORCE_LEAF:
+        if (value < 1) break;
+        c->force_leaf = value;
+        return 0;
+    case TC_OPT_PROFILE:
+        c->profile = value != 0;
+        return 0;
+    case TC_OPT_COUNTERS:
+        c->count = value != 0;
+        return 0;
+    default:
+        break;
+    }
+    set_error("treecode: invalid option " + std::to_string(key) + " = " + std::to_string(value));
+    return -3;
+}
+
+TC_API int tc_potential(tc_context *c, double theta, const double *xs, const double *ys, const double *vs, int n,
+                        const double *xt, const double *yt, int m, double *out, int device_ptrs)
+{
+    return run_potential(c, theta, xs, ys, vs, n, xt, yt, m, out, device_ptrs, true);
+}
+
+TC_API int tc_potential_resident(tc_context *c, double theta, const double *xt, const double *yt, int m,
+                                 double *out, int device_ptrs)
+{
+    return run_potential(c, theta, nullptr, nullptr, nullptr, 0, xt, yt, m, out, device_ptrs, false);
+}
+
+TC_API int tc_force_mrag(tc_context *c, double theta, const double *xs, const double *ys, const double *vs,
+                         int n, const double *x0s, const double *y0s, const double *hs, int nblocks, double *xr,
+                         double *yr, int device_ptrs)
+{
+    return run_force(c, theta, xs, ys, vs, n, x0s, y0s, hs, nblocks, xr, yr, device_ptrs, true);
+}
+
+TC_API int tc_force_mrag_resident(tc_context *c, double theta, const double *x0s, const double *y0s,
+                                  const double *hs, int nblocks, double *xr, double *yr, int device_ptrs)
+{
+    return run_force(c, theta, nullptr, nullptr, nullptr, 0, x0s, y0s, hs, nblocks, xr, yr, device_ptrs, false);
+}
+
+TC_API int tc_synchronize(tc_context *c)
+{
+    return guarded(c, [&] { TC_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+TC_API int tc_get_stats(tc_context *c, double *times_ms, unsigned long long *counters, int *tree_info)
+{
+    return guarded(c, [&] {
+        TC_HIP(hipStreamSynchronize(c->stream));
+        if (times_ms) {
+            for (int i = 0; i < TC_NPHASES; ++i)
+                times_ms[i] = 0;
+            if (c->ev_valid) {
+                float a = 0, b = 0, d = 0, t = 0;
+                TC_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+                TC_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+                TC_HIP(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+                TC_HIP(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+                times_ms[TC_PH_BUILD] = a;
+                times_ms[TC_PH_TSORT] = b;
+                times_ms[TC_PH_EVAL] = d;
+                times_ms[TC_PH_TOTAL] = t;
+            }
+        }
+        if (counters) {
+            if (c->count && c->cnt.ptr)
+                TC_HIP(hipMemcpy(counters, c->cnt.ptr, sizeof(unsigned long long) * TC_NCOUNTERS,
+                                 hipMemcpyDeviceToHost));
+            else
+                for (int i = 0; i < TC_NCOUNTERS; ++i)
+                    counters[i] = 0;
+        }
+        if (tree_info) {
+            tree_info[0] = c->tree.n;
+            tree_info[1] = c->tree.nnodes;
+            tree_info[2] = c->tree.ninternal;
+            tree_info[3] = c->tree.order;
+        }
+    });
+}
+
+TC_API int tc_tree_export(tc_context *c, int *keys, int *perm, double *box, double *geom, int *topo, double *exps)
+{
+    return guarded(c, [&] {
+        if (c->tree_kind == TC_KIND_NONE)
+            throw TcError(-5, "treecode: no resident tree in this context");
+        TC_HIP(hipStreamSynchronize(c->stream));
+        const TcTree &t = c->tree;
+        if (keys)
+            TC_HIP(hipMemcpy(keys, t.keys.ptr, sizeof(int) * (size_t)t.n, hipMemcpyDeviceToHost));
+        if (perm)
+            TC_HIP(hipMemcpy(perm, t.perm.ptr, sizeof(int) * (size_t)t.n, hipMemcpyDeviceToHost));
+        if (box)
+            for (int i = 0; i < 3; ++i)
+                box[i] = t.box[i];
+        if (geom || topo) {
+            std::vector<TcNode> nodes((size_t)t.nnodes);
+            TC_HIP(hipMemcpy(nodes.data(), t.nodes.ptr, sizeof(TcNode) * nodes.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < nodes.size(); ++i) {
+                if (geom) {
+                    geom[4 * i + 0] = nodes[i].xcom;
+                    geom[4 * i + 1] = nodes[i].ycom;
+                    geom[4 * i + 2] = nodes[i].r;
+                    geom[4 * i + 3] = nodes[i].mass;
+                }
+                if (topo) {
+                    topo[4 * i + 0] = nodes[i].s;
+                    topo[4 * i + 1] = nodes[i].e;
+                    topo[4 * i + 2] = nodes[i].child;
+                    topo[4 * i + 3] = nodes[i].level;
+                }
+            }
+        }
+        if (exps)
+            TC_HIP(hipMemcpy(exps, t.exps.ptr, sizeof(double) * 2 * (size_t)t.order * t.nnodes,
+                             hipMemcpyDeviceToHost));
+    });
+}
+
+TC_API void *tc_device_alloc(tc_context *c, size_t bytes)
+{
+    void *p = nullptr;
+    const int rc = guarded(c, [&] { TC_HIP(hipMalloc(&p, bytes ? bytes : 1)); });
+    return rc ? nullptr : p;
+}
+
+TC_API int tc_device_free(tc_context *c, void *p)
+{
+    return guarded(c, [&] {
+        TC_HIP(hipStreamSynchronize(c->stream));
+        TC_HIP(hipFree(p));
+    });
+}
+
+TC_API int tc_copy_to_device(tc_context *c, void *dst, const void *src, size_t bytes)
+{
+    return guarded(c, [&] {
+        TC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        TC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+TC_API int tc_copy_to_host(tc_context *c, void *dst, const void *src, size_t bytes)
+{
+    return guarded(c, [&] {
+        TC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        TC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+TC_API int tc_morton_sort_device(tc_context *c, const double *x, const double *y, int n, double *box,
+                                 unsigned int *keys, unsigned int *perm)
+{
+    return guarded(c, [&] {
+        if (n <= 0)
+            throw TcError(-3, "tc_morton_sort_device: n must be positive");
+        tc_bbox(x, y, n, box, c->ws, c->stream);
+        tc_morton_sort(x, y, n, box, keys, perm, c->ws, c->stream);
+    });
+}
+
+TC_API void treecode_potential(const double theta, const double *const xsources, const double *const ysources,
+                               const double *const sourcevalues, const int nsources,
+                               const double *const xtargets, const double *const ytargets, const int ntargets,
+                               double *const targetvalues)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    tc_context *c = default_context();
+    int rc = c ? tc_potential(c, theta, xsources, ysources, sourcevalues, nsources, xtargets, ytargets, ntargets,
+                              targetvalues, 0)
+               : -1;
+    if (rc) {
+        report_dropin_failure("treecode_potential");
+        for (int i = 0; i < ntargets; ++i)
+            targetvalues[i] = std::numeric_limits<double>::quiet_NaN();
+    }
+}
+
+TC_API void treecode_force_mrag(const double theta, const double *const xsources, const double *const ysources,
+                                const double *const sourcevalues, const int nsources, const double *const x0s,
+                                const double *const y0s, const double *const hs, const int nblocks,
+                                double *const xresults, double *const yresults)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    tc_context *c = default_context();
+    int rc = c ? tc_force_mrag(c, theta, xsources, ysources, sourcevalues, nsources, x0s, y0s, hs, nblocks,
+                               xresults, yresults, 0)
+               : -1;
+    if (rc) {
+        report_dropin_failure("treecode_force_mrag");
+        for (size_t i = 0; i < (size_t)(nblocks > 0 ? nblocks : 0) * 1024; ++i)
+            xresults[i] = yresults[i] = std::numeric_limits<double>::quiet_NaN();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,1019 @@
+// tc_build.hip -- Tree::build on gfx950: bbox, Morton keys, LDS radix sort,
+// quadtree topology from sorted keys, upward pass (P2E leaves, E2E parents).
+//
+// Replaces drivers/sort-sources.cpp:22-132, drivers/upward.cpp:61-314,
+// kernels/upward-common.ispc:13-150 and kernels/upward-kernels.m4:18-149.
+// Compiled with -ffp-contract=off: keys, centres of mass and radii must be
+// rounded exactly as the canonical CPU restatement rounds them, because every
+// traversal decision depends on them.
+#include "tc_common.h"
+
+#include <cfloat>
+#include <vector>
+
+namespace {
+
+constexpr int RS_NT = 256;     // radix sort block size (4 waves)
+constexpr int RS_ITEMS = 16;   // keys per thread per tile
+constexpr int RS_TILE = RS_NT * RS_ITEMS;
+constexpr int RS_BINS = 256;   // 8-bit digits: passes at bits 0, 8, 16, 24 (30-bit keys)
+constexpr int SCAN_NT = 1024;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_CHUNK = SCAN_NT * SCAN_ITEMS;
+constexpr int TB_CHUNK = 1024;  // positions per wave in the internal-cell census
+
+__constant__ double c_binom[64 * 64];  // C(n, k) rounded to double, n, k < 64
+
+// ------------------------------------------------------------------ scans
+template <int NT>
+TC_DEV uint32_t block_scan_excl(uint32_t v, uint32_t *s_w, uint32_t *total)
+{
+    constexpr int NW = NT / 64;
+    const int lane = tc_lane(), wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == 63)
+        s_w[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t w = lane < NW ? s_w[lane] : 0u;
+        uint32_t z = w;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(z, off, 64);
+            if (lane >= off)
+                z += y;
+        }
+        if (lane < NW)
+            s_w[lane] = z - w;
+        if (lane == NW - 1)
+            s_w[NW] = z;
+    }
+    __syncthreads();
+    const uint32_t res = s_w[wid] + x - v;
+    if (total)
+        *total = s_w[NW];
+    __syncthreads();
+    return res;
+}
+
+// Single-block exclusive scan of n items with a running carry.
+__global__ __launch_bounds__(SCAN_NT) void k_scan_single(const uint32_t *in, uint32_t *out, int n, uint32_t *total)
+{
+    __shared__ uint32_t s_w[SCAN_NT / 64 + 1];
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += SCAN_CHUNK) {
+        uint32_t v[SCAN_ITEMS];
+        uint32_t sum = 0;
+        const int i0 = base + threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            v[j] = (i0 + j < n) ? in[i0 + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint32_t pre = block_scan_excl<SCAN_NT>(sum, s_w, &tot) + carry;
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            if (i0 + j < n)
+                out[i0 + j] = pre;
+            pre += v[j];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total)
+        *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_partial(const uint32_t *in, int n, uint32_t *part)
+{
+    __shared__ uint32_t s_w[SCAN_NT / 64 + 1];
+    const int i0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+        sum += (i0 + j < n) ? in[i0 + j] : 0u;
+    uint32_t tot;
+    block_scan_excl<SCAN_NT>(sum, s_w, &tot);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_final(const uint32_t *in, uint32_t *out, int n,
+                                                         const uint32_t *part)
+{
+    __shared__ uint32_t s_w[SCAN_NT / 64 + 1];
+    const int i0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        v[j] = (i0 + j < n) ? in[i0 + j] : 0u;
+        sum += v[j];
+    }
+    uint32_t pre = block_scan_excl<SCAN_NT>(sum, s_w, nullptr) + part[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        if (i0 + j < n)
+            out[i0 + j] = pre;
+        pre += v[j];
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ------------------------------------------------------------------ bbox
+// sort-sources.cpp:44-82 with upward-common.ispc:126-150's accumulators:
+// min starts at 1e13, max at 0.
+__global__ __launch_bounds__(256) void k_bbox_partial(const double *x, const double *y, int n, double4 *part,
+                                                       uint32_t *bad)
+{
+    __shared__ double4 s[4];
+    double xmi = 1e13, ymi = 1e13, xma = 0, yma = 0;
+    bool nonfinite = false;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double a = x[i], b = y[i];
+        nonfinite |= !isfinite(a) || !isfinite(b);
+        xmi = a < xmi ? a : xmi;
+        xma = xma < a ? a : xma;
+        ymi = b < ymi ? b : ymi;
+        yma = yma < b ? b : yma;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o;
+        o = __shfl_xor(xmi, off, 64); xmi = o < xmi ? o : xmi;
+        o = __shfl_xor(ymi, off, 64); ymi = o < ymi ? o : ymi;
+        o = __shfl_xor(xma, off, 64); xma = xma < o ? o : xma;
+        o = __shfl_xor(yma, off, 64); yma = yma < o ? o : yma;
+    }
+    if (tc_ballot(nonfinite) && tc_lane() == 0)
+        atomicOr(bad, 1u);
+    const int wid = threadIdx.x >> 6;
+    if (tc_lane() == 0)
+        s[wid] = make_double4(xmi, xma, ymi, yma);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double4 r = s[0];
+        for (int w = 1; w < 4; ++w) {
+            r.x = s[w].x < r.x ? s[w].x : r.x;
+            r.y = r.y < s[w].y ? s[w].y : r.y;
+            r.z = s[w].z < r.z ? s[w].z : r.z;
+            r.w = r.w < s[w].w ? s[w].w : r.w;
+        }
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_bbox_final(const double4 *part, int np, double *box)
+{
+    double xmi = 1e13, xma = 0, ymi = 1e13, yma = 0;
+    for (int i = threadIdx.x; i < np; i += 64) {
+        const double4 r = part[i];
+        xmi = r.x < xmi ? r.x : xmi;
+        xma = xma < r.y ? r.y : xma;
+        ymi = r.z < ymi ? r.z : ymi;
+        yma = yma < r.w ? r.w : yma;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o;
+        o = __shfl_xor(xmi, off, 64); xmi = o < xmi ? o : xmi;
+        o = __shfl_xor(ymi, off, 64); ymi = o < ymi ? o : ymi;
+        o = __shfl_xor(xma, off, 64); xma = xma < o ? o : xma;
+        o = __shfl_xor(yma, off, 64); yma = yma < o ? o : yma;
+    }
+    if (threadIdx.x == 0) {
+        const double ext0 = xma - xmi, ext1 = yma - ymi;
+        const double eps = 10000 * DBL_EPSILON;
+        double ext = (ext0 < ext1 ? ext1 : ext0) * (1 + 2 * eps);
+        if (!(ext > 0))
+            ext = 1.0;
+        box[0] = xmi - eps * ext;
+        box[1] = ymi - eps * ext;
+        box[2] = ext;
+        box[3] = (1.0 / ext) * (double)(1 << TC_LMAX);
+    }
+}
+
+// ------------------------------------------------------------------ keys
+TC_DEV uint32_t quantise(double v, double vmin, double scale)
+{
+    double q = floor((v - vmin) * scale);
+    if (!(q >= 0))
+        q = 0;
+    if (q > (double)((1 << TC_LMAX) - 1))
+        q = (double)((1 << TC_LMAX) - 1);
+    return (uint32_t)q;
+}
+
+// sort-sources.cpp:87-110 (quantise + interleave, x in even bits)
+__global__ __launch_bounds__(256) void k_keys(const double *x, const double *y, int n, const double *box,
+                                               uint32_t *keys)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const double xmin = box[0], ymin = box[1], scale = box[3];
+    const uint32_t qx = quantise(x[i], xmin, scale), qy = quantise(y[i], ymin, scale);
+    keys[i] = tc_spread15(qx) | (tc_spread15(qy) << 1);
+}
+
+// ------------------------------------------------------------------ radix sort
+// Stable LSD radix sort of (key, index) pairs, 8-bit digits.  Per pass:
+// k_rs_hist (tile histograms, digit-major), one exclusive scan over the
+// digit-major table (= global output offset of each (digit, tile)), and
+// k_rs_scatter, which ranks a tile stably in LDS with wavefront ballots and
+// writes digit runs contiguously.
+__global__ __launch_bounds__(RS_NT) void k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *hist, int ntiles)
+{
+    __shared__ uint32_t h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int i = base + j * RS_NT + threadIdx.x;
+        if (i < n)
+            atomicAdd(&h[(keys[i] >> shift) & (RS_BINS - 1)], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                       uint32_t *vout, int n, int shift, const uint32_t *offs,
+                                                       int ntiles)
+{
+    __shared__ uint32_t s_k[RS_TILE];
+    __shared__ uint32_t s_v[RS_TILE];
+    __shared__ uint32_t s_tileoff[RS_BINS];
+    __shared__ uint32_t s_gbase[RS_BINS];
+    __shared__ uint32_t s_run[RS_BINS];
+    __shared__ uint32_t s_wcnt[RS_NT / 64][RS_BINS];
+    __shared__ uint32_t s_w[RS_NT / 64 + 1];
+
+    const int tid = threadIdx.x, lane = tc_lane(), wid = tid >> 6;
+    const int base = blockIdx.x * RS_TILE;
+    const int cnt = min(RS_TILE, n - base);
+
+    s_run[tid] = 0;
+    for (int w = 0; w < RS_NT / 64; ++w)
+        s_wcnt[w][tid] = 0;
+    s_gbase[tid] = offs[tid * ntiles + blockIdx.x];
+    __syncthreads();
+
+    uint32_t k[RS_ITEMS], v[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int i = base + j * RS_NT + tid;
+        const bool ok = i < n;
+        k[j] = ok ? kin[i] : 0u;
+        v[j] = ok ? (vin ? vin[i] : (uint32_t)i) : 0u;
+        if (ok)
+            atomicAdd(&s_run[(k[j] >> shift) & (RS_BINS - 1)], 1u);
+    }
+    __syncthreads();
+    {
+        const uint32_t c = s_run[tid];
+        s_tileoff[tid] = block_scan_excl<RS_NT>(c, s_w, nullptr);
+        s_run[tid] = 0;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const bool ok = base + j * RS_NT + tid < n;
+        const uint32_t d = (k[j] >> shift) & (RS_BINS - 1);
+        unsigned long long peers = tc_ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bb = tc_ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t lower = __popcll(peers & tc_lanemask_lt());
+        const bool leader = ok && lower == 0;
+        if (leader)
+            s_wcnt[wid][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (ok) {
+            uint32_t pre = s_run[d];
+            for (int w = 0; w < wid; ++w)
+                pre += s_wcnt[w][d];
+            const uint32_t pos = s_tileoff[d] + pre + lower;
+            s_k[pos] = k[j];
+            s_v[pos] = v[j];
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+            for (int w = 0; w < RS_NT / 64; ++w) {
+                add += s_wcnt[w][tid];
+                s_wcnt[w][tid] = 0;
+            }
+            s_run[tid] += add;
+        }
+        __syncthreads();
+    }
+
+    for (int p = tid; p < cnt; p += RS_NT) {
+        const uint32_t kk = s_k[p];
+        const uint32_t d = (kk >> shift) & (RS_BINS - 1);
+        const uint32_t dst = s_gbase[d] + (uint32_t)p - s_tileoff[d];
+        kout[dst] = kk;
+        vout[dst] = s_v[p];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_src(const double *x, const double *y, const double *v,
+                                                     const uint32_t *perm, int n, double4 *src)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t j = perm[i];
+    src[i] = make_double4(x[j], y[j], v[j], 0.0);
+}
+
+// ------------------------------------------------------------------ topology
+// Internal cells are exactly the cells of levels 0..14 holding more than
+// `cap` sources (their ancestors then hold more too), i.e. the breadth-first
+// build_tree recursion of upward.cpp:223-277.  A level-L cell starting at
+// sorted position i is internal iff key[i-1] leaves the cell and key[i+cap]
+// stays in it, so every internal cell is found from three key reads, and its
+// breadth-first rank is a scan over (level, position).
+TC_DEV void cell_range(const uint32_t *keys, int n, int cap, int i, int &lo, int &hi)
+{
+    lo = 1;
+    hi = 0;
+    if (i >= n)
+        return;
+    const uint32_t k = keys[i];
+    lo = (i == 0) ? 0 : tc_common_level(keys[i - 1], k) + 1;
+    hi = (i + cap < n) ? min(TC_LMAX - 1, tc_common_level(k, keys[i + cap])) : -1;
+}
+
+__global__ __launch_bounds__(256) void k_int_count(const uint32_t *keys, int n, int cap, uint32_t *cnt_tab,
+                                                    int nblk)
+{
+    __shared__ uint32_t s_cnt[4][TC_LMAX];
+    const int lane = tc_lane(), wid = threadIdx.x >> 6;
+    const int base = (blockIdx.x * 4 + wid) * TB_CHUNK;
+    uint32_t c[TC_LMAX];
+#pragma unroll
+    for (int L = 0; L < TC_LMAX; ++L)
+        c[L] = 0;
+    for (int j = 0; j < TB_CHUNK / 64; ++j) {
+        int lo, hi;
+        cell_range(keys, n, cap, base + j * 64 + lane, lo, hi);
+        if (!tc_ballot(hi >= lo))
+            continue;
+#pragma unroll
+        for (int L = 0; L < TC_LMAX; ++L)
+            c[L] += __popcll(tc_ballot(L >= lo && L <= hi));
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int L = 0; L < TC_LMAX; ++L)
+            s_cnt[wid][L] = c[L];
+    __syncthreads();
+    if (threadIdx.x < TC_LMAX)
+        cnt_tab[threadIdx.x * nblk + blockIdx.x] =
+            s_cnt[0][threadIdx.x] + s_cnt[1][threadIdx.x] + s_cnt[2][threadIdx.x] + s_cnt[3][threadIdx.x];
+}
+
+__global__ void k_level_starts(const uint32_t *offs, int nblk, const uint32_t *total, const uint32_t *bad,
+                               int *info)
+{
+    const int L = threadIdx.x;
+    if (L < TC_LMAX)
+        info[L] = (int)offs[L * nblk];
+    if (L == TC_LMAX)
+        info[TC_LMAX] = (int)*total;
+    if (L == TC_LMAX + 1)
+        info[TC_LMAX + 1] = (int)*bad;
+}
+
+__global__ __launch_bounds__(256) void k_int_emit(const uint32_t *keys, int n, int cap, const uint32_t *offs,
+                                                   int nblk, int *istart, int *ilevel)
+{
+    __shared__ uint32_t s_cnt[4][TC_LMAX];
+    const int lane = tc_lane(), wid = threadIdx.x >> 6;
+    const int base = (blockIdx.x * 4 + wid) * TB_CHUNK;
+    uint32_t c[TC_LMAX];
+#pragma unroll
+    for (int L = 0; L < TC_LMAX; ++L)
+        c[L] = 0;
+    for (int j = 0; j < TB_CHUNK / 64; ++j) {
+        int lo, hi;
+        cell_range(keys, n, cap, base + j * 64 + lane, lo, hi);
+        if (!tc_ballot(hi >= lo))
+            continue;
+#pragma unroll
+        for (int L = 0; L < TC_LMAX; ++L)
+            c[L] += __popcll(tc_ballot(L >= lo && L <= hi));
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int L = 0; L < TC_LMAX; ++L)
+            s_cnt[wid][L] = c[L];
+    __syncthreads();
+#pragma unroll
+    for (int L = 0; L < TC_LMAX; ++L) {
+        uint32_t o = offs[L * nblk + blockIdx.x];
+        for (int w = 0; w < wid; ++w)
+            o += s_cnt[w][L];
+        c[L] = o;
+    }
+    for (int j = 0; j < TB_CHUNK / 64; ++j) {
+        const int i = base + j * 64 + lane;
+        int lo, hi;
+        cell_range(keys, n, cap, i, lo, hi);
+        if (!tc_ballot(hi >= lo))
+            continue;
+#pragma unroll
+        for (int L = 0; L < TC_LMAX; ++L) {
+            const bool f = L >= lo && L <= hi;
+            const unsigned long long b = tc_ballot(f);
+            if (f) {
+                const uint32_t r = c[L] + __popcll(b & tc_lanemask_lt());
+                istart[r] = i;
+                ilevel[r] = L;
+            }
+            c[L] += __popcll(b);
+        }
+    }
+}
+
+__global__ void k_root_init(int n, int2 *se, int *level, int2 *cell)
+{
+    se[0] = make_int2(0, n);
+    level[0] = 0;
+    cell[0] = make_int2(0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_int_nodes(const uint32_t *keys, int n, int cap, const int *istart,
+                                                    const int *ilevel, const int *info, int T, int *selfid,
+                                                    uint32_t *child, int2 *se, int *level, int2 *cell)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= T)
+        return;
+    const int L = ilevel[r], i = istart[r];
+    const uint32_t key = keys[i];
+    const int sh = 2 * (TC_LMAX - L);
+    const uint32_t prefix = sh >= 32 ? 0u : (key >> sh);
+    int self = 0;
+    if (L > 0) {
+        int lo = info[L - 1], hi = info[L];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (istart[mid] <= i)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        self = 1 + 4 * (lo - 1) + (int)(prefix & 3u);
+    }
+    selfid[r] = self;
+    child[self] = (uint32_t)(1 + 4 * r);
+
+    // end of the cell: first position whose level-L prefix differs
+    int e;
+    if (L == 0) {
+        e = n;
+    } else {
+        int lo = i + cap + 1, hi = n, step = 1;
+        while (true) {
+            const int probe = lo + step - 1;
+            if (probe >= n) {
+                hi = n;
+                break;
+            }
+            if ((keys[probe] >> sh) != prefix) {
+                hi = probe;
+                break;
+            }
+            lo = probe + 1;
+            step <<= 1;
+        }
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((keys[mid] >> sh) == prefix)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        e = lo;
+    }
+    const int csh = sh - 2;
+    int b[5];
+    b[0] = i;
+    b[4] = e;
+    for (int c = 1; c < 4; ++c) {
+        const uint32_t target = ((prefix << 2) | (uint32_t)c) << csh;
+        int lo = b[c - 1], hi = e;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (keys[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        b[c] = lo;
+    }
+    const int cx = (int)tc_compact15(prefix), cy = (int)tc_compact15(prefix >> 1);
+    for (int c = 0; c < 4; ++c) {
+        const int k = 1 + 4 * r + c;
+        se[k] = make_int2(b[c], b[c + 1]);
+        level[k] = L + 1;
+        cell[k] = make_int2(2 * cx + (c & 1), 2 * cy + (c >> 1));
+    }
+}
+
+// ------------------------------------------------------------------ upward
+// One wavefront per leaf: build_leaf (upward.cpp:61-93) = upward_setup
+// (upward-common.ispc:13-57) + upward_p2e (upward-kernels.m4:52-92), in the
+// canonical 64-lane strided order with the pairwise lane tree.
+template <int P>
+__global__ __launch_bounds__(256) void k_leaf_upward(const uint32_t *child, const int2 *se, const int *level,
+                                                      const int2 *cell, const double4 *src, const double *box,
+                                                      int nnodes, TcNode *nodes, double *exps, double4 *nodew,
+                                                      float4 *src32)
+{
+    const int id = tc_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (id >= nnodes || child[id] != 0)
+        return;
+    const int lane = tc_lane();
+    const int2 r = se[id];
+    const int s = r.x, e = r.y, l = level[id];
+    const int2 cc = cell[id];
+    const double xmin = box[0], ymin = box[1], ext = box[2];
+    const double h = ext / (double)(1 << l);
+    const double x0 = xmin + h * (double)cc.x, y0 = ymin + h * (double)cc.y;
+
+    double pm = 0, pw = 0, px = 0, py = 0;
+    for (int i = s + lane; i < e; i += 64) {
+        const double4 q = src[i];
+        const double a = fabs(q.z);
+        pm += q.z;
+        pw += a;
+        px += q.x * a;
+        py += q.y * a;
+    }
+    const double m = tc_wave_tree_sum(pm), w = tc_wave_tree_sum(pw);
+    const double wx = tc_wave_tree_sum(px), wy = tc_wave_tree_sum(py);
+    const double xc = w != 0 ? wx / w : (x0 + 0.5 * h);
+    const double yc = w != 0 ? wy / w : (y0 + 0.5 * h);
+
+    double r2 = 0;
+    for (int i = s + lane; i < e; i += 64) {
+        const double4 q = src[i];
+        const double xr = q.x - xc, yr = q.y - yc;
+        const double d = xr * xr + yr * yr;
+        r2 = r2 < d ? d : r2;
+    }
+    r2 = tc_wave_max(r2);
+    const double rad = sqrt(r2);
+
+    double re[P], im[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        re[k] = im[k] = 0;
+    for (int i = s + lane; i < e; i += 64) {
+        const double4 q = src[i];
+        const double r0 = q.x - xc, i0 = q.y - yc, v = q.z;
+        double rtmp = r0 * v, itmp = i0 * v;
+        re[0] -= rtmp;
+        im[0] -= itmp;
+        double rprod = r0, iprod = i0;
+#pragma unroll
+        for (int k = 1; k < P; ++k) {
+            rtmp = rprod * r0 - iprod * i0;
+            itmp = rprod * i0 + iprod * r0;
+            const double term = v * (1.0 / (double)(k + 1));
+            rprod = rtmp;
+            iprod = itmp;
+            rtmp = rprod * term;
+            itmp = iprod * term;
+            re[k] -= rtmp;
+            im[k] -= itmp;
+        }
+        if (src32)
+            src32[i] = make_float4((float)r0, (float)i0, (float)v, 0.f);
+    }
+    double *ex = exps + (size_t)2 * P * id;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const double a = tc_wave_tree_sum(re[k]);
+        const double b = tc_wave_tree_sum(im[k]);
+        if (lane == k) {
+            ex[k] = a;
+            ex[P + k] = b;
+        }
+    }
+    if (lane == 0) {
+        TcNode nd;
+        nd.xcom = xc;
+        nd.ycom = yc;
+        nd.r = rad;
+        nd.mass = m;
+        nd.s = s;
+        nd.e = e;
+        nd.child = 0;
+        nd.level = l;
+        nd.cx = cc.x;
+        nd.cy = cc.y;
+        nd.pad0 = nd.pad1 = 0;
+        nodes[id] = nd;
+        nodew[id] = make_double4(w, wx, wy, 0.0);
+    }
+}
+
+// One 64-thread block per internal node of one level: the parent update of
+// upward.cpp:95-218 and upward_e2e (upward-kernels.m4:95-149), children
+// summed in order c = 0..3 (lane l owns coefficient l+1).
+template <int P>
+__global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int r1, const int2 *se,
+                                                   const int *level, const int2 *cell, const double *box,
+                                                   TcNode *nodes, double4 *nodew, double *exps)
+{
+    __shared__ double s_rinv[P + 1], s_iinv[P + 1], s_rc[P + 1], s_ic[P + 1];
+    const int rank = r0 + blockIdx.x;
+    if (rank >= r1)
+        return;
+    const int lane = threadIdx.x;
+    const int id = selfid[rank];
+    const int cb = 1 + 4 * rank;
+    const int l = level[id];
+    const int2 cc = cell[id];
+    const double xmin = box[0], ymin = box[1], ext = box[2];
+
+    TcNode ch[4];
+    double4 cw[4];
+    double msum = 0, wsum = 0, wxsum = 0, wysum = 0;
+    for (int c = 0; c < 4; ++c) {
+        ch[c] = nodes[cb + c];
+        cw[c] = nodew[cb + c];
+        msum += ch[c].mass;
+        wsum += cw[c].x;
+        wxsum += cw[c].y;
+        wysum += cw[c].z;
+    }
+    const double h = ext / (double)(1 << l);
+    const double xcp = wsum != 0 ? wxsum / wsum : (xmin + h * (double)cc.x) + 0.5 * h;
+    const double ycp = wsum != 0 ? wysum / wsum : (ymin + h * (double)cc.y) + 0.5 * h;
+    double rr = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (cw[c].x != 0) {
+            const double rx = xcp - ch[c].xcom, ry = ycp - ch[c].ycom;
+            const double v = ch[c].r + sqrt(rx * rx + ry * ry);
+            rr = rr < v ? v : rr;
+        }
+    }
+    const double rcap = (double)1.4143f * ext / (double)(1 << l);
+    const double rad = rcap < rr ? rcap : rr;
+
+    int zcount = 0, nz = 0;
+    for (int c = 0; c < 4; ++c) {
+        const bool neg = fabs(cw[c].x) < 1.1e-16;
+        zcount += neg;
+        if (!neg)
+            nz = c;
+    }
+    double dre = 0, dim = 0;
+    if (zcount == 3) {
+        if (lane < P) {
+            dre = exps[(size_t)2 * P * (cb + nz) + lane];
+            dim = exps[(size_t)2 * P * (cb + nz) + P + lane];
+        }
+    } else if (zcount < 3) {
+        for (int c = 0; c < 4; ++c) {
+            if (cw[c].x == 0)
+                continue;
+            const double x0 = ch[c].xcom - xcp, y0 = ch[c].ycom - ycp;
+            const double mass = ch[c].mass;
+            const double *ca = exps + (size_t)2 * P * (cb + c);
+            if (x0 == 0 && y0 == 0) {
+                if (lane < P) {
+                    dre += ca[lane];
+                    dim += ca[P + lane];
+                }
+                continue;
+            }
+            if (lane == 0) {
+                const double r2z0 = x0 * x0 + y0 * y0;
+                const double ri1 = x0 / r2z0, ii1 = -y0 / r2z0;
+                double ri = ri1, ii = ii1;
+                s_rinv[1] = ri;
+                s_iinv[1] = ii;
+                for (int j = 2; j <= P; ++j) {
+                    const double rn = ri * ri1 - ii * ii1;
+                    const double in = ri * ii1 + ii * ri1;
+                    ri = rn;
+                    ii = in;
+                    s_rinv[j] = ri;
+                    s_iinv[j] = ii;
+                }
+            }
+            __syncthreads();
+            if (lane < P) {
+                const int j = lane + 1;
+                s_rc[j] = ca[j - 1] * s_rinv[j] - ca[P + j - 1] * s_iinv[j];
+                s_ic[j] = ca[j - 1] * s_iinv[j] + ca[P + j - 1] * s_rinv[j];
+            }
+            __syncthreads();
+            if (lane < P) {
+                const int ll = lane + 1;
+                double rt = (ll == 1) ? -mass : mass * (-(1.0 / (double)ll));
+                double it = s_ic[1];
+                rt = rt + s_rc[1];
+                for (int k = 2; k <= ll; ++k) {
+                    const double b = c_binom[(ll - 1) * 64 + (k - 1)];
+                    rt = rt + s_rc[k] * b;
+                    it = it + s_ic[k] * b;
+                }
+                const double ri = s_rinv[ll], ii = s_iinv[ll];
+                const double invz2 = ri * ri + ii * ii;
+                const double invinv = invz2 != 0 ? 1 / invz2 : 0;
+                const double rz = ri * invinv;
+                const double iz = -ii * invinv;
+                dre += rt * rz - it * iz;
+                dim += rt * iz + it * rz;
+            }
+            __syncthreads();
+        }
+    }
+    if (lane < P) {
+        exps[(size_t)2 * P * id + lane] = dre;
+        exps[(size_t)2 * P * id + P + lane] = dim;
+    }
+    if (lane == 0) {
+        const int2 r = se[id];
+        TcNode nd;
+        nd.xcom = xcp;
+        nd.ycom = ycp;
+        nd.r = rad;
+        nd.mass = msum;
+        nd.s = r.x;
+        nd.e = r.y;
+        nd.child = cb;
+        nd.level = l;
+        nd.cx = cc.x;
+        nd.cy = cc.y;
+        nd.pad0 = nd.pad1 = 0;
+        nodes[id] = nd;
+        nodew[id] = make_double4(wsum, wxsum, wysum, 0.0);
+    }
+}
+
+// FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, tiny), so
+// that the FP32 E2P evaluates sum b_j (s/z)^j without over/underflow.
+template <int P>
+__global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const double *exps, int nnodes, float2 *ex32)
+{
+    const int id = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = tc_lane();
+    if (id >= nnodes || lane >= P)
+        return;
+    const double r = nodes[id].r;
+    const double sc = r > 1e-300 ? r : 1e-300;
+    // s^-(j) by repeated division keeps the scale exact enough in FP64
+    double inv = 1.0;
+    for (int j = 0; j <= lane; ++j)
+        inv /= sc;
+    const double *ex = exps + (size_t)2 * P * id;
+    ex32[(size_t)P * id + lane] = make_float2((float)(ex[lane] * inv), (float)(ex[P + lane] * inv));
+}
+
+int tiles_of(int n, int t) { return (n + t - 1) / t; }
+
+}  // namespace
+
+// ================================================================== host side
+
+void tc_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st)
+{
+    if (!n)
+        return;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<grid, 256, 0, st>>>(p, v, n);
+    TC_HIP(hipGetLastError());
+}
+
+void tc_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total_d, TcScratch &ws,
+                           hipStream_t st)
+{
+    if (n <= 4 * SCAN_CHUNK) {
+        k_scan_single<<<1, SCAN_NT, 0, st>>>(in, out, n, total_d);
+        TC_HIP(hipGetLastError());
+        return;
+    }
+    const int nb = tiles_of(n, SCAN_CHUNK);
+    ws.scan_tmp.reserve(sizeof(uint32_t) * (size_t)(2 * nb + 4));
+    uint32_t *part = ws.scan_tmp.as<uint32_t>();
+    uint32_t *partx = part + nb;
+    k_scan_partial<<<nb, SCAN_NT, 0, st>>>(in, n, part);
+    k_scan_single<<<1, SCAN_NT, 0, st>>>(part, partx, nb, total_d);
+    k_scan_final<<<nb, SCAN_NT, 0, st>>>(in, out, n, partx);
+    TC_HIP(hipGetLastError());
+}
+
+void tc_upload_binomials(hipStream_t st)
+{
+    std::vector<double> tab(64 * 64, 0.0);
+    for (int n = 0; n < 64; ++n)
+        for (int k = 0; k <= n; ++k) {
+            unsigned __int128 r = 1;
+            const int kk = k < n - k ? k : n - k;
+            for (int i = 1; i <= kk; ++i)
+                r = r * (unsigned __int128)(n - kk + i) / (unsigned __int128)i;
+            tab[n * 64 + k] = (double)r;
+        }
+    TC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_binom), tab.data(), sizeof(double) * tab.size(), 0,
+                                  hipMemcpyHostToDevice, st));
+    TC_HIP(hipStreamSynchronize(st));
+}
+
+void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st)
+{
+    const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
+    ws.bbox_part.reserve(sizeof(double4) * (size_t)np + 256);
+    double4 *part = ws.bbox_part.as<double4>();
+    uint32_t *bad = reinterpret_cast<uint32_t *>(part + np);
+    tc_fill_u32(bad, 0u, 1, st);
+    k_bbox_partial<<<np, 256, 0, st>>>(x, y, n, part, bad);
+    k_bbox_final<<<1, 64, 0, st>>>(part, np, box_d);
+    TC_HIP(hipGetLastError());
+}
+
+static const uint32_t *tc_bbox_bad_flag(TcScratch &ws, int n)
+{
+    const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
+    return reinterpret_cast<const uint32_t *>(ws.bbox_part.as<double4>() + np);
+}
+
+void tc_morton_sort(const double *x, const double *y, int n, const double *box_d, uint32_t *keys_out,
+                    uint32_t *perm_out, TcScratch &ws, hipStream_t st)
+{
+    ws.k2.reserve(sizeof(uint32_t) * (size_t)n);
+    ws.v2.reserve(sizeof(uint32_t) * (size_t)n);
+    k_keys<<<tiles_of(n, 256), 256, 0, st>>>(x, y, n, box_d, keys_out);
+    TC_HIP(hipGetLastError());
+    const int ntiles = tiles_of(n, RS_TILE);
+    ws.hist.reserve(sizeof(uint32_t) * (size_t)RS_BINS * ntiles * 2);
+    uint32_t *hist = ws.hist.as<uint32_t>();
+    uint32_t *offs = hist + (size_t)RS_BINS * ntiles;
+    // ping-pong out -> scratch -> out -> scratch -> out (four 8-bit passes over 30-bit keys);
+    // the first pass generates the index payload itself
+    uint32_t *kin[4] = {keys_out, ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>()};
+    uint32_t *vin[4] = {nullptr, ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>()};
+    uint32_t *kout[4] = {ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>(), keys_out};
+    uint32_t *vout[4] = {ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>(), perm_out};
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 8 * pass;
+        k_rs_hist<<<ntiles, RS_NT, 0, st>>>(kin[pass], n, shift, hist, ntiles);
+        tc_scan_exclusive_u32(hist, offs, RS_BINS * ntiles, nullptr, ws, st);
+        k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift, offs,
+                                                ntiles);
+        TC_HIP(hipGetLastError());
+    }
+}
+
+template <int P>
+static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, const int *lev, const int2 *cell,
+                          const int *selfid, bool fp32, hipStream_t st)
+{
+    const double *box = t.box_d.as<double>();
+    k_leaf_upward<P><<<tiles_of(t.nnodes, 4), 256, 0, st>>>(child, se, lev, cell, t.src.as<double4>(), box,
+                                                             t.nnodes, t.nodes.as<TcNode>(), t.exps.as<double>(),
+                                                             t.nodew.as<double4>(),
+                                                             fp32 ? t.src32.as<float4>() : nullptr);
+    TC_HIP(hipGetLastError());
+    for (int L = TC_LMAX - 1; L >= 0; --L) {
+        const int r0 = t.level_start[L], r1 = t.level_start[L + 1];
+        if (r1 > r0) {
+            k_e2e_level<P><<<r1 - r0, 64, 0, st>>>(selfid, r0, r1, se, lev, cell, box, t.nodes.as<TcNode>(),
+                                                    t.nodew.as<double4>(), t.exps.as<double>());
+            TC_HIP(hipGetLastError());
+        }
+    }
+    if (fp32) {
+        k_exps32<P><<<tiles_of(t.nnodes, 4), 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), t.nnodes,
+                                                            t.exps32.as<float2>());
+        TC_HIP(hipGetLastError());
+    }
+}
+
+void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
+                   bool fp32, TcScratch &ws, hipStream_t st)
+{
+    if (n <= 0)
+        throw TcError(-3, "tc_build_tree: no sources");
+    if (leafcap < 1)
+        throw TcError(-3, "tc_build_tree: leaf capacity must be >= 1");
+    if (n >= (1 << 30))
+        throw TcError(-3, "tc_build_tree: at most 2^30 - 1 sources per tree");
+    if (order != 8 && order != 12 && order != 16 && order != 20 && order != 24 && order != 32)
+        throw TcError(-3, "tc_build_tree: unsupported expansion order " + std::to_string(order));
+    t.n = n;
+    t.order = order;
+    t.leafcap = leafcap;
+    t.has_fp32 = fp32;
+    t.box_d.reserve(sizeof(double) * 4);
+    t.keys.reserve(sizeof(uint32_t) * (size_t)n);
+    t.perm.reserve(sizeof(uint32_t) * (size_t)n);
+    t.src.reserve(sizeof(double4) * (size_t)n);
+    if (fp32)
+        t.src32.reserve(sizeof(float4) * (size_t)n);
+
+    double *box = t.box_d.as<double>();
+    tc_bbox(x, y, n, box, ws, st);
+    uint32_t *keys = t.keys.as<uint32_t>(), *perm = t.perm.as<uint32_t>();
+    tc_morton_sort(x, y, n, box, keys, perm, ws, st);
+    k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(x, y, v, perm, n, t.src.as<double4>());
+    TC_HIP(hipGetLastError());
+
+    // census of internal cells
+    const int nblk = tiles_of(n, 4 * TB_CHUNK);
+    ws.part.reserve(sizeof(uint32_t) * ((size_t)2 * TC_LMAX * nblk + 8));
+    uint32_t *cnt = ws.part.as<uint32_t>();
+    uint32_t *offs = cnt + (size_t)TC_LMAX * nblk;
+    uint32_t *total = offs + (size_t)TC_LMAX * nblk;
+    ws.info.reserve(sizeof(int) * 32);
+    int *info = ws.info.as<int>();
+    k_int_count<<<nblk, 256, 0, st>>>(keys, n, leafcap, cnt, nblk);
+    tc_scan_exclusive_u32(cnt, offs, TC_LMAX * nblk, total, ws, st);
+    k_level_starts<<<1, 64, 0, st>>>(offs, nblk, total, tc_bbox_bad_flag(ws, n), info);
+    TC_HIP(hipGetLastError());
+    int hinfo[TC_LMAX + 2];
+    TC_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(t.box, box, sizeof(t.box), hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    if (hinfo[TC_LMAX + 1])
+        throw TcError(-4, "treecode: non-finite source coordinate");
+    const int T = hinfo[TC_LMAX];
+    for (int L = 0; L < TC_LMAX; ++L)
+        t.level_start[L] = hinfo[L];
+    t.level_start[TC_LMAX] = T;
+    t.level_start[TC_LMAX + 1] = T;
+    t.ninternal = T;
+    t.nnodes = 1 + 4 * T;
+    const int nn = t.nnodes;
+
+    t.nodes.reserve(sizeof(TcNode) * (size_t)nn);
+    t.exps.reserve(sizeof(double) * 2 * (size_t)order * nn);
+    t.nodew.reserve(sizeof(double4) * (size_t)nn);
+    t.selfid.reserve(sizeof(int) * (size_t)(T + 1));
+    if (fp32)
+        t.exps32.reserve(sizeof(float2) * (size_t)order * nn);
+    ws.istart.reserve(sizeof(int) * (size_t)(T + 1));
+    ws.ilevel.reserve(sizeof(int) * (size_t)(T + 1));
+    // per-node topology scratch: child | se | level | cell
+    ws.ints.reserve(sizeof(int) * (size_t)nn * 6 + 64);
+    uint32_t *child = ws.ints.as<uint32_t>();
+    int2 *se = reinterpret_cast<int2 *>(child + ((nn + 1) & ~1));
+    int *lev = reinterpret_cast<int *>(se + nn);
+    int2 *cell = reinterpret_cast<int2 *>(lev + ((nn + 1) & ~1));
+
+    tc_fill_u32(child, 0u, (size_t)nn, st);
+    k_root_init<<<1, 1, 0, st>>>(n, se, lev, cell);
+    if (T > 0) {
+        k_int_emit<<<nblk, 256, 0, st>>>(keys, n, leafcap, offs, nblk, ws.istart.as<int>(), ws.ilevel.as<int>());
+        k_int_nodes<<<tiles_of(T, 256), 256, 0, st>>>(keys, n, leafcap, ws.istart.as<int>(), ws.ilevel.as<int>(),
+                                                      info, T, t.selfid.as<int>(), child, se, lev, cell);
+    }
+    TC_HIP(hipGetLastError());
+
+    const int *sid = t.selfid.as<int>();
+    switch (order) {
+    case 8: upward_launch<8>(t, child, se, lev, cell, sid, fp32, st); break;
+    case 12: upward_launch<12>(t, child, se, lev, cell, sid, fp32, st); break;
+    case 16: upward_launch<16>(t, child, se, lev, cell, sid, fp32, st); break;
+    case 20: upward_launch<20>(t, child, se, lev, cell, sid, fp32, st); break;
+    case 24: upward_launch<24>(t, child, se, lev, cell, sid, fp32, st); break;
+    case 32: upward_launch<32>(t, child, se, lev, cell, sid, fp32, st); break;
+    }
+}
+
+void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st)
+{
+    ws.tkeys.reserve(sizeof(uint32_t) * (size_t)m + 64);
+    ws.tperm.reserve(sizeof(uint32_t) * (size_t)m + 64);
+    ws.tsorted.reserve(sizeof(double) * 2 * (size_t)m + 64);
+    // the target box lives after the keys' tail
+    DevBuf &boxbuf = ws.info;
+    ws.info.reserve(sizeof(int) * 32 + sizeof(double) * 4);
+    double *box = reinterpret_cast<double *>(boxbuf.as<char>() + 128);
+    tc_bbox(xt, yt, m, box, ws, st);
+    tc_morton_sort(xt, yt, m, box, ws.tkeys.as<uint32_t>(), ws.tperm.as<uint32_t>(), ws, st);
+}

@@ -1,0 +1,206 @@
+// tc_common.h -- shared definitions of the MI355X (gfx950) treecode library.
+//
+// Data layout in HBM (one tree per context):
+//   sorted sources   double4 {x, y, v, 0}         32 B/source, Morton order
+//   sorted keys      uint32  30-bit Morton key     4 B/source
+//   permutation      uint32  sorted -> input index 4 B/source
+//   nodes            TcNode (64 B), breadth-first ids: root 0, children of the
+//                    k-th internal node (level, Morton order) at 1 + 4k + c
+//   expansions       double[2p] per node: [Re a_1..a_p][Im a_1..a_p]
+//   FP32 mirrors     float4 {x-xc, y-yc, v, 0} per source (leaf-relative) and
+//                    float2[p] per node of scaled coefficients a_j / r^j
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#define TC_LMAX 15
+#define TC_MAXORDER 32
+#define TC_STACK 48  // DFS stack: 3 * LMAX + 1 entries suffice
+
+struct alignas(16) TcNode {
+    double xcom, ycom, r, mass;
+    int s, e, child, level;  // child: first child id, 0 for a leaf
+    int cx, cy, pad0, pad1;  // integer cell coordinates at `level`
+};
+static_assert(sizeof(TcNode) == 64, "TcNode must stay 64 B (one s_load_dwordx16)");
+
+enum TcCounter { TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_NCOUNTERS = 4 };
+
+struct TcError : std::runtime_error {
+    int code;
+    TcError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define TC_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (call);                                                                       \
+        if (_e != hipSuccess)                                                                         \
+            throw TcError(-2, std::string("HIP error ") + hipGetErrorString(_e) + " at " __FILE__ ":" + \
+                                  std::to_string(__LINE__) + " in " #call);                          \
+    } while (0)
+
+// ------------------------------------------------------------------ device helpers
+#ifdef __HIP_DEVICE_COMPILE__
+#define TC_DEV __device__ __forceinline__
+#else
+#define TC_DEV __device__ inline
+#endif
+
+TC_DEV int tc_lane() { return __lane_id(); }
+
+// Orders LDS traffic between lanes of one wavefront (no instruction, only a
+// code-motion and memory-model fence at wavefront scope).
+TC_DEV void tc_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+TC_DEV int tc_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+TC_DEV double tc_uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+TC_DEV unsigned long long tc_uniform64(unsigned long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffu));
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+TC_DEV unsigned long long tc_ballot(bool p) { return __ballot(p); }
+
+TC_DEV unsigned long long tc_lanemask_lt() { return (1ull << tc_lane()) - 1ull; }
+
+// Canonical pairwise tree over the 64 lanes (lane j gets p[j] + p[j ^ off] for
+// off = 32..1); lane 0 ends with ((p0+p32)+(p16+p48))+..., the same order the
+// oracle uses for leaf sums.
+TC_DEV double tc_wave_tree_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+        v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+TC_DEV double tc_wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = v < o ? o : v;
+    }
+    return v;
+}
+
+TC_DEV uint32_t tc_spread15(uint32_t x)
+{
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+TC_DEV uint32_t tc_compact15(uint32_t x)
+{
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    x = (x | (x >> 8)) & 0x0000FFFFu;
+    return x;
+}
+
+// Deepest level at which the Morton prefixes of a and b agree (0..15).
+TC_DEV int tc_common_level(uint32_t a, uint32_t b)
+{
+    const uint32_t x = a ^ b;
+    if (!x)
+        return TC_LMAX;
+    const int hb = 31 - __clz((int)x);
+    return (TC_LMAX - 1) - (hb >> 1);
+}
+
+// ------------------------------------------------------------------ host-side internals
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+    void reserve(size_t bytes)
+    {
+        if (bytes <= cap)
+            return;
+        if (ptr)
+            TC_HIP(hipFree(ptr));
+        ptr = nullptr;
+        cap = 0;
+        size_t b = bytes < 256 ? 256 : bytes;
+        b = (b + 4095) & ~size_t(4095);
+        TC_HIP(hipMalloc(&ptr, b));
+        cap = b;
+    }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+// A built tree resident in HBM.
+struct TcTree {
+    int n = 0, order = 0, leafcap = 0;
+    int nnodes = 0, ninternal = 0;
+    int level_start[TC_LMAX + 2] = {0};  // internal ranks of level L: [level_start[L], level_start[L+1])
+    double box[4] = {0, 0, 0, 0};        // xmin, ymin, ext, scale (host copy)
+    bool has_fp32 = false;
+    DevBuf keys, perm, src, src32, nodes, exps, exps32, nodew, selfid;
+    DevBuf box_d;
+    void release()
+    {
+        for (DevBuf *b : {&keys, &perm, &src, &src32, &nodes, &exps, &exps32, &nodew, &selfid, &box_d})
+            b->release();
+    }
+};
+
+// Scratch reused across calls of one context.
+struct TcScratch {
+    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints;
+    void release()
+    {
+        for (DevBuf *b : {&k2, &v2, &hist, &part, &scan_tmp, &istart, &ilevel, &info, &tkeys, &tperm, &tsorted,
+                          &bbox_part, &ints})
+            b->release();
+    }
+};
+
+// Internal entry points (tc_build.hip, tc_eval.hip)
+void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st);
+void tc_morton_sort(const double *x, const double *y, int n, const double *box_d, uint32_t *keys_out,
+                    uint32_t *perm_out, TcScratch &ws, hipStream_t st);
+void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
+                   bool fp32, TcScratch &ws, hipStream_t st);
+void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);
+void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
+                       int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
+                       hipEvent_t ev0, hipEvent_t ev1);
+void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
+                   int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
+                   hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+void tc_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total_d, TcScratch &ws,
+                           hipStream_t st);
+void tc_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+void tc_upload_binomials(hipStream_t st);

@@ -1,0 +1,546 @@
+// tc_eval.hip -- theta-criterion traversals on gfx950.
+//
+// Potential (drivers/treecode-potential.cpp:25-86): one wavefront per 64
+// Morton-sorted targets.  The wave walks the tree once with a stack of
+// (node, lane mask) entries; every lane takes exactly the accept/open
+// decisions of its own per-target DFS, so each target sees the reference's
+// interaction list.  Node data and expansion coefficients are wave-uniform
+// (scalar loads); leaf sources are staged through LDS 64 at a time and
+// broadcast to the lanes.
+//
+// Force (drivers/treecode-force.cpp:94-199): one wavefront per 8x8 brick of a
+// 32x32 block, lane = target; decisions are brick-wide, so the DFS is
+// uniform.  E2L is wave-cooperative (lane l owns local coefficient l+1).
+//
+// precision 0 = plain FP64; 1 = FP32 bulk: P2P and E2P run in FP32 on
+// leaf-relative offsets with per-tile flushes into FP64 accumulators, while
+// decisions, E2L and L2P stay FP64 (same interaction lists as FP64).
+#include "tc_common.h"
+
+#include <cfloat>
+#include <vector>
+
+namespace {
+
+constexpr double EPS_REG = 10.0 * DBL_EPSILON;  // potential-kernels.m4:15, force-kernels.m4:26
+constexpr float EPS_REG32 = (float)(10.0 * DBL_EPSILON);
+constexpr float LN2_HALF = 0.34657359027997264f;  // ln(2)/2: log(r^2)/2 = log2(r^2) * ln2/2
+constexpr int TILE = 8;
+
+__constant__ double c_binomT[TC_MAXORDER * 64];  // c_binomT[k*64 + n] = C(n, k)
+
+TC_DEV float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+TC_DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// 1/d to ~1 ulp for positive normal d: hardware estimate + two Newton steps.
+TC_DEV double rcp64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+// ------------------------------------------------------------------ potential kernels
+// potential_e2p (potential-kernels.m4:59-89) for one expansion
+template <int P>
+TC_DEV double e2p_pot64(double rz, double iz, double mass, const double *__restrict__ ex)
+{
+#pragma clang fp contract(fast)
+    const double r2 = rz * rz + iz * iz;
+    const double inv = 1.0 / r2;
+    const double ri1 = rz * inv, ii1 = -iz * inv;
+    double ri = ri1, ii = ii1;
+    double sum = ex[0] * ri - ex[P] * ii;
+#pragma unroll
+    for (int j = 1; j < P; ++j) {
+        const double rn = ri * ri1 - ii * ii1;
+        const double in = ri * ii1 + ii * ri1;
+        ri = rn;
+        ii = in;
+        sum += ex[j] * ri - ex[P + j] * ii;
+    }
+    return mass * log(r2) * 0.5 + sum;
+}
+
+// FP32 E2P on scaled coefficients b_j = a_j / s^j:  Re sum b_j (s/z)^j.
+template <int P>
+TC_DEV float e2p_pot32(float rz, float iz, float mass, float sc, const float2 *__restrict__ ex)
+{
+#pragma clang fp contract(fast)
+    const float r2 = rz * rz + iz * iz;
+    const float inv = fast_rcp(r2);
+    const float wr = sc * rz * inv, wi = -sc * iz * inv;
+    float pr = wr, pi = wi;
+    float sum = ex[0].x * pr - ex[0].y * pi;
+#pragma unroll
+    for (int j = 1; j < P; ++j) {
+        const float rn = pr * wr - pi * wi;
+        const float in = pr * wi + pi * wr;
+        pr = rn;
+        pi = in;
+        sum += ex[j].x * pr - ex[j].y * pi;
+    }
+    return mass * fast_log2(r2) * LN2_HALF + sum;
+}
+
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+                                                   const float2 *__restrict__ exps32,
+                                                   const double4 *__restrict__ src,
+                                                   const float4 *__restrict__ src32, const double *__restrict__ xt,
+                                                   const double *__restrict__ yt, const uint32_t *__restrict__ tperm,
+                                                   int m, double th2, double *__restrict__ out,
+                                                   unsigned long long *counters)
+{
+    __shared__ int s_node[4][TC_STACK];
+    __shared__ unsigned long long s_mask[4][TC_STACK];
+    __shared__ double4 s_src[4][64];
+    const int wid = threadIdx.x >> 6, lane = tc_lane();
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < m;
+    const uint32_t ti = valid ? tperm[k] : 0u;
+    const double x = valid ? xt[ti] : 0.0, y = valid ? yt[ti] : 0.0;
+    const unsigned long long live = tc_ballot(valid);
+    if (!live)
+        return;
+    double acc = 0;
+    unsigned long long np2p = 0, ne2p = 0;
+    int sp = 0;
+    s_node[wid][0] = 0;
+    s_mask[wid][0] = live;
+    sp = 1;
+    while (sp > 0) {
+        --sp;
+        tc_wave_sync();
+        const int id = tc_uniform(s_node[wid][sp]);
+        const unsigned long long mask = tc_uniform64(s_mask[wid][sp]);
+        const TcNode nd = nodes[id];
+        if (nd.s == nd.e)
+            continue;
+        const bool in = (mask >> lane) & 1ull;
+        const double dx = x - nd.xcom, dy = y - nd.ycom;
+        const double r2 = dx * dx + dy * dy;
+        const bool acc_ok = in && (nd.r * nd.r < th2 * r2);
+        const unsigned long long amask = tc_ballot(acc_ok);
+        if (acc_ok) {
+            if (F32) {
+                const float sc = (float)(nd.r > 1e-30 ? nd.r : 1e-30);
+                acc += (double)e2p_pot32<P>((float)dx, (float)dy, (float)nd.mass, sc, exps32 + (size_t)P * id);
+            } else {
+                acc += e2p_pot64<P>(dx, dy, nd.mass, exps + (size_t)2 * P * id);
+            }
+        }
+        ne2p += __popcll(amask);
+        const unsigned long long omask = mask & ~amask;
+        if (!omask)
+            continue;
+        if (nd.child == 0) {
+            const bool act = (omask >> lane) & 1ull;
+            np2p += (unsigned long long)__popcll(omask) * (unsigned long long)(nd.e - nd.s);
+            if (F32) {
+                const float tx = (float)(x - nd.xcom), ty = (float)(y - nd.ycom);
+                float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
+                double leafsum = 0;
+                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
+                    const int cnt = min(64, nd.e - b0);
+                    if (lane < cnt)
+                        tile[lane] = src32[b0 + lane];
+                    tc_wave_sync();
+                    if (act) {
+#pragma clang fp contract(fast)
+                        float s32 = 0;
+                        int j = 0;
+                        for (; j + 2 <= cnt; j += 2) {
+                            const float4 q0 = tile[j], q1 = tile[j + 1];
+                            const float ax = tx - q0.x, ay = ty - q0.y;
+                            const float bx = tx - q1.x, by = ty - q1.y;
+                            const float da = ax * ax + (ay * ay + EPS_REG32);
+                            const float db = bx * bx + (by * by + EPS_REG32);
+                            s32 += q0.z * fast_log2(da);
+                            s32 += q1.z * fast_log2(db);
+                        }
+                        for (; j < cnt; ++j) {
+                            const float4 q = tile[j];
+                            const float ax = tx - q.x, ay = ty - q.y;
+                            s32 += q.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                        }
+                        leafsum += (double)s32;
+                    }
+                    tc_wave_sync();
+                }
+                acc += leafsum * (double)LN2_HALF;
+            } else {
+                double leafsum = 0;
+                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
+                    const int cnt = min(64, nd.e - b0);
+                    if (lane < cnt)
+                        s_src[wid][lane] = src[b0 + lane];
+                    tc_wave_sync();
+                    if (act) {
+#pragma clang fp contract(fast)
+                        for (int j = 0; j < cnt; ++j) {
+                            const double4 q = s_src[wid][j];
+                            const double ax = x - q.x, ay = y - q.y;
+                            leafsum += log(ax * ax + ay * ay + EPS_REG) * q.z;
+                        }
+                    }
+                    tc_wave_sync();
+                }
+                acc += leafsum * 0.5;
+            }
+        } else {
+            // children pushed in order 0..3, popped 3..0 (treecode-potential.cpp:76-79)
+            if (lane < 4) {
+                s_node[wid][sp + lane] = nd.child + lane;
+                s_mask[wid][sp + lane] = omask;
+            }
+            sp += 4;
+        }
+    }
+    if (valid)
+        out[ti] = acc;
+    if (counters && lane == 0) {
+        atomicAdd(&counters[TC_CNT_P2P], np2p);
+        atomicAdd(&counters[TC_CNT_E2P], ne2p);
+    }
+}
+
+// ------------------------------------------------------------------ force kernels
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+                                               const float2 *__restrict__ exps32, const double4 *__restrict__ src,
+                                               const float4 *__restrict__ src32, const double *__restrict__ x0s,
+                                               const double *__restrict__ y0s, const double *__restrict__ hs,
+                                               int nbricks, double theta, double *__restrict__ xres,
+                                               double *__restrict__ yres, unsigned long long *counters)
+{
+    __shared__ int s_node[4][TC_STACK];
+    __shared__ double4 s_src[4][64];
+    __shared__ double2 s_coef[4][TC_MAXORDER + 1];
+    __shared__ double s_binom[TC_MAXORDER][64];
+
+    for (int i = threadIdx.x; i < P * 64; i += 256)
+        s_binom[i >> 6][i & 63] = c_binomT[i];
+    __syncthreads();
+
+    const int wid = threadIdx.x >> 6, lane = tc_lane();
+    const int wave = tc_uniform((int)(blockIdx.x * 4 + wid));
+    if (wave >= nbricks)
+        return;
+    const int b = wave >> 4, brick = wave & 15;
+    const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
+    const double x0 = x0s[b], y0 = y0s[b], h = hs[b];
+    const int ix = lane & 7, iy = lane >> 3;
+    // targets as in force_p2p_8x8 (force-kernels.m4:28-32)
+    const double xb = x0 + (bx + 0) * h, yb = y0 + (by + 0) * h;
+    const double xt = xb + ix * h, yt = yb + iy * h;
+    // brick geometry (treecode-force.cpp:108,113-114,146-147)
+    const double x0b = x0 + h * (bx + 0.5 * (TILE - 1));
+    const double y0b = y0 + h * (by + 0.5 * (TILE - 1));
+    const double rbrick = 1.4142135623730951 * h * (TILE - 1) * 0.5;
+    const double xlo = x0 + bx * h, xhi = x0 + (bx + TILE - 1) * h;
+    const double ylo = y0 + by * h, yhi = y0 + (by + TILE - 1) * h;
+    const double invtheta = 1 / theta;
+    const double th2 = theta * theta;
+
+    double fx = 0, fy = 0;
+    double lre = 0, lim = 0;
+    unsigned long long np2p = 0, ne2p = 0, ne2l = 0;
+
+    int sp = 1;
+    s_node[wid][0] = 0;
+    while (sp > 0) {
+        --sp;
+        tc_wave_sync();
+        const int id = tc_uniform(s_node[wid][sp]);
+        const TcNode nd = nodes[id];
+        if (nd.s == nd.e)
+            continue;
+        const double xcom = nd.xcom, ycom = nd.ycom, r = nd.r;
+        const double ddx = x0b - xcom, ddy = y0b - ycom;
+        const double distance = sqrt(ddx * ddx + ddy * ddy);
+        if ((distance / r - 1) > invtheta && rbrick <= r) {
+            // downward_e2l (force-kernels.m4:102-155), z0 = com - brick centre
+            ++ne2l;
+            const double zx = xcom - x0b, zy = ycom - y0b;
+            const double *ex = exps + (size_t)2 * P * id;
+            double ri, ii;
+            {
+#pragma clang fp contract(fast)
+                const double r2z0 = zx * zx + zy * zy;
+                const double inv = 1.0 / r2z0;
+                const double ri1 = zx * inv, ii1 = -zy * inv;
+                ri = ri1;
+                ii = ii1;
+                double pr = ri1, pi = ii1;
+                for (int j = 2; j <= P; ++j) {
+                    const double rn = pr * ri1 - pi * ii1;
+                    const double in = pr * ii1 + pi * ri1;
+                    pr = rn;
+                    pi = in;
+                    if (j == lane + 1) {
+                        ri = pr;
+                        ii = pi;
+                    }
+                }
+                if (lane < P) {
+                    const double ar = ex[lane], ai = ex[P + lane];
+                    const double sgn = (lane & 1) ? 1.0 : -1.0;  // (-1)^k with k = lane+1
+                    s_coef[wid][lane + 1] = make_double2(sgn * (ar * ri - ai * ii), sgn * (ar * ii + ai * ri));
+                }
+            }
+            tc_wave_sync();
+            if (lane < P) {
+#pragma clang fp contract(fast)
+                const int L = lane + 1;
+                double rt = (L == 1) ? -nd.mass : nd.mass * (-(1.0 / (double)L));
+                double it = 0;
+                for (int kk = 1; kk <= P; ++kk) {
+                    const double bb = s_binom[kk - 1][L + kk - 1];
+                    const double2 c = s_coef[wid][kk];
+                    rt += bb * c.x;
+                    it += bb * c.y;
+                }
+                lre += rt * ri - it * ii;
+                lim += rt * ii + it * ri;
+            }
+            tc_wave_sync();
+            continue;
+        }
+        const double cxt = xlo < (xcom < xhi ? xcom : xhi) ? (xcom < xhi ? xcom : xhi) : xlo;
+        const double cyt = ylo < (ycom < yhi ? ycom : yhi) ? (ycom < yhi ? ycom : yhi) : ylo;
+        const double r2 = (cxt - xcom) * (cxt - xcom) + (cyt - ycom) * (cyt - ycom);
+        if (r * r < th2 * r2) {
+            // force_e2p_8x8 (force-kernels.m4:57-98)
+            ne2p += 64;
+            const double rz = (xb - xcom) + ix * h, iz = (yb - ycom) + iy * h;
+            if (F32) {
+#pragma clang fp contract(fast)
+                const float2 *ex = exps32 + (size_t)P * id;
+                const float zr = (float)rz, zi = (float)iz;
+                const float inv = fast_rcp(zr * zr + zi * zi);
+                const float ir = zr * inv, iim = -zi * inv;
+                const float sc = (float)(r > 1e-30 ? r : 1e-30);
+                const float wr = sc * ir, wi = sc * iim;
+                float pr = 1.f, pi = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const float rn = pr * wr - pi * wi;
+                    const float in = pr * wi + pi * wr;
+                    pr = rn;
+                    pi = in;
+                    const float cr = ex[j].x * pr - ex[j].y * pi;
+                    const float ci = ex[j].x * pi + ex[j].y * pr;
+                    sr += (float)(j + 1) * cr;
+                    si += (float)(j + 1) * ci;
+                }
+                // dphi/dz = (Q - sum j b_j w^j) / z
+                const float qr = (float)nd.mass - sr, qi = -si;
+                fx += (double)(qr * ir - qi * iim);
+                fy -= (double)(qr * iim + qi * ir);
+            } else {
+#pragma clang fp contract(fast)
+                const double *ex = exps + (size_t)2 * P * id;
+                const double inv = 1.0 / (rz * rz + iz * iz);
+                const double ri1 = rz * inv, ii1 = -iz * inv;
+                double rsum = nd.mass * ri1, isum = nd.mass * ii1;
+                double pr = ri1, pi = ii1;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const double rn = pr * ri1 - pi * ii1;
+                    const double in = pr * ii1 + pi * ri1;
+                    pr = rn;
+                    pi = in;
+                    rsum -= (j + 1) * (ex[j] * pr - ex[P + j] * pi);
+                    isum -= (j + 1) * (ex[j] * pi + ex[P + j] * pr);
+                }
+                fx += rsum;
+                fy -= isum;
+            }
+            continue;
+        }
+        if (nd.child == 0) {
+            // force_p2p_8x8 (force-kernels.m4:15-55)
+            np2p += 64ull * (unsigned long long)(nd.e - nd.s);
+            if (F32) {
+                const float tx = (float)(xt - xcom), ty = (float)(yt - ycom);
+                float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
+                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
+                    const int cnt = min(64, nd.e - b0);
+                    if (lane < cnt)
+                        tile[lane] = src32[b0 + lane];
+                    tc_wave_sync();
+                    float ax32 = 0.f, ay32 = 0.f;
+                    for (int j = 0; j < cnt; ++j) {
+#pragma clang fp contract(fast)
+                        const float4 q = tile[j];
+                        const float dx = tx - q.x, dy = ty - q.y;
+                        const float f = q.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+                        ax32 += dx * f;
+                        ay32 += dy * f;
+                    }
+                    fx += (double)ax32;
+                    fy += (double)ay32;
+                    tc_wave_sync();
+                }
+            } else {
+                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
+                    const int cnt = min(64, nd.e - b0);
+                    if (lane < cnt)
+                        s_src[wid][lane] = src[b0 + lane];
+                    tc_wave_sync();
+                    for (int j = 0; j < cnt; ++j) {
+#pragma clang fp contract(fast)
+                        const double4 q = s_src[wid][j];
+                        const double dx = xt - q.x, dy = yt - q.y;
+                        const double f = q.z * rcp64(dx * dx + (dy * dy + EPS_REG));
+                        fx += dx * f;
+                        fy += dy * f;
+                    }
+                    tc_wave_sync();
+                }
+            }
+            continue;
+        }
+        if (lane < 4)
+            s_node[wid][sp + lane] = nd.child + lane;
+        sp += 4;
+    }
+
+    // downward_l2p_8x8 (force-kernels.m4:157-191) about the brick centre
+    if (lane < P)
+        s_coef[wid][lane + 1] = make_double2(lre, lim);
+    tc_wave_sync();
+    {
+#pragma clang fp contract(fast)
+        const double zl = h * (0 - 0.5 * (TILE - 1));
+        const double zr1 = zl + ix * h, zi1 = zl + iy * h;
+        const double2 b1 = s_coef[wid][1];
+        double rres = b1.x, ires = b1.y;
+        double zr = zr1, zi = zi1;
+        for (int l = 2; l <= P; ++l) {
+            const double2 bl = s_coef[wid][l];
+            rres += l * (bl.x * zr - bl.y * zi);
+            ires += l * (bl.x * zi + bl.y * zr);
+            const double rn = zr * zr1 - zi * zi1;
+            const double in = zr * zi1 + zi * zr1;
+            zr = rn;
+            zi = in;
+        }
+        fx += rres;
+        fy -= ires;
+    }
+    const size_t o = (size_t)b * 1024 + (size_t)(bx + ix) + 32 * (size_t)(by + iy);
+    xres[o] = fx;
+    yres[o] = fy;
+    if (counters && lane == 0) {
+        atomicAdd(&counters[TC_CNT_P2P], np2p);
+        atomicAdd(&counters[TC_CNT_E2P], ne2p);
+        atomicAdd(&counters[TC_CNT_E2L], ne2l);
+        atomicAdd(&counters[TC_CNT_L2P], 64ull);
+    }
+}
+
+int tiles_of(int n, int t) { return (n + t - 1) / t; }
+
+template <int P>
+void launch_potential(const TcTree &t, double th2, const double *xt, const double *yt, const uint32_t *tperm,
+                      int m, double *out, int precision, unsigned long long *counters, hipStream_t st)
+{
+    const int grid = tiles_of(m, 256);
+    if (precision == 1)
+        k_potential<P, true><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(),
+                                                   t.exps32.as<float2>(), t.src.as<double4>(),
+                                                   t.src32.as<float4>(), xt, yt, tperm, m, th2, out, counters);
+    else
+        k_potential<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
+                                                    t.src.as<double4>(), nullptr, xt, yt, tperm, m, th2, out,
+                                                    counters);
+    TC_HIP(hipGetLastError());
+}
+
+template <int P>
+void launch_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
+                  int nblocks, double *xr, double *yr, int precision, unsigned long long *counters, hipStream_t st)
+{
+    const int nbricks = nblocks * 16;
+    const int grid = tiles_of(nbricks, 4);
+    if (precision == 1)
+        k_force<P, true><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), t.exps32.as<float2>(),
+                                               t.src.as<double4>(), t.src32.as<float4>(), x0s, y0s, hs, nbricks,
+                                               theta, xr, yr, counters);
+    else
+        k_force<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
+                                                t.src.as<double4>(), nullptr, x0s, y0s, hs, nbricks, theta, xr, yr,
+                                                counters);
+    TC_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void tc_upload_binomials_eval(hipStream_t st)
+{
+    std::vector<double> tab(TC_MAXORDER * 64, 0.0);
+    for (int k = 0; k < TC_MAXORDER; ++k)
+        for (int n = k; n < 64; ++n) {
+            unsigned __int128 r = 1;
+            const int kk = k < n - k ? k : n - k;
+            for (int i = 1; i <= kk; ++i)
+                r = r * (unsigned __int128)(n - kk + i) / (unsigned __int128)i;
+            tab[k * 64 + n] = (double)r;
+        }
+    TC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_binomT), tab.data(), sizeof(double) * tab.size(), 0,
+                                  hipMemcpyHostToDevice, st));
+    TC_HIP(hipStreamSynchronize(st));
+}
+
+void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
+                       int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
+                       hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (m <= 0)
+        return;
+    if (precision == 1 && !t.has_fp32)
+        throw TcError(-3, "tc_eval_potential: tree was built without FP32 mirrors");
+    tc_sort_targets(xt, yt, m, ws, st);
+    const double th2 = theta * theta;
+    const uint32_t *tperm = ws.tperm.as<uint32_t>();
+    if (ev0)
+        TC_HIP(hipEventRecord(ev0, st));
+    switch (t.order) {
+    case 8: launch_potential<8>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    case 12: launch_potential<12>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    case 16: launch_potential<16>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    case 20: launch_potential<20>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    case 24: launch_potential<24>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    case 32: launch_potential<32>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    default: throw TcError(-3, "tc_eval_potential: unsupported order");
+    }
+    if (ev1)
+        TC_HIP(hipEventRecord(ev1, st));
+}
+
+void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
+                   int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
+                   hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (nblocks <= 0)
+        return;
+    if (precision == 1 && !t.has_fp32)
+        throw TcError(-3, "tc_eval_force: tree was built without FP32 mirrors");
+    if (ev0)
+        TC_HIP(hipEventRecord(ev0, st));
+    switch (t.order) {
+    case 8: launch_force<8>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    case 12: launch_force<12>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    case 16: launch_force<16>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    case 20: launch_force<20>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    case 24: launch_force<24>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    case 32: launch_force<32>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    default: throw TcError(-3, "tc_eval_force: unsupported order");
+    }
+    if (ev1)
+        TC_HIP(hipEventRecord(ev1, st));
+}

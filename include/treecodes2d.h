@@ -1,0 +1,118 @@
+/*
+ * treecodes2d.h -- C ABI of the MI355X treecode library (libtreecodes2d.so).
+ *
+ * The two drop-in entry points keep the reference prototypes exactly:
+ *   treecode_potential   drivers/treecode-potential.h:21-29 (README.md:5-13)
+ *   treecode_force_mrag  drivers/treecode-force.h:21-31     (README.md:15-25)
+ * and are also declared, byte-compatible with the reference's generated
+ * headers, in 2d-treecodes-potential.h and 2d-treecodes-force.h.
+ *
+ * Everything below the drop-in pair is an extension with no counterpart in
+ * the reference (whose library has no error channel, no device-resident
+ * interface and no options; it is configured at compile time by
+ * Makefile:13-15, drivers/Makefile:20).  All pointers are plain pointers,
+ * sizes are ints, no C++ or HIP types cross this boundary (streams are void*).
+ */
+#ifndef TREECODES2D_H
+#define TREECODES2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_API __attribute__((visibility("default")))
+
+/* ---- drop-in entry points (reference semantics, FP64, host memory, synchronous) ---- */
+
+/* drivers/treecode-potential.h:21-29: out[i] = sum_j v_j log|t_i - s_j| with the
+ * multipole order 12 / leaf capacity 512 of the reference build. */
+TC_API void treecode_potential(const double theta, const double *const xsources, const double *const ysources,
+                               const double *const sourcevalues, const int nsources,
+                               const double *const xtargets, const double *const ytargets, const int ntargets,
+                               double *const targetvalues);
+
+/* drivers/treecode-force.h:21-31: gradient of the potential at the 32x32 points of each
+ * block, (x0s[b] + ix*hs[b], y0s[b] + iy*hs[b]) written at ix + 32*(iy + 32*b);
+ * order 24 / leaf capacity 192 / 8x8 bricks as the reference build. */
+TC_API void treecode_force_mrag(const double theta, const double *const xsources, const double *const ysources,
+                                const double *const sourcevalues, const int nsources, const double *const x0s,
+                                const double *const y0s, const double *const hs, const int nblocks,
+                                double *const xresults, double *const yresults);
+
+/* ---- extensions (no counterpart in the reference) ---- */
+
+/* Message of the last failure on the calling thread ("" if none).  The void
+ * drop-in functions report failures here, print them once to stderr and fill
+ * their outputs with NaN. */
+TC_API const char *treecode_last_error(void);
+
+typedef struct tc_context tc_context;
+
+/* Options (tc_set_option keys) */
+enum {
+    TC_OPT_PRECISION = 0,      /* 0 = plain FP64 (default), 1 = FP32 bulk P2P/E2P */
+    TC_OPT_POT_ORDER = 1,      /* potential expansion order (8,12,16,20,24,32), default 12 */
+    TC_OPT_POT_LEAF = 2,       /* potential leaf capacity, default 512 */
+    TC_OPT_FORCE_ORDER = 3,    /* force expansion order, default 24 */
+    TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
+    TC_OPT_PROFILE = 5,        /* 1 = record phase times with HIP events */
+    TC_OPT_COUNTERS = 6        /* 1 = count interactions (p2p pairs, e2p, e2l, l2p) */
+};
+
+/* Phase times (tc_get_stats times_ms[]) */
+enum { TC_PH_BUILD = 0, TC_PH_TSORT = 1, TC_PH_EVAL = 2, TC_PH_TOTAL = 3, TC_NPHASES = 4 };
+
+/* A context owns one HIP stream, grow-only HBM buffers and the resident tree. */
+TC_API tc_context *tc_context_create(int device);
+TC_API void tc_context_destroy(tc_context *ctx);
+TC_API int tc_set_option(tc_context *ctx, int key, int value);
+TC_API void *tc_context_stream(tc_context *ctx);
+
+/* Potential / force with all pointers either in host memory (device_ptrs = 0:
+ * synchronous, copies in and out) or in device memory of the context's GPU
+ * (device_ptrs = 1: enqueued on the context stream; tc_synchronize waits).
+ * Return 0 on success, a negative code on failure (see treecode_last_error). */
+TC_API int tc_potential(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
+                        const double *xt, const double *yt, int m, double *out, int device_ptrs);
+TC_API int tc_force_mrag(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs,
+                         int n, const double *x0s, const double *y0s, const double *hs, int nblocks, double *xr,
+                         double *yr, int device_ptrs);
+TC_API int tc_synchronize(tc_context *ctx);
+
+/* Evaluate new targets against the tree left resident by the last tc_potential /
+ * tc_force_mrag call of this context (the sources are not rebuilt).  The tree must
+ * have been built for the same kind of solve (order, leaf capacity, precision). */
+TC_API int tc_potential_resident(tc_context *ctx, double theta, const double *xt, const double *yt, int m,
+                                 double *out, int device_ptrs);
+TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x0s, const double *y0s,
+                                  const double *hs, int nblocks, double *xr, double *yr, int device_ptrs);
+
+/* Times of the last call's phases (requires TC_OPT_PROFILE), interaction counters
+ * (requires TC_OPT_COUNTERS), and tree_info = {n, nnodes, ninternal, order}. */
+TC_API int tc_get_stats(tc_context *ctx, double *times_ms, unsigned long long *counters, int *tree_info);
+
+/* Copy the resident tree to host memory (any pointer may be NULL):
+ * keys[n], perm[n] (sorted -> input index), box[3] (xmin, ymin, ext),
+ * geom[4*nn] (xcom, ycom, r, mass), topo[4*nn] (s, e, child, level),
+ * exps[2*order*nn] ([Re a_1..a_p][Im a_1..a_p] per node). */
+TC_API int tc_tree_export(tc_context *ctx, int *keys, int *perm, double *box, double *geom, int *topo,
+                          double *exps);
+
+/* Device memory on the context's GPU, for callers that keep inputs resident in HBM
+ * (no PyTorch or other runtime needed).  Copies are synchronous. */
+TC_API void *tc_device_alloc(tc_context *ctx, size_t bytes);
+TC_API int tc_device_free(tc_context *ctx, void *ptr);
+TC_API int tc_copy_to_device(tc_context *ctx, void *dst, const void *src, size_t bytes);
+TC_API int tc_copy_to_host(tc_context *ctx, void *dst, const void *src, size_t bytes);
+
+/* Morton keys + stable radix sort of n points in device memory (the source-sort
+ * stage on its own): box[4] out (xmin, ymin, ext, scale), keys/perm out. */
+TC_API int tc_morton_sort_device(tc_context *ctx, const double *x, const double *y, int n, double *box,
+                                 unsigned int *keys, unsigned int *perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

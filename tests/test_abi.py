@@ -1,0 +1,99 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+the headers are valid C, and with no GPU every entry point fails loudly
+(there is no CPU fallback)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADERS = ["treecodes2d.h", "2d-treecodes-potential.h", "2d-treecodes-force.h"]
+HAVE_GPU = os.path.exists("/dev/kfd")
+
+
+def declared_symbols():
+    names = set()
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(tc_\w+|treecode_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_the_dropin_pair():
+    names = declared_symbols()
+    assert "treecode_potential" in names and "treecode_force_mrag" in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    L = ctypes.CDLL(pkg.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(pkg.EXPORTED) == sorted(s for s in declared_symbols())
+
+
+def test_compat_library_names(pkg):
+    for alias in ["lib2d-treecodes-potential.so", "lib2d-treecodes-force.so"]:
+        p = os.path.join(os.path.dirname(pkg.LIB_PATH), alias)
+        L = ctypes.CDLL(p)
+        assert hasattr(L, "treecode_potential") and hasattr(L, "treecode_force_mrag")
+
+
+def test_only_abi_symbols_are_exported(pkg):
+    out = subprocess.run(["nm", "-D", "--defined-only", pkg.LIB_PATH], capture_output=True, text=True).stdout
+    text_syms = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert text_syms == set(declared_symbols())
+
+
+@pytest.mark.parametrize("lang", ["c", "c++"])
+def test_headers_compile(tmp_path, lang):
+    src = tmp_path / ("t.c" if lang == "c" else "t.cpp")
+    src.write_text("\n".join('#include "%s"' % h for h in HEADERS) + "\nint main(void){return 0;}\n")
+    cc = "gcc" if lang == "c" else "g++"
+    r = subprocess.run([cc, "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(HAVE_GPU, reason="checks the no-GPU failure path")
+def test_no_gpu_fails_loudly(pkg):
+    with pytest.raises(pkg.TreecodeError):
+        pkg.Context()
+    x = np.linspace(0, 1, 10)
+    with pytest.raises(pkg.TreecodeError):
+        pkg.treecode_potential(0.5, x, x, x, x, x)
+    with pytest.raises(pkg.TreecodeError):
+        pkg.treecode_force_mrag(0.5, x, x, x, x[:1], x[:1], x[:1])
+    assert pkg.last_error()
+
+
+def test_missing_library_is_an_error(pkg, monkeypatch):
+    monkeypatch.setattr(pkg, "_lib", None)
+    monkeypatch.setattr(pkg, "LIB_PATH", os.path.join(ROOT, "does-not-exist.so"))
+    with pytest.raises(pkg.TreecodeError):
+        pkg.lib()
+
+
+def test_workloads_deterministic(wl):
+    a = wl.uniform_potential(1000)
+    b = wl.uniform_potential(1000)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+    xs, ys, vs, xt, yt = a
+    assert xs.min() >= 0 and xs.max() < 1 and vs.min() >= -1 and vs.max() < 1
+    assert xt is not xs and np.array_equal(xt, xs)
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(5000)
+    assert len(x0s) == 4096 and np.all(hs == 1.0 / 2048)
+    assert x0s.max() + 31 * hs[0] < 1.0
+    xs, ys, vs, _, _ = wl.mixed_plummer_potential(10000)
+    assert xs.min() >= 0 and xs.max() <= 1 and ys.min() >= 0 and ys.max() <= 1
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian16_force(10000, nblocks=64)
+    assert len(x0s) == 64 and set(np.unique(hs)) <= {1 / 1024, 1 / 2048, 1 / 4096}
+    reqs = wl.request_stream(200)
+    assert len(reqs) == 200 and all(r["arrival"] > 0 for r in reqs)
+    assert any(r["reuses_sources"] for r in reqs)

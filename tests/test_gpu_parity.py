@@ -1,0 +1,247 @@
+"""HIP path vs the CPU oracle, through the C ABI (libtreecodes2d.so).
+
+Bars: keys, permutation, topology and node geometry bit-exact; expansion
+coefficients within 1e-13 relative; FP64 potentials and velocities within
+1e-12 relative L2 with identical interaction counts; the FP32-bulk path within
+1e-6 relative L2 of the FP64 direct sum and of the FP64 treecode.
+"""
+import numpy as np
+import pytest
+
+from conftest import mrag_targets, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-6
+
+
+def _device_sort(pkg, ctx, x, y):
+    xd, yd = ctx.to_device(x), ctx.to_device(y)
+    n = len(x)
+    box = ctx.empty(4)
+    keys = ctx.empty(n, np.int32)
+    perm = ctx.empty(n, np.int32)
+    ctx.morton_sort_device(xd, yd, box, keys, perm)
+    ctx.synchronize()
+    return box.numpy(), keys.numpy(), perm.numpy()
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 100003])
+def test_morton_sort_bit_exact(pkg, oracle, wl, ctx64, n):
+    st = wl.Stream(100 + n)
+    x, y = st.uniform(n), st.uniform(n)
+    box, keys, perm = _device_sort(pkg, ctx64, x, y)
+    obox = oracle.bbox(x, y)
+    assert np.array_equal(box[:3], obox)
+    okeys = oracle.keys(x, y, obox)
+    operm, oks = oracle.sort(okeys)
+    assert np.array_equal(keys, oks)
+    assert np.array_equal(perm, operm)
+
+
+def test_morton_sort_duplicates_stable(pkg, oracle, wl, ctx64):
+    # many equal keys: the permutation must keep input order within a key (stable sort)
+    st = wl.Stream(5)
+    n = 50000
+    x = np.floor(st.uniform(n) * 8) / 8 + 0.01
+    y = np.floor(st.uniform(n) * 8) / 8 + 0.01
+    box, keys, perm = _device_sort(pkg, ctx64, x, y)
+    okeys = oracle.keys(x, y, oracle.bbox(x, y))
+    operm, oks = oracle.sort(okeys)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(keys, oks)
+
+
+def test_morton_sort_cell_edges(pkg, oracle, ctx64):
+    from golden_util import load_fixture
+    fx = load_fixture("morton_edges")
+    box, keys, perm = _device_sort(pkg, ctx64, fx["x"], fx["y"])
+    assert np.array_equal(box[:3], fx["box"])
+    assert np.array_equal(np.sort(keys), np.sort(fx["keys"]))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    assert np.array_equal(keys[inv], fx["keys"])
+
+
+def _compare_tree(ctx, ot, order):
+    t = ctx.tree()
+    assert np.array_equal(t["box"], ot.bbox)
+    assert np.array_equal(t["keys"], ot.keys)
+    assert np.array_equal(t["perm"], ot.perm)
+    nn = ot.nnodes
+    assert t["topo"].shape[0] == nn
+    assert np.array_equal(t["topo"][:, 0], ot.se[:, 0])
+    assert np.array_equal(t["topo"][:, 1], ot.se[:, 1])
+    assert np.array_equal(t["topo"][:, 2], ot.child)
+    assert np.array_equal(t["topo"][:, 3], ot.level)
+    # geometry decides every accept/open test: bit-exact
+    assert np.array_equal(t["geom"], ot.geom)
+    ex = t["exps"]
+    oe = ot.exp
+    scale = np.maximum(np.abs(oe).max(axis=(1, 2), keepdims=True), 1e-300)
+    assert np.max(np.abs(ex - oe) / scale) <= 1e-13
+
+
+@pytest.mark.parametrize("case", ["uniform", "gauss", "small", "tiny", "plummer"])
+def test_tree_potential_config(pkg, oracle, wl, ctx64, case):
+    if case == "uniform":
+        xs, ys, vs, xt, yt = wl.uniform_potential(65536)
+    elif case == "gauss":
+        xs, ys, vs, *_ = wl.gaussian4_force(50000)
+        xt, yt = xs.copy(), ys.copy()
+    elif case == "small":
+        xs, ys, vs, xt, yt = wl.uniform_potential(300, seed=3)
+    elif case == "tiny":
+        xs, ys, vs, xt, yt = wl.uniform_potential(1, seed=4)
+    else:
+        xs, ys, vs, xt, yt = wl.mixed_plummer_potential(40000, seed=9)
+    out = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    ot = oracle.Tree(xs, ys, vs, 512, 12)
+    _compare_tree(ctx64, ot, 12)
+    ref, cnt = ot.potential(0.5, xt, yt)
+    assert rel_l2(out, ref) <= FP64_TOL
+    st = ctx64.stats()["counters"]
+    assert st["p2p"] == cnt[0] and st["e2p"] == cnt[1]
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_potential_fp64_vs_oracle(pkg, oracle, wl, ctx64, theta):
+    xs, ys, vs, xt, yt = wl.uniform_potential(65536)
+    out = ctx64.potential(theta, xs, ys, vs, xt, yt)
+    ref, cnt = oracle.potential(theta, xs, ys, vs, xt, yt)
+    assert rel_l2(out, ref) <= FP64_TOL
+    st = ctx64.stats()["counters"]
+    assert (st["p2p"], st["e2p"]) == (cnt[0], cnt[1])
+
+
+def test_potential_targets_elsewhere(pkg, oracle, wl, ctx64):
+    # targets outside the source box, in random order, fewer than sources
+    st = wl.Stream(77)
+    xs, ys, vs = st.uniform(30000), st.uniform(30000), st.uniform(30000, -1, 1)
+    xt, yt = st.uniform(7777, -1.0, 2.0), st.uniform(7777, -0.5, 1.5)
+    out = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    ref, _ = oracle.potential(0.5, xs, ys, vs, xt, yt)
+    assert rel_l2(out, ref) <= FP64_TOL
+
+
+def test_potential_direct_sum_and_theta0(pkg, oracle, wl, ctx64):
+    xs, ys, vs, xt, yt = wl.uniform_potential(3000, seed=11)
+    idx = np.arange(len(xt), dtype=np.int32)
+    direct = oracle.direct_potential(xs, ys, vs, xt, yt, idx)
+    # theta = 0 never accepts: the treecode becomes the direct sum (P2P only)
+    out0 = ctx64.potential(0.0, xs, ys, vs, xt, yt)
+    assert rel_l2(out0, direct) <= 1e-13
+    out = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    assert rel_l2(out, direct) <= 1e-6
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_force_fp64_vs_oracle(pkg, oracle, wl, ctx64, theta):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(65536, nbx=16, h=1.0 / 512)
+    fx, fy = ctx64.force(theta, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, cnt = oracle.force_mrag(theta, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(fx, rx) <= FP64_TOL
+    assert rel_l2(fy, ry) <= FP64_TOL
+    st = ctx64.stats()["counters"]
+    assert (st["p2p"], st["e2p"], st["e2l"], st["l2p"]) == tuple(int(c) for c in cnt)
+
+
+def test_force_tree_and_mixed_blocks(pkg, oracle, wl, ctx64):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian16_force(60000, nblocks=96, seed=21)
+    fx, fy = ctx64.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    ot = oracle.Tree(xs, ys, vs, 192, 24)
+    _compare_tree(ctx64, ot, 24)
+    rx, ry, cnt = ot.force(0.5, x0s, y0s, hs)
+    assert rel_l2(fx, rx) <= FP64_TOL
+    assert rel_l2(fy, ry) <= FP64_TOL
+
+
+def test_force_direct_sum(pkg, oracle, wl, ctx64):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(20000, nbx=4, h=1.0 / 128, seed=31)
+    fx, fy = ctx64.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    tx, ty = mrag_targets(x0s, y0s, hs)
+    idx = np.arange(7, len(tx), 7, dtype=np.int32)
+    dx, dy = oracle.direct_force(xs, ys, vs, tx, ty, idx)
+    assert rel_l2(fx[idx], dx) <= 1e-10
+    assert rel_l2(fy[idx], dy) <= 1e-10
+    f0x, f0y = ctx64.force(0.0, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(f0x[idx], dx) <= 1e-13
+    assert rel_l2(f0y[idx], dy) <= 1e-13
+
+
+def test_dropin_abi(pkg, oracle, wl):
+    xs, ys, vs, xt, yt = wl.uniform_potential(20000, seed=12)
+    out = pkg.treecode_potential(0.5, xs, ys, vs, xt, yt)
+    ref, _ = oracle.potential(0.5, xs, ys, vs, xt, yt)
+    assert rel_l2(out, ref) <= FP64_TOL
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(20000, nbx=4, h=1.0 / 128, seed=13)
+    fx, fy = pkg.treecode_force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, _ = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(fx, rx) <= FP64_TOL and rel_l2(fy, ry) <= FP64_TOL
+
+
+def test_deterministic_and_resident(pkg, wl, ctx64):
+    xs, ys, vs, xt, yt = wl.uniform_potential(65536, seed=14)
+    a = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    b = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    assert np.array_equal(a, b)
+    c = ctx64.potential_resident(0.5, xt, yt)
+    assert np.array_equal(a, c)
+    # new targets against the resident tree == full solve with those targets
+    xt2, yt2 = xt[::3].copy(), yt[::3].copy()
+    d = ctx64.potential_resident(0.5, xt2, yt2)
+    e = ctx64.potential(0.5, xs, ys, vs, xt2, yt2)
+    assert np.array_equal(d, e)
+
+
+def test_empty_and_error_paths(pkg, wl, ctx64):
+    out = ctx64.potential(0.5, np.zeros(0), np.zeros(0), np.zeros(0), np.ones(5), np.ones(5))
+    assert np.array_equal(out, np.zeros(5))
+    xs, ys, vs, xt, yt = wl.uniform_potential(1000, seed=15)
+    xs[10] = np.nan
+    with pytest.raises(pkg.TreecodeError):
+        ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    # the context stays usable after an error
+    xs[10] = 0.5
+    ctx64.potential(0.5, xs, ys, vs, xt, yt)
+
+
+def test_coincident_sources(pkg, oracle, ctx64):
+    n = 2000
+    xs = np.full(n, 0.25)
+    ys = np.full(n, 0.75)
+    vs = np.linspace(-1, 1, n) + 0.3
+    xt = np.array([0.0, 0.25, 1.0, 0.3])
+    yt = np.array([0.0, 0.75, 1.0, 0.7])
+    out = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    ref, _ = oracle.potential(0.5, xs, ys, vs, xt, yt)
+    assert rel_l2(out, ref) <= FP64_TOL
+
+
+# ------------------------------------------------------------------ FP32 bulk
+def test_fp32_potential_accuracy(pkg, oracle, wl, ctx32, ctx64):
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(1 << 18, seed=16)
+    out32 = ctx32.potential(0.5, xs, ys, vs, xt, yt)
+    c32 = ctx32.stats()["counters"]
+    out64 = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    c64 = ctx64.stats()["counters"]
+    assert (c32["p2p"], c32["e2p"]) == (c64["p2p"], c64["e2p"])  # same interaction lists
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(len(xt), 4096, replace=False)).astype(np.int32)
+    direct = oracle.direct_potential(xs, ys, vs, xt, yt, idx)
+    assert rel_l2(out32[idx], direct) <= FP32_TOL
+    assert rel_l2(out32, out64) <= FP32_TOL
+
+
+def test_fp32_force_accuracy(pkg, oracle, wl, ctx32, ctx64):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(1 << 18, nbx=16, h=1.0 / 512, seed=17)
+    fx32, fy32 = ctx32.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    fx, fy = ctx64.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(fx32, fx) <= FP32_TOL
+    assert rel_l2(fy32, fy) <= FP32_TOL
+    tx, ty = mrag_targets(x0s, y0s, hs)
+    idx = np.arange(7, len(tx), 433, dtype=np.int32)
+    dx, dy = oracle.direct_force(xs, ys, vs, tx, ty, idx)
+    assert rel_l2(fx32[idx], dx) <= FP32_TOL
+    assert rel_l2(fy32[idx], dy) <= FP32_TOL

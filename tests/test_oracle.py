@@ -1,0 +1,376 @@
+"""Pins the CPU oracle (the checker) without trusting its own formulas.
+
+Each test checks an oracle function against something the mathematics or an
+independent implementation fixes: closed forms, brute force, numpy's stable
+sort, a reference-style recursive tree build, the translation identities of
+multipole expansions, and the FP64 direct sum.
+"""
+import bisect
+import math
+
+import numpy as np
+import pytest
+
+from conftest import mrag_targets, rel_l2
+from golden_util import load_fixture
+
+LMAX = 15
+
+
+def np_bbox(x, y):
+    # sort-sources.cpp:44-82 + upward-common.ispc:132 (max starts at 0, min at 1e13)
+    xmi = min(1e13, float(np.min(x)))
+    ymi = min(1e13, float(np.min(y)))
+    xma = max(0.0, float(np.max(x)))
+    yma = max(0.0, float(np.max(y)))
+    eps = 10000 * np.finfo(float).eps
+    ext = max(xma - xmi, yma - ymi) * (1 + 2 * eps)
+    return np.array([xmi - eps * ext, ymi - eps * ext, ext])
+
+
+def np_keys(x, y, box):
+    s = (1.0 / box[2]) * 32768.0
+    qx = np.clip(np.floor((x - box[0]) * s), 0, 32767).astype(np.int64)
+    qy = np.clip(np.floor((y - box[1]) * s), 0, 32767).astype(np.int64)
+    k = np.zeros_like(qx)
+    for b in range(15):
+        k |= ((qx >> b) & 1) << (2 * b)
+        k |= ((qy >> b) & 1) << (2 * b + 1)
+    return k.astype(np.int32)
+
+
+def test_bbox_closed_form(oracle, wl):
+    st = wl.Stream(1)
+    for lo, hi in [(0, 1), (-3, -1), (2, 5), (-1, 1)]:
+        x, y = st.uniform(1000, lo, hi), st.uniform(1000, lo * 0.5, hi * 2)
+        assert np.array_equal(oracle.bbox(x, y), np_bbox(x, y))
+    # all-negative coordinates: the box still reaches 0 (max accumulators start at 0)
+    b = oracle.bbox(np.array([-2.0, -1.0]), np.array([-1.0, -3.0]))
+    assert b[0] + b[2] > 0
+
+
+def test_bbox_degenerate(oracle):
+    b = oracle.bbox(np.full(5, 0.3), np.full(5, 0.3))
+    assert b[2] == 1.0  # all coincident: extent replaced by 1 (reference: undefined)
+
+
+def test_keys_closed_form(oracle, wl):
+    st = wl.Stream(2)
+    x, y = st.uniform(20000), st.uniform(20000)
+    box = oracle.bbox(x, y)
+    assert np.array_equal(oracle.keys(x, y, box), np_keys(x, y, box))
+
+
+def test_keys_cell_edge_fixture(oracle):
+    fx = load_fixture("morton_edges")
+    assert np.array_equal(oracle.bbox(fx["x"], fx["y"]), fx["box"])
+    k = oracle.keys(fx["x"], fx["y"], fx["box"])
+    assert np.array_equal(k, fx["keys"])
+    # the fixture separates the compiled quantisation form from the source form
+    assert int(fx["n_differ"]) > 0
+    assert not np.array_equal(k, fx["keys_source_form"])
+
+
+def test_sort_is_stable_argsort(oracle, wl):
+    st = wl.Stream(3)
+    k = (st.uniform(50000) * 2000).astype(np.int32) * 1000003 % (1 << 30)
+    k = k.astype(np.int32)
+    perm, ks = oracle.sort(k)
+    ref = np.argsort(k, kind="stable")
+    assert np.array_equal(perm, ref)
+    assert np.array_equal(ks, k[ref])
+
+
+def ref_style_tree(keys, n, cap):
+    """build_tree (upward.cpp:223-277) recursion with sequential fetch_and_add numbering,
+    std::lower_bound / std::upper_bound, children c=0..3 in order."""
+    nodes = {0: dict(s=0, e=n, l=0, mask=0, child=0)}
+    counter = [1]
+
+    def rec(nid):
+        nd = nodes[nid]
+        s, e, l, mask = nd["s"], nd["e"], nd["l"], nd["mask"]
+        if e - s <= cap or l + 1 > LMAX:
+            return
+        cb = counter[0]
+        counter[0] += 4
+        nd["child"] = cb
+        for c in range(4):
+            shift = 2 * (LMAX - l - 1)
+            key1 = mask | (c << shift)
+            key2 = key1 + (1 << shift) - 1
+            imin = s if c == 0 else bisect.bisect_left(keys, key1, s, e)
+            isup = e if c == 3 else bisect.bisect_right(keys, key2, s, e)
+            nodes[cb + c] = dict(s=imin, e=isup, l=l + 1, mask=key1, child=0)
+        for c in range(4):
+            rec(cb + c)
+
+    rec(0)
+    return nodes
+
+
+def canonical_bfs(nodes):
+    """Renumber breadth first: children of the k-th internal node (level order) at 1+4k+c."""
+    out, queue, nxt = [], [0], 1
+    while queue:
+        level_next = []
+        for old in queue:
+            nd = dict(nodes[old])
+            if nd["child"]:
+                kids = [nd["child"] + c for c in range(4)]
+                nd["child"] = nxt
+                nxt += 4
+                level_next += kids
+            out.append(nd)
+        queue = level_next
+    return out
+
+
+@pytest.mark.parametrize("gen,cap", [("uniform", 16), ("gauss", 8), ("coincident", 4)])
+def test_topology_vs_recursive_build(oracle, wl, gen, cap):
+    if gen == "uniform":
+        xs, ys, vs, _, _ = wl.uniform_potential(3000, seed=5)
+    elif gen == "gauss":
+        xs, ys, vs, *_ = wl.gaussian4_force(3000, seed=6)
+    else:
+        xs = np.concatenate([np.full(40, 0.3), wl.Stream(7).uniform(200)])
+        ys = np.concatenate([np.full(40, 0.6), wl.Stream(8).uniform(200)])
+        vs = np.ones(len(xs))
+    t = oracle.Tree(xs, ys, vs, cap, 12)
+    keys = [int(k) for k in t.keys]
+    ref = canonical_bfs(ref_style_tree(keys, len(keys), cap))
+    assert len(ref) == t.nnodes
+    assert [d["s"] for d in ref] == list(t.se[:, 0])
+    assert [d["e"] for d in ref] == list(t.se[:, 1])
+    assert [d["child"] for d in ref] == list(t.child)
+    assert [d["l"] for d in ref] == list(t.level)
+    if gen == "coincident":
+        assert t.level.max() == LMAX  # 40 coincident sources force level-15 leaves
+
+
+def test_tree_invariants_and_geometry(oracle, wl):
+    xs, ys, vs, *_ = wl.gaussian4_force(20000, seed=9)
+    cap = 64
+    t = oracle.Tree(xs, ys, vs, cap, 12)
+    xsrt, ysrt, vsrt = t.xyv[: t.n], t.xyv[t.n: 2 * t.n], t.xyv[2 * t.n:]
+    assert np.all(np.diff(t.keys) >= 0)
+    assert np.array_equal(xsrt, xs[t.perm])
+    xmin, ymin, ext = t.bbox
+    for i in range(t.nnodes):
+        s, e = t.se[i]
+        lvl = t.level[i]
+        h = ext / (1 << lvl)
+        x0, y0 = xmin + h * t.cell[i, 0], ymin + h * t.cell[i, 1]
+        if e > s:  # sources inside the cell (upward.cpp:240-243)
+            assert np.all((xsrt[s:e] >= x0) & (xsrt[s:e] < x0 + h))
+            assert np.all((ysrt[s:e] >= y0) & (ysrt[s:e] < y0 + h))
+        xc, yc, r, m = t.geom[i]
+        w = np.abs(vsrt[s:e])
+        if t.child[i]:
+            cb = t.child[i]
+            assert t.se[cb, 0] == s and t.se[cb + 3, 1] == e
+            assert all(t.se[cb + c, 1] == t.se[cb + c + 1, 0] for c in range(3))
+            assert e - s > cap and lvl < LMAX
+        else:
+            assert e - s <= cap or lvl == LMAX
+        if e > s:
+            com_x = math.fsum(xsrt[s:e] * w) / math.fsum(w)
+            com_y = math.fsum(ysrt[s:e] * w) / math.fsum(w)
+            assert abs(xc - com_x) <= 1e-13 * max(1.0, abs(com_x))
+            assert abs(yc - com_y) <= 1e-13 * max(1.0, abs(com_y))
+            assert abs(m - math.fsum(vsrt[s:e])) <= 1e-12 * max(1.0, math.fsum(w))
+            dist = np.sqrt((xsrt[s:e] - xc) ** 2 + (ysrt[s:e] - yc) ** 2).max()
+            if t.child[i]:
+                assert r >= dist * (1 - 1e-15)  # parent radius encloses its sources (upward.cpp:149-159)
+                assert r <= float(np.float32(1.4143)) * h * (1 + 1e-15)
+            else:
+                assert r == np.sqrt(((xsrt[s:e] - xc) ** 2 + (ysrt[s:e] - yc) ** 2).max())
+                assert r < 1.5 * h  # upward.cpp:87
+                assert x0 <= xc < x0 + h and y0 <= yc < y0 + h  # upward.cpp:89-93
+
+
+def test_leaf_sum_order_canonical(oracle, wl):
+    # leaf sums use 64 lane-strided partials + a pairwise tree; check bit-exactly with numpy
+    xs, ys, vs, _, _ = wl.uniform_potential(3000, seed=10)
+    t = oracle.Tree(xs, ys, vs, 600, 12)
+    n = t.n
+    xsrt, vsrt = t.xyv[:n], t.xyv[2 * n:]
+    for i in range(t.nnodes):
+        if t.child[i]:
+            continue
+        s, e = t.se[i]
+        p = np.zeros(64)
+        for lane in range(64):
+            acc = 0.0
+            for j in range(s + lane, e, 64):
+                acc = acc + vsrt[j]
+            p[lane] = acc
+        off = 32
+        while off >= 1:
+            p[:off] = p[:off] + p[off:2 * off]
+            off //= 2
+        assert t.geom[i, 3] == p[0]
+        pw = np.zeros(64)
+        px = np.zeros(64)
+        for lane in range(64):
+            aw = ax = 0.0
+            for j in range(s + lane, e, 64):
+                a = abs(vsrt[j])
+                aw = aw + a
+                ax = ax + xsrt[j] * a
+            pw[lane], px[lane] = aw, ax
+        off = 32
+        while off >= 1:
+            pw[:off] = pw[:off] + pw[off:2 * off]
+            px[:off] = px[:off] + px[off:2 * off]
+            off //= 2
+        assert t.w[i, 0] == pw[0] and t.w[i, 1] == px[0]
+        assert t.geom[i, 0] == px[0] / pw[0]
+
+
+def test_p2e_single_source_closed_form(oracle):
+    for d in [0.3 + 0.1j, -0.05 + 0.2j, 1e-3 - 2e-3j]:
+        v = -0.7
+        xs, ys = np.array([0.5 + d.real]), np.array([0.25 + d.imag])
+        a = oracle.p2e(xs, ys, np.array([v]), 0.5, 0.25, 24)
+        d = complex(xs[0] - 0.5, ys[0] - 0.25)  # the offset as the kernel sees it
+        k = np.arange(1, 25)
+        ref = -v * d ** k / k
+        # complex relative error: the power recursion rounds ~k times
+        assert np.all(np.abs(a - ref) <= 4e-16 * k * np.abs(ref))
+
+
+def test_e2e_equals_p2e_about_parent(oracle, wl):
+    xs, ys, vs, *_ = wl.gaussian4_force(8000, seed=11)
+    for p in (12, 24):
+        t = oracle.Tree(xs, ys, vs, 32, p)
+        n = t.n
+        xsrt, ysrt, vsrt = t.xyv[:n], t.xyv[n:2 * n], t.xyv[2 * n:]
+        checked = 0
+        for i in range(t.nnodes):
+            if not t.child[i]:
+                continue
+            s, e = t.se[i]
+            a_ref = oracle.p2e(xsrt[s:e], ysrt[s:e], vsrt[s:e], t.geom[i, 0], t.geom[i, 1], p)
+            a = t.exp[i, 0] + 1j * t.exp[i, 1]
+            # M2M is exact for finite p (compare coefficient-wise, scaled by r^k)
+            r = t.geom[i, 2]
+            sc = r ** np.arange(1, p + 1)
+            err = np.abs((a - a_ref) / sc).max() / max(np.abs(a_ref / sc).max(), 1e-300)
+            assert err <= 1e-11, (i, err)
+            checked += 1
+        assert checked > 10
+
+
+def test_e2p_converges_to_direct(oracle, wl):
+    st = wl.Stream(12)
+    m = 500
+    ang = st.uniform(m) * 2 * np.pi
+    rad = 0.1 * np.sqrt(st.uniform(m))
+    xs, ys = rad * np.cos(ang), rad * np.sin(ang)
+    vs = st.uniform(m, -1, 1)
+    xc = float(np.sum(xs * np.abs(vs)) / np.sum(np.abs(vs)))
+    yc = float(np.sum(ys * np.abs(vs)) / np.sum(np.abs(vs)))
+    r = float(np.sqrt(((xs - xc) ** 2 + (ys - yc) ** 2).max()))
+    z = complex(xc + 4 * r * 0.6, yc + 4 * r * 0.8)
+    direct = oracle.direct_potential(xs, ys, vs, np.array([z.real]), np.array([z.imag]), np.array([0], np.int32))[0]
+    errs = []
+    for p in (4, 8, 12, 16, 24, 32):
+        a = oracle.p2e(xs, ys, vs, xc, yc, p)
+        val = oracle.e2p_potential(complex(z.real - xc, z.imag - yc), float(np.sum(vs)), a)
+        errs.append(abs(val - direct))
+        bound = np.sum(np.abs(vs)) * (0.25 ** (p + 1)) / (p + 1) / (1 - 0.25) + 1e-12
+        assert errs[-1] <= bound, (p, errs[-1], bound)
+    assert errs[-1] < errs[0] * 1e-9  # down to the regularisation + rounding floor
+
+
+def test_force_e2p_e2l_l2p_vs_direct(oracle, wl):
+    st = wl.Stream(13)
+    m = 400
+    xs = 0.02 * st.normal(m) + 0.8
+    ys = 0.02 * st.normal(m) + 0.7
+    vs = st.uniform(m, -1, 1)
+    w = np.abs(vs)
+    xc, yc = float(np.sum(xs * w) / w.sum()), float(np.sum(ys * w) / w.sum())
+    p = 24
+    a = oracle.p2e(xs, ys, vs, xc, yc, p)
+    mass = float(vs.sum())
+    # local expansion about a brick centre far away, evaluated at points around it
+    bc = complex(0.2, 0.25)
+    b = oracle.e2l(complex(xc - bc.real, yc - bc.imag), mass, a)
+    pts = [bc + complex(dx, dy) for dx in (-0.01, 0.0, 0.007) for dy in (-0.006, 0.01)]
+    tx = np.array([q.real for q in pts])
+    ty = np.array([q.imag for q in pts])
+    dx, dy = oracle.direct_force(xs, ys, vs, tx, ty, np.arange(len(pts), dtype=np.int32))
+    for k, q in enumerate(pts):
+        fx, fy = oracle.e2p_force(complex(q.real - xc, q.imag - yc), mass, a)
+        gx, gy = oracle.l2p(complex(q.real - bc.real, q.imag - bc.imag), b)
+        scale = math.hypot(dx[k], dy[k])
+        assert abs(fx - dx[k]) <= 1e-12 * scale and abs(fy - dy[k]) <= 1e-12 * scale
+        assert abs(gx - dx[k]) <= 1e-10 * scale and abs(gy - dy[k]) <= 1e-10 * scale
+
+
+def test_l2p_is_polynomial_derivative(oracle, wl):
+    st = wl.Stream(14)
+    p = 12
+    b = st.uniform(p + 1, -1, 1) + 1j * st.uniform(p + 1, -1, 1)
+    z = complex(0.3, -0.2)
+    fx, fy = oracle.l2p(z, b)
+    deriv = sum(l * b[l] * z ** (l - 1) for l in range(1, p + 1))
+    assert abs(fx - deriv.real) <= 1e-14 and abs(fy + deriv.imag) <= 1e-14
+
+
+def test_potential_theta0_is_direct_sum(oracle, wl):
+    xs, ys, vs, xt, yt = wl.uniform_potential(2000, seed=15)
+    out, cnt = oracle.potential(0.0, xs, ys, vs, xt, yt)
+    ref = oracle.direct_potential(xs, ys, vs, xt, yt, np.arange(len(xt), dtype=np.int32))
+    assert rel_l2(out, ref) <= 1e-13
+    assert cnt[0] == len(xs) * len(xt) and cnt[1] == 0
+
+
+def test_potential_accuracy_vs_direct(oracle, wl):
+    xs, ys, vs, xt, yt = wl.uniform_potential(20000, seed=16)
+    idx = np.arange(7, len(xt), 433, dtype=np.int32)  # main.cpp:203-204 sampling
+    ref = oracle.direct_potential(xs, ys, vs, xt, yt, idx)
+    for theta, tol in [(0.3, 1e-9), (0.5, 1e-6)]:
+        out, _ = oracle.potential(theta, xs, ys, vs, xt, yt)
+        assert rel_l2(out[idx], ref) <= tol
+
+
+def test_force_theta0_and_accuracy(oracle, wl):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(5000, nbx=2, h=1.0 / 64, seed=17)
+    tx, ty = mrag_targets(x0s, y0s, hs)
+    idx = np.arange(7, len(tx), 13, dtype=np.int32)
+    dx, dy = oracle.direct_force(xs, ys, vs, tx, ty, idx)
+    fx, fy, cnt = oracle.force_mrag(0.0, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(fx[idx], dx) <= 1e-13 and rel_l2(fy[idx], dy) <= 1e-13
+    assert cnt[0] == len(xs) * len(tx) and cnt[1] == 0 and cnt[2] == 0
+    fx, fy, cnt = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(fx[idx], dx) <= 1e-10 and rel_l2(fy[idx], dy) <= 1e-10
+    assert cnt[2] > 0 and cnt[3] == len(tx)
+
+
+def test_output_layout_mrag(oracle, wl):
+    # point (ix, iy) of block b is written at ix + 32*(iy + 32*b) (README.md:36-37)
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(3000, nbx=2, h=1.0 / 64, seed=18)
+    fx, fy, _ = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    b, ix, iy = 3, 5, 17
+    t = np.array([x0s[b] + ix * hs[b]]), np.array([y0s[b] + iy * hs[b]])
+    dx, dy = oracle.direct_force(xs, ys, vs, t[0], t[1], np.array([0], np.int32))
+    k = ix + 32 * (iy + 32 * b)
+    assert abs(fx[k] - dx[0]) <= 1e-9 * abs(dx[0]) and abs(fy[k] - dy[0]) <= 1e-9 * abs(dy[0])
+
+
+def test_binomials_exact(oracle):
+    fx = load_fixture("binomials")
+    for n in range(64):
+        for k in range(n + 1):
+            exact = (int(fx["hi"][n, k]) << 32) | int(fx["lo"][n, k])
+            assert exact == math.comb(n, k)
+            assert oracle.binomial(n, k) == float(exact)
+
+
+def test_bc_constants_round_to_nearest(oracle):
+    # -1/l printed by `bc -l` with 20 digits parses to the nearest double of -1/l
+    fx = load_fixture("neg_inv")
+    assert np.array_equal(fx["bc20"], fx["nearest"])
