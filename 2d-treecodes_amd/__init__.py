@@ -6,7 +6,7 @@ The reference exposes two C functions (README.md:5-25):
 `treecode_potential` / `treecode_force_mrag` below call exactly those C
 symbols (host numpy arrays in, results out).  `Context` exposes the
 extensions: precision, order and leaf options, device-resident buffers
-(torch tensors on the GPU), phase timers, interaction counters and the
+(DeviceArray buffers in HBM), phase timers, interaction counters and the
 resident tree.
 
 There is no CPU fallback: importing works without a GPU, but every call
@@ -30,11 +30,23 @@ EXPORTED = [
     "tc_context_destroy", "tc_set_option", "tc_context_stream", "tc_potential", "tc_force_mrag",
     "tc_synchronize", "tc_potential_resident", "tc_force_mrag_resident", "tc_get_stats", "tc_tree_export",
     "tc_morton_sort_device", "tc_device_alloc", "tc_device_free", "tc_copy_to_device", "tc_copy_to_host",
+    "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
+    "tc_server_wait", "tc_server_stats",
 ]
 
 
 class TreecodeError(RuntimeError):
     pass
+
+
+class TcRequest(ctypes.Structure):
+    """struct tc_request of treecodes2d.h"""
+    _fields_ = [("kind", ctypes.c_int), ("precision", ctypes.c_int), ("theta", ctypes.c_double),
+                ("xs", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("vs", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("xt", ctypes.c_void_p), ("yt", ctypes.c_void_p), ("m", ctypes.c_int),
+                ("x0s", ctypes.c_void_p), ("y0s", ctypes.c_void_p), ("hs", ctypes.c_void_p),
+                ("nblocks", ctypes.c_int), ("out0", ctypes.c_void_p), ("out1", ctypes.c_void_p),
+                ("source_id", ctypes.c_longlong)]
 
 
 _lib = None
@@ -77,6 +89,15 @@ def lib():
     L.tc_device_free.argtypes = [_P, _P]
     L.tc_copy_to_device.argtypes = [_P, _P, _P, ctypes.c_size_t]
     L.tc_copy_to_host.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    L.tc_server_create.argtypes = [it, it, dbl]
+    L.tc_server_create.restype = _P
+    L.tc_server_destroy.argtypes = [_P]
+    L.tc_server_submit.argtypes = [_P, ctypes.POINTER(TcRequest)]
+    L.tc_server_submit.restype = ctypes.c_longlong
+    L.tc_server_wait.argtypes = [_P, ctypes.c_longlong, _D]
+    L.tc_server_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong)]
+    L.tc_direct_potential.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P]
+    L.tc_direct_force.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P, _P]
     _lib = L
     return L
 
@@ -228,11 +249,11 @@ class Context:
 
     def stats(self):
         t = np.zeros(4)
-        c = (ctypes.c_ulonglong * 4)()
+        c = (ctypes.c_ulonglong * 5)()
         info = np.zeros(4, np.int32)
         self._check(lib().tc_get_stats(self.h, _dp(t), c, info.ctypes.data_as(_I)))
         return {"times_ms": {"build": t[0], "tsort": t[1], "eval": t[2], "total": t[3]},
-                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3]},
+                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3], "p2p_slots": c[4]},
                 "n": int(info[0]), "nnodes": int(info[1]), "ninternal": int(info[2]), "order": int(info[3])}
 
     def tree(self):
@@ -256,6 +277,20 @@ class Context:
         a = np.ascontiguousarray(a, dtype=dtype)
         return DeviceArray(self, a.size, dtype).upload(a)
 
+    def direct_potential(self, xs, ys, vs, xt, yt, idx):
+        """FP64 direct sum at targets idx on the GPU (arguments: DeviceArrays)."""
+        out = DeviceArray(self, len(idx), np.float64)
+        self._check(lib().tc_direct_potential(self.h, xs.ptr, ys.ptr, vs.ptr, len(xs), xt.ptr, yt.ptr, idx.ptr,
+                                              len(idx), out.ptr))
+        return out.numpy()
+
+    def direct_force(self, xs, ys, vs, xt, yt, idx):
+        ox = DeviceArray(self, len(idx), np.float64)
+        oy = DeviceArray(self, len(idx), np.float64)
+        self._check(lib().tc_direct_force(self.h, xs.ptr, ys.ptr, vs.ptr, len(xs), xt.ptr, yt.ptr, idx.ptr,
+                                          len(idx), ox.ptr, oy.ptr))
+        return ox.numpy(), oy.numpy()
+
     def morton_sort_device(self, x, y, box, keys, perm):
         self._check(lib().tc_morton_sort_device(self.h, _ptr(x), _ptr(y), len(x), _ptr(box), _ptr(keys),
                                                 _ptr(perm)))
@@ -263,6 +298,68 @@ class Context:
     def close(self):
         if getattr(self, "h", None):
             lib().tc_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server (tc_server): slots x (stream + thread + resident tree)."""
+
+    def __init__(self, device=0, slots=4, hbm_budget_gib=64.0):
+        self.h = lib().tc_server_create(device, slots, float(hbm_budget_gib))
+        if not self.h:
+            raise TreecodeError("tc_server_create failed: " + last_error())
+        self._keep = {}
+
+    def submit_potential(self, theta, xs, ys, vs, xt, yt, precision=FP64, source_id=0):
+        xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
+        out = np.empty(len(xt))
+        r = TcRequest(kind=0, precision=precision, theta=theta, xs=xs.ctypes.data, ys=ys.ctypes.data,
+                      vs=vs.ctypes.data, n=len(xs), xt=xt.ctypes.data, yt=yt.ctypes.data, m=len(xt),
+                      out0=out.ctypes.data, source_id=source_id)
+        return self._submit(r, (xs, ys, vs, xt, yt), (out,))
+
+    def submit_force(self, theta, xs, ys, vs, x0s, y0s, hs, precision=FP64, source_id=0):
+        xs, ys, vs, x0s, y0s, hs = map(_f64, (xs, ys, vs, x0s, y0s, hs))
+        xr = np.empty(len(x0s) * 1024)
+        yr = np.empty(len(x0s) * 1024)
+        r = TcRequest(kind=1, precision=precision, theta=theta, xs=xs.ctypes.data, ys=ys.ctypes.data,
+                      vs=vs.ctypes.data, n=len(xs), x0s=x0s.ctypes.data, y0s=y0s.ctypes.data, hs=hs.ctypes.data,
+                      nblocks=len(x0s), out0=xr.ctypes.data, out1=yr.ctypes.data, source_id=source_id)
+        return self._submit(r, (xs, ys, vs, x0s, y0s, hs), (xr, yr))
+
+    def _submit(self, req, inputs, outputs):
+        t = lib().tc_server_submit(self.h, ctypes.byref(req))
+        if t < 0:
+            raise TreecodeError("request does not fit the server's HBM budget")
+        self._keep[t] = (inputs, outputs)
+        return t
+
+    def wait(self, ticket):
+        """-> (outputs, {queue_ms, service_ms, latency_ms, reused, slot})"""
+        times = np.zeros(5)
+        st = lib().tc_server_wait(self.h, ticket, _dp(times))
+        inputs, outputs = self._keep.pop(ticket)
+        if st != 0:
+            raise TreecodeError("request %d failed (status %d)" % (ticket, st))
+        info = {"queue_ms": times[0], "service_ms": times[1], "latency_ms": times[2], "reused": bool(times[3]),
+                "slot": int(times[4])}
+        return (outputs[0] if len(outputs) == 1 else outputs), info
+
+    def stats(self):
+        out = (ctypes.c_longlong * 7)()
+        lib().tc_server_stats(self.h, out)
+        keys = ["completed", "reused", "failed", "max_inflight", "peak_bytes", "budget_bytes", "queued"]
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tc_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

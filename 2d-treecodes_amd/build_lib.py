@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtreecodes2d.so")
 ALIASES = ["lib2d-treecodes-potential.so", "lib2d-treecodes-force.so"]
-SOURCES = ["tc_build.hip", "tc_eval.hip", "tc_api.cpp"]
+SOURCES = ["tc_build.hip", "tc_eval.hip", "tc_api.cpp", "tc_serve.cpp"]
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
 
 CFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

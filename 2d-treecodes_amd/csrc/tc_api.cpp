@@ -34,7 +34,7 @@ struct tc_context {
     DevBuf io, cnt;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_valid = false, eval_has_tsort = false;
-    unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0};
+    unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -481,6 +481,25 @@ TC_API int tc_copy_to_host(tc_context *c, void *dst, const void *src, size_t byt
 {
     return guarded(c, [&] {
         TC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        TC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+TC_API int tc_direct_potential(tc_context *c, const double *xs, const double *ys, const double *vs, int n,
+                               const double *xt, const double *yt, const int *idx, int nidx, double *out)
+{
+    return guarded(c, [&] {
+        tc_direct_sums(xs, ys, vs, n, xt, yt, idx, nidx, 0, out, nullptr, c->stream);
+        TC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+TC_API int tc_direct_force(tc_context *c, const double *xs, const double *ys, const double *vs, int n,
+                           const double *xt, const double *yt, const int *idx, int nidx, double *outx,
+                           double *outy)
+{
+    return guarded(c, [&] {
+        tc_direct_sums(xs, ys, vs, n, xt, yt, idx, nidx, 1, outx, outy, c->stream);
         TC_HIP(hipStreamSynchronize(c->stream));
     });
 }

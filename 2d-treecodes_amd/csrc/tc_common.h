@@ -29,7 +29,9 @@ struct alignas(16) TcNode {
 };
 static_assert(sizeof(TcNode) == 64, "TcNode must stay 64 B (one s_load_dwordx16)");
 
-enum TcCounter { TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_NCOUNTERS = 4 };
+// P2P_SLOTS = 64 x leaf size per wave-level leaf visit: the lane slots the masked
+// P2P loop occupies (P2P / P2P_SLOTS is its SIMD efficiency)
+enum TcCounter { TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_CNT_P2P_SLOTS = 4, TC_NCOUNTERS = 5 };
 
 struct TcError : std::runtime_error {
     int code;
@@ -200,6 +202,9 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
                    hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n, const double *xt,
+                    const double *yt, const int *idx, int nidx, int force, double *out0, double *out1,
+                    hipStream_t st);
 void tc_scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total_d, TcScratch &ws,
                            hipStream_t st);
 void tc_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);

@@ -42,6 +42,47 @@ TC_DEV double rcp64(double d)
     return fma(r, e, r);
 }
 
+// Natural log of a positive normal double to about 2 ulp (the P2P argument is
+// r^2 + 10 eps >= 2.2e-15).  x = 2^k m with m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, and
+// 2 atanh(s) = 2s (1 + z/3 + z^2/5 + ... + z^10/21) with z = s^2 <= 0.0295,
+// the Taylor series truncated where the next term is below 2^-57 relative.
+// k ln 2 uses ln 2 split into a 32-bit head (k * head exact) and a tail.
+TC_DEV double log64(double x)
+{
+#pragma clang fp contract(fast)
+    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+    int k = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654757) {
+        m = m + m;
+        k -= 1;
+    }
+    const double f = m - 1.0;  // exact (Sterbenz)
+    const double d = m + 1.0;
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    double q = f * r;
+    q = fma(r, fma(-q, d, f), q);  // s = f / d to ~0.5 ulp
+    const double z = q * q;
+    double P = 1.0 / 21.0;
+    P = fma(P, z, 1.0 / 19.0);
+    P = fma(P, z, 1.0 / 17.0);
+    P = fma(P, z, 1.0 / 15.0);
+    P = fma(P, z, 1.0 / 13.0);
+    P = fma(P, z, 1.0 / 11.0);
+    P = fma(P, z, 1.0 / 9.0);
+    P = fma(P, z, 1.0 / 7.0);
+    P = fma(P, z, 1.0 / 5.0);
+    P = fma(P, z, 1.0 / 3.0);
+    const double q2 = q + q;
+    const double logm = fma(q2 * z, P, q2);
+    const double kd = (double)k;
+    return fma(kd, 0.6931471803691238, fma(kd, 1.9082149292705877e-10, logm));
+}
+
 // ------------------------------------------------------------------ potential kernels
 // potential_e2p (potential-kernels.m4:59-89) for one expansion
 template <int P>
@@ -61,7 +102,7 @@ TC_DEV double e2p_pot64(double rz, double iz, double mass, const double *__restr
         ii = in;
         sum += ex[j] * ri - ex[P + j] * ii;
     }
-    return mass * log(r2) * 0.5 + sum;
+    return mass * log64(r2) * 0.5 + sum;
 }
 
 // FP32 E2P on scaled coefficients b_j = a_j / s^j:  Re sum b_j (s/z)^j.
@@ -106,7 +147,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     if (!live)
         return;
     double acc = 0;
-    unsigned long long np2p = 0, ne2p = 0;
+    unsigned long long np2p = 0, ne2p = 0, nslots = 0;
     int sp = 0;
     s_node[wid][0] = 0;
     s_mask[wid][0] = live;
@@ -139,6 +180,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         if (nd.child == 0) {
             const bool act = (omask >> lane) & 1ull;
             np2p += (unsigned long long)__popcll(omask) * (unsigned long long)(nd.e - nd.s);
+            nslots += 64ull * (unsigned long long)(nd.e - nd.s);
             if (F32) {
                 const float tx = (float)(x - nd.xcom), ty = (float)(y - nd.ycom);
                 float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
@@ -150,29 +192,34 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                     tc_wave_sync();
                     if (act) {
 #pragma clang fp contract(fast)
-                        float s32 = 0;
+                        float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
                         int j = 0;
-                        for (; j + 2 <= cnt; j += 2) {
-                            const float4 q0 = tile[j], q1 = tile[j + 1];
-                            const float ax = tx - q0.x, ay = ty - q0.y;
-                            const float bx = tx - q1.x, by = ty - q1.y;
-                            const float da = ax * ax + (ay * ay + EPS_REG32);
-                            const float db = bx * bx + (by * by + EPS_REG32);
-                            s32 += q0.z * fast_log2(da);
-                            s32 += q1.z * fast_log2(db);
+                        for (; j + 4 <= cnt; j += 4) {
+                            const float4 q0 = tile[j], q1 = tile[j + 1], q2 = tile[j + 2], q3 = tile[j + 3];
+                            float ax = tx - q0.x, ay = ty - q0.y;
+                            s0 += q0.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                            ax = tx - q1.x;
+                            ay = ty - q1.y;
+                            s1 += q1.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                            ax = tx - q2.x;
+                            ay = ty - q2.y;
+                            s2 += q2.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                            ax = tx - q3.x;
+                            ay = ty - q3.y;
+                            s3 += q3.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
                         }
                         for (; j < cnt; ++j) {
                             const float4 q = tile[j];
                             const float ax = tx - q.x, ay = ty - q.y;
-                            s32 += q.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                            s0 += q.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
                         }
-                        leafsum += (double)s32;
+                        leafsum += (double)((s0 + s1) + (s2 + s3));
                     }
                     tc_wave_sync();
                 }
                 acc += leafsum * (double)LN2_HALF;
             } else {
-                double leafsum = 0;
+                double leafsum = 0, leafsum2 = 0;
                 for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
                     const int cnt = min(64, nd.e - b0);
                     if (lane < cnt)
@@ -180,15 +227,23 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                     tc_wave_sync();
                     if (act) {
 #pragma clang fp contract(fast)
-                        for (int j = 0; j < cnt; ++j) {
+                        int j = 0;
+                        for (; j + 2 <= cnt; j += 2) {
+                            const double4 q0 = s_src[wid][j], q1 = s_src[wid][j + 1];
+                            const double ax = x - q0.x, ay = y - q0.y;
+                            const double bx = x - q1.x, by = y - q1.y;
+                            leafsum += log64(ax * ax + ay * ay + EPS_REG) * q0.z;
+                            leafsum2 += log64(bx * bx + by * by + EPS_REG) * q1.z;
+                        }
+                        if (j < cnt) {
                             const double4 q = s_src[wid][j];
                             const double ax = x - q.x, ay = y - q.y;
-                            leafsum += log(ax * ax + ay * ay + EPS_REG) * q.z;
+                            leafsum += log64(ax * ax + ay * ay + EPS_REG) * q.z;
                         }
                     }
                     tc_wave_sync();
                 }
-                acc += leafsum * 0.5;
+                acc += (leafsum + leafsum2) * 0.5;
             }
         } else {
             // children pushed in order 0..3, popped 3..0 (treecode-potential.cpp:76-79)
@@ -204,6 +259,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     if (counters && lane == 0) {
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
+        atomicAdd(&counters[TC_CNT_P2P_SLOTS], nslots);
     }
 }
 
@@ -440,6 +496,48 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
         atomicAdd(&counters[TC_CNT_E2L], ne2l);
         atomicAdd(&counters[TC_CNT_L2P], 64ull);
+        atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
+    }
+}
+
+// ------------------------------------------------------------------ direct sums
+// The FP64 O(N) sums of main.cpp:206-245 at selected targets, one block per
+// target: potential sum_j v_j log(sqrt(r^2 + 10 eps)), force sum_j v_j d/(r^2 + 10 eps).
+__global__ __launch_bounds__(256) void k_direct(const double *__restrict__ xs, const double *__restrict__ ys,
+                                                const double *__restrict__ vs, int n,
+                                                const double *__restrict__ xt, const double *__restrict__ yt,
+                                                const int *__restrict__ idx, int force, double *__restrict__ out0,
+                                                double *__restrict__ out1)
+{
+    __shared__ double s0[4], s1[4];
+    const int k = blockIdx.x;
+    const int t = idx[k];
+    const double xd = xt[t], yd = yt[t];
+    double a = 0, b = 0;
+    if (force) {
+        for (int j = threadIdx.x; j < n; j += 256) {
+            const double xr = xd - xs[j], yr = yd - ys[j];
+            const double f = vs[j] / (xr * xr + yr * yr + EPS_REG);
+            a += xr * f;
+            b += yr * f;
+        }
+    } else {
+        for (int j = threadIdx.x; j < n; j += 256) {
+            const double xr = xd - xs[j], yr = yd - ys[j];
+            a += log(sqrt(xr * xr + yr * yr + EPS_REG)) * vs[j];
+        }
+    }
+    a = tc_wave_tree_sum(a);
+    b = tc_wave_tree_sum(b);
+    if (tc_lane() == 0) {
+        s0[threadIdx.x >> 6] = a;
+        s1[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out0[k] = (s0[0] + s0[1]) + (s0[2] + s0[3]);
+        if (force)
+            out1[k] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
     }
 }
 
@@ -494,6 +592,16 @@ void tc_upload_binomials_eval(hipStream_t st)
     TC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_binomT), tab.data(), sizeof(double) * tab.size(), 0,
                                   hipMemcpyHostToDevice, st));
     TC_HIP(hipStreamSynchronize(st));
+}
+
+void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n, const double *xt,
+                    const double *yt, const int *idx, int nidx, int force, double *out0, double *out1,
+                    hipStream_t st)
+{
+    if (nidx <= 0)
+        return;
+    k_direct<<<nidx, 256, 0, st>>>(xs, ys, vs, n, xt, yt, idx, force, out0, out1);
+    TC_HIP(hipGetLastError());
 }
 
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,

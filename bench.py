@@ -1,0 +1,298 @@
+#!/usr/bin/env python3
+"""Benchmark of the treecode hot path on one MI355X.
+
+Default workload (the precision form of treecode_potential): N = M = 2^24,
+half uniform in [0,1]^2 and half Plummer (a = 0.02) about (0.5, 0.5)
+truncated to [0,1]^2, values uniform in [-1, 1), theta = 0.5, order 12,
+leaf capacity 512.  One step = one whole solve (bbox, Morton sort, tree
+build, upward pass, target sort, traversal) with inputs already resident
+in HBM.  `value` is the FP32-bulk rate in Mtargets/s; the plain FP64 GPU
+path is timed beside it, and both are checked against the FP64 direct sum
+on 4096 random targets (GPU direct-sum kernel).
+
+Other workloads (--config): c1 (potential, 65,536 uniform), c2 (force_mrag,
+1,048,576 sources in 4 Gaussians, 4,096 blocks), c5 (force_mrag, 8,388,608
+sources in 16 Gaussians, 16,384 blocks of mixed spacing).
+
+The JSON line also carries `roofline` (the traversal kernel's algorithmic
+FLOP/s from deterministic interaction counts over its HIP-event duration)
+and `cpu_baseline` (the C restatement timed on this host's cores on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+pkg = __graft_entry__.load_package()
+from treecodes2d_amd import workloads  # noqa: E402
+
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X vector FP32 / FP64 (non-packed FP32 issues at the FP64 rate)
+
+
+def flops_potential(cnt, p):
+    # per-interaction model: P2P 9 flops per pair (log = 1), E2P 10p + 6 per (target, node)
+    return 9.0 * cnt["p2p"] + (10.0 * p + 6.0) * cnt["e2p"]
+
+
+def flops_force(cnt, p):
+    # P2P 11 per pair, E2P 16p + 11 per (target, node), E2L 4p^2 + 21p - 1 per (brick, node), L2P 16p - 12
+    return (11.0 * cnt["p2p"] + (16.0 * p + 11.0) * cnt["e2p"] + (4.0 * p * p + 21.0 * p - 1.0) * cnt["e2l"]
+            + (16.0 * p - 12.0) * cnt["l2p"])
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(cfg, n):
+    if cfg == "c1":
+        n = n or 65536
+        xs, ys, vs, xt, yt = workloads.uniform_potential(n)
+        return "potential", dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt), "potential uniform N=M=%d" % n
+    if cfg == "c3":
+        n = n or (1 << 24)
+        xs, ys, vs, xt, yt = workloads.mixed_plummer_potential(n)
+        return "potential", dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt), \
+            "potential half-uniform half-Plummer(a=0.02) N=M=%d" % n
+    if cfg == "c2":
+        n = n or (1 << 20)
+        xs, ys, vs, x0s, y0s, hs = workloads.gaussian4_force(n)
+        return "force", dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs), \
+            "force_mrag 4-Gaussian N=%d, 4096 blocks of 32x32 (h=1/2048)" % n
+    if cfg == "c5":
+        n = n or (1 << 23)
+        xs, ys, vs, x0s, y0s, hs = workloads.gaussian16_force(n, nblocks=16384)
+        return "force", dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs), \
+            "force_mrag 16-Gaussian N=%d, 16384 blocks of 32x32, h in {1/1024,1/2048,1/4096}" % n
+    raise SystemExit("unknown config " + cfg)
+
+
+class GpuCase:
+    def __init__(self, kind, data, precision, theta, order, leaf):
+        self.kind, self.theta = kind, theta
+        kw = dict(precision=precision, profile=True)
+        if kind == "potential":
+            kw.update(pot_order=order or 12, pot_leaf=leaf or 512)
+        else:
+            kw.update(force_order=order or 24, force_leaf=leaf or 192)
+        self.ctx = pkg.Context(**kw)
+        self.order = order or (12 if kind == "potential" else 24)
+        self.d = {k: self.ctx.to_device(v) for k, v in data.items()}
+        if kind == "potential":
+            self.m = len(data["xt"])
+            self.out = self.ctx.empty(self.m)
+        else:
+            self.m = len(data["x0s"]) * 1024
+            self.xr = self.ctx.empty(self.m)
+            self.yr = self.ctx.empty(self.m)
+
+    def step(self):
+        d = self.d
+        if self.kind == "potential":
+            self.ctx.potential(self.theta, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=self.out, device=True)
+        else:
+            self.ctx.force(self.theta, d["xs"], d["ys"], d["vs"], d["x0s"], d["y0s"], d["hs"], xr=self.xr,
+                           yr=self.yr, device=True)
+        self.ctx.synchronize()
+
+    def timed(self, steps, warmup):
+        for _ in range(warmup):
+            self.step()
+        phases = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+            phases.append(self.ctx.stats()["times_ms"])
+        dt = (time.perf_counter() - t0) / steps
+        avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+        return dt, avg
+
+    def counters(self):
+        self.ctx.set(pkg.OPT_COUNTERS, 1)
+        self.step()
+        st = self.ctx.stats()
+        self.ctx.set(pkg.OPT_COUNTERS, 0)
+        return st
+
+    def result(self):
+        if self.kind == "potential":
+            return self.out.numpy()
+        return self.xr.numpy(), self.yr.numpy()
+
+
+def sampled_error(case32, case64, kind, data, nsample=4096, seed=2024):
+    rng = np.random.default_rng(seed)
+    ctx = case64.ctx
+    if kind == "potential":
+        m = len(data["xt"])
+        idx = np.sort(rng.choice(m, min(nsample, m), replace=False)).astype(np.int32)
+        didx = ctx.to_device(idx, np.int32)
+        d = case64.d
+        ref = ctx.direct_potential(d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], didx)
+        out = {}
+        for name, c in (("f32", case32), ("f64", case64)):
+            r = c.result()[idx]
+            out[name] = float(np.linalg.norm(r - ref) / np.linalg.norm(ref))
+        return out
+    x0s, y0s, hs = data["x0s"], data["y0s"], data["hs"]
+    m = len(x0s) * 1024
+    idx = np.sort(rng.choice(m, min(nsample, m), replace=False)).astype(np.int32)
+    b = idx // 1024
+    tx = x0s[b] + (idx % 32) * hs[b]
+    ty = y0s[b] + ((idx // 32) % 32) * hs[b]
+    dtx, dty = ctx.to_device(tx), ctx.to_device(ty)
+    d = case64.d
+    rx, ry = ctx.direct_force(d["xs"], d["ys"], d["vs"], dtx, dty, ctx.to_device(np.arange(len(idx), dtype=np.int32),
+                                                                                  np.int32))
+    out = {}
+    for name, c in (("f32", case32), ("f64", case64)):
+        fx, fy = c.result()
+        num = np.sqrt(np.sum((fx[idx] - rx) ** 2) + np.sum((fy[idx] - ry) ** 2))
+        out[name] = float(num / np.sqrt(np.sum(rx ** 2) + np.sum(ry ** 2)))
+    return out
+
+
+def cpu_baseline(kind, data, theta, order, leaf, budget_s):
+    """The C restatement (oracle) on this host: full tree build + a timed sample of targets,
+    extrapolated to the whole workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = oracle.max_threads()
+    t0 = time.perf_counter()
+    if kind == "potential":
+        tree = oracle.Tree(data["xs"], data["ys"], data["vs"], leaf or 512, order or 12)
+    else:
+        tree = oracle.Tree(data["xs"], data["ys"], data["vs"], leaf or 192, order or 24)
+    t_build = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    if kind == "potential":
+        m = len(data["xt"])
+        k = 2048
+        while True:
+            idx = rng.choice(m, min(k, m), replace=False)
+            t1 = time.perf_counter()
+            tree.potential(theta, data["xt"][idx], data["yt"][idx], nthreads=threads)
+            te = time.perf_counter() - t1
+            if te > 0.25 * budget_s or k >= m:
+                break
+            k = int(min(m, k * max(2.0, 0.5 * budget_s / max(te, 1e-3))))
+        per_target = te / len(idx)
+        sample = "oracle tree build on all %d sources (%.1f s, 1 thread) + %d random targets (%.1f s, %d threads)" % (
+            len(data["xs"]), t_build, len(idx), te, threads)
+    else:
+        nb = len(data["x0s"])
+        k = 16
+        while True:
+            bidx = rng.choice(nb, min(k, nb), replace=False)
+            t1 = time.perf_counter()
+            tree.force(theta, data["x0s"][bidx], data["y0s"][bidx], data["hs"][bidx], nthreads=threads)
+            te = time.perf_counter() - t1
+            if te > 0.25 * budget_s or k >= nb:
+                break
+            k = int(min(nb, k * max(2.0, 0.5 * budget_s / max(te, 1e-3))))
+        m = nb * 1024
+        per_target = te / (len(bidx) * 1024)
+        sample = "oracle tree build on all %d sources (%.1f s, 1 thread) + %d random blocks (%.1f s, %d threads)" % (
+            len(data["xs"]), t_build, len(bidx), te, threads)
+    total = t_build + per_target * m
+    return {"value": m / total / 1e6, "unit": "Mtargets/s", "cores": threads, "kind": "port",
+            "sample": sample + "; extrapolated to %d targets" % m}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--n", type=int, default=0, help="override the number of sources")
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--order", type=int, default=0)
+    ap.add_argument("--leaf", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--only", choices=["f32", "f64"], default=None, help="time one precision only (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not supported (--gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    log("generating workload", args.config)
+    kind, data, desc = make_inputs(args.config, args.n)
+    p = args.order or (12 if kind == "potential" else 24)
+    cases = {}
+    timings = {}
+    for prec in (["f32", "f64"] if args.only is None else [args.only]):
+        log("timing", prec)
+        c = GpuCase(kind, data, 1 if prec == "f32" else 0, args.theta, args.order, args.leaf)
+        dt, phases = c.timed(args.steps, args.warmup)
+        cases[prec] = c
+        timings[prec] = (dt, phases)
+        log("  %s: %.3f ms/step, phases %s" % (prec, dt * 1e3, {k: round(v, 3) for k, v in phases.items()}))
+
+    head = "f32" if "f32" in cases else "f64"
+    dt, phases = timings[head]
+    m = cases[head].m
+    st = cases[head].counters()
+    cnt = st["counters"]
+    fl = flops_potential(cnt, p) if kind == "potential" else flops_force(cnt, p)
+    eval_ms = phases["eval"]
+    achieved = fl / (eval_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[head]
+
+    errs = None
+    if not args.no_error and "f32" in cases and "f64" in cases:
+        log("direct-sum error check")
+        errs = sampled_error(cases["f32"], cases["f64"], kind, data)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(kind, data, args.theta, args.order, args.leaf, args.cpu_budget)
+
+    res = {
+        "metric": "Mtargets/s at rel err <=1e-6, FP32-bulk vs FP64 plain",
+        "value": m / dt / 1e6,
+        "unit": "Mtargets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head,
+        "data": "synthetic",
+        "config": {"workload": desc, "theta": args.theta, "order": p,
+                   "leaf": args.leaf or (512 if kind == "potential" else 192),
+                   "precision": "FP32 P2P/E2P on leaf-relative offsets, FP64 decisions/P2E/E2E/E2L"
+                   if head == "f32" else "plain FP64", "inputs": "resident in HBM"},
+        "phases_ms": {k: round(v, 4) for k, v in phases.items()},
+        "counters": {k: int(v) for k, v in cnt.items()},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_potential (traversal)" if kind == "potential" else "k_force (traversal)",
+                     "note": "VALU-bound: peak is the vector %s rate (no MFMA in this path)" % head.upper()},
+        "cpu_baseline": cpu,
+    }
+    if "f32" in timings and "f64" in timings:
+        res["fp64_value"] = m / timings["f64"][0] / 1e6
+        res["speedup_fp32_vs_fp64"] = timings["f64"][0] / timings["f32"][0]
+        res["fp64_phases_ms"] = {k: round(v, 4) for k, v in timings["f64"][1].items()}
+    if errs:
+        res["rel_err_vs_direct"] = errs
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,7 +58,7 @@ enum {
     TC_OPT_FORCE_ORDER = 3,    /* force expansion order, default 24 */
     TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
     TC_OPT_PROFILE = 5,        /* 1 = record phase times with HIP events */
-    TC_OPT_COUNTERS = 6        /* 1 = count interactions (p2p pairs, e2p, e2l, l2p) */
+    TC_OPT_COUNTERS = 6        /* 1 = count interactions (p2p pairs, e2p, e2l, l2p, p2p lane slots) */
 };
 
 /* Phase times (tc_get_stats times_ms[]) */
@@ -89,8 +89,10 @@ TC_API int tc_potential_resident(tc_context *ctx, double theta, const double *xt
 TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x0s, const double *y0s,
                                   const double *hs, int nblocks, double *xr, double *yr, int device_ptrs);
 
-/* Times of the last call's phases (requires TC_OPT_PROFILE), interaction counters
- * (requires TC_OPT_COUNTERS), and tree_info = {n, nnodes, ninternal, order}. */
+/* Times of the last call's phases (requires TC_OPT_PROFILE), counters[5] (requires
+ * TC_OPT_COUNTERS): {target-source pairs evaluated directly, (target, node) multipole
+ * evaluations, (brick, node) multipole-to-local shifts, targets given a local expansion,
+ * lane slots occupied by the direct loops}, and tree_info = {n, nnodes, ninternal, order}. */
 TC_API int tc_get_stats(tc_context *ctx, double *times_ms, unsigned long long *counters, int *tree_info);
 
 /* Copy the resident tree to host memory (any pointer may be NULL):
@@ -107,10 +109,53 @@ TC_API int tc_device_free(tc_context *ctx, void *ptr);
 TC_API int tc_copy_to_device(tc_context *ctx, void *dst, const void *src, size_t bytes);
 TC_API int tc_copy_to_host(tc_context *ctx, void *dst, const void *src, size_t bytes);
 
+/* FP64 direct sums at the targets idx[0..nidx) (main.cpp:206-245 regularisation):
+ * potential out0[k] = sum_j v_j log(sqrt(|t - s_j|^2 + 10 eps)); force (out0, out1)[k] =
+ * sum_j v_j (t - s_j) / (|t - s_j|^2 + 10 eps).  All pointers in device memory of the
+ * context's GPU; synchronous.  For measuring treecode error at full size. */
+TC_API int tc_direct_potential(tc_context *ctx, const double *xs, const double *ys, const double *vs, int n,
+                               const double *xt, const double *yt, const int *idx, int nidx, double *out);
+TC_API int tc_direct_force(tc_context *ctx, const double *xs, const double *ys, const double *vs, int n,
+                           const double *xt, const double *yt, const int *idx, int nidx, double *outx,
+                           double *outy);
+
 /* Morton keys + stable radix sort of n points in device memory (the source-sort
  * stage on its own): box[4] out (xmin, ymin, ext, scale), keys/perm out. */
 TC_API int tc_morton_sort_device(tc_context *ctx, const double *x, const double *y, int n, double *box,
                                  unsigned int *keys, unsigned int *perm);
+
+/* ---- request server: continuous batching over slots (no counterpart in the reference) ----
+ * Each slot is a context with its own stream and host thread; queued requests are
+ * admitted to free slots while their estimated HBM footprint fits the budget, and
+ * requests in different slots overlap on the GPU.  All request pointers are host
+ * memory and must stay valid until tc_server_wait returns for that ticket. */
+typedef struct tc_server tc_server;
+
+enum { TC_REQ_POTENTIAL = 0, TC_REQ_FORCE = 1 };
+
+typedef struct tc_request {
+    int kind;                         /* TC_REQ_POTENTIAL or TC_REQ_FORCE */
+    int precision;                    /* 0 plain FP64, 1 FP32 bulk */
+    double theta;
+    const double *xs, *ys, *vs;       /* sources */
+    int n;
+    const double *xt, *yt;            /* potential targets */
+    int m;
+    const double *x0s, *y0s, *hs;     /* force: 32x32 blocks */
+    int nblocks;
+    double *out0, *out1;              /* potential: out0[m]; force: out0/out1[1024*nblocks] */
+    long long source_id;              /* nonzero: same id = same sources, the tree may be reused */
+} tc_request;
+
+TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib);
+TC_API void tc_server_destroy(tc_server *srv);
+/* Returns a ticket > 0, or < 0 if the request can never fit the HBM budget. */
+TC_API long long tc_server_submit(tc_server *srv, const tc_request *req);
+/* Blocks until the request finished; times_ms[5] = {queue, service, latency, reused, slot}.
+ * Returns the request's status (0 ok, < 0 error). */
+TC_API int tc_server_wait(tc_server *srv, long long ticket, double *times_ms);
+/* out[7] = {completed, reused trees, failed, max in flight, peak admitted bytes, budget bytes, queued} */
+TC_API int tc_server_stats(tc_server *srv, long long *out);
 
 #ifdef __cplusplus
 }

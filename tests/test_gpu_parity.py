@@ -245,3 +245,16 @@ def test_fp32_force_accuracy(pkg, oracle, wl, ctx32, ctx64):
     dx, dy = oracle.direct_force(xs, ys, vs, tx, ty, idx)
     assert rel_l2(fx32[idx], dx) <= FP32_TOL
     assert rel_l2(fy32[idx], dy) <= FP32_TOL
+
+
+def test_gpu_direct_sums_vs_oracle(pkg, oracle, wl, ctx64):
+    # the GPU direct-sum kernels used by bench.py to measure error at full size
+    xs, ys, vs, xt, yt = wl.uniform_potential(5000, seed=19)
+    idx = np.arange(0, 5000, 7, dtype=np.int32)
+    d = [ctx64.to_device(a) for a in (xs, ys, vs, xt, yt)]
+    di = ctx64.to_device(idx, np.int32)
+    g = ctx64.direct_potential(*d, di)
+    assert rel_l2(g, oracle.direct_potential(xs, ys, vs, xt, yt, idx)) <= 1e-14
+    gx, gy = ctx64.direct_force(*d, di)
+    rx, ry = oracle.direct_force(xs, ys, vs, xt, yt, idx)
+    assert rel_l2(gx, rx) <= 1e-14 and rel_l2(gy, ry) <= 1e-14
