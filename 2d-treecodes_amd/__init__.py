@@ -31,7 +31,8 @@ EXPORTED = [
     "tc_synchronize", "tc_potential_resident", "tc_force_mrag_resident", "tc_get_stats", "tc_tree_export",
     "tc_morton_sort_device", "tc_device_alloc", "tc_device_free", "tc_copy_to_device", "tc_copy_to_host",
     "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
-    "tc_server_wait", "tc_server_stats",
+    "tc_server_wait", "tc_server_stats", "treecode_potential_float", "treecode_force_mrag_float",
+    "treecode_velocity_mrag", "tc_copy_kernel",
 ]
 
 
@@ -70,34 +71,78 @@ def lib():
     L.treecode_force_mrag.argtypes = [dbl, _D, _D, _D, it, _D, _D, _D, it, _D, _D]
     L.treecode_force_mrag.restype = None
     L.treecode_last_error.restype = ctypes.c_char_p
-    L.tc_context_create.argtypes = [it]
-    L.tc_context_create.restype = _P
-    L.tc_context_destroy.argtypes = [_P]
-    L.tc_set_option.argtypes = [_P, it, it]
-    L.tc_context_stream.argtypes = [_P]
-    L.tc_context_stream.restype = _P
-    L.tc_potential.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P, it]
-    L.tc_force_mrag.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, _P, it, _P, _P, it]
-    L.tc_potential_resident.argtypes = [_P, dbl, _P, _P, it, _P, it]
-    L.tc_force_mrag_resident.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it]
-    L.tc_synchronize.argtypes = [_P]
-    L.tc_get_stats.argtypes = [_P, _D, ctypes.POINTER(ctypes.c_ulonglong), _I]
-    L.tc_tree_export.argtypes = [_P, _I, _I, _D, _D, _I, _D]
-    L.tc_morton_sort_device.argtypes = [_P, _P, _P, it, _P, _P, _P]
-    L.tc_device_alloc.argtypes = [_P, ctypes.c_size_t]
-    L.tc_device_alloc.restype = _P
-    L.tc_device_free.argtypes = [_P, _P]
-    L.tc_copy_to_device.argtypes = [_P, _P, _P, ctypes.c_size_t]
-    L.tc_copy_to_host.argtypes = [_P, _P, _P, ctypes.c_size_t]
-    L.tc_server_create.argtypes = [it, it, dbl]
-    L.tc_server_create.restype = _P
-    L.tc_server_destroy.argtypes = [_P]
-    L.tc_server_submit.argtypes = [_P, ctypes.POINTER(TcRequest)]
-    L.tc_server_submit.restype = ctypes.c_longlong
-    L.tc_server_wait.argtypes = [_P, ctypes.c_longlong, _D]
-    L.tc_server_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong)]
-    L.tc_direct_potential.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P]
-    L.tc_direct_force.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P, _P]
+    fl = ctypes.c_float
+    _F = ctypes.POINTER(ctypes.c_float)
+    if hasattr(L, 'treecode_potential_float'):
+        L.treecode_potential_float.argtypes = [fl, _F, _F, _F, it, _F, _F, it, _F]
+    if hasattr(L, 'treecode_potential_float'):
+        L.treecode_potential_float.restype = None
+    if hasattr(L, 'treecode_force_mrag_float'):
+        L.treecode_force_mrag_float.argtypes = [fl, _F, _F, _F, it, _F, _F, _F, it, _F, _F]
+    if hasattr(L, 'treecode_force_mrag_float'):
+        L.treecode_force_mrag_float.restype = None
+    if hasattr(L, 'treecode_velocity_mrag'):
+        L.treecode_velocity_mrag.argtypes = [dbl, _D, _D, _D, it, _D, _D, _D, it, _D, _D]
+    if hasattr(L, 'treecode_velocity_mrag'):
+        L.treecode_velocity_mrag.restype = None
+    if hasattr(L, 'tc_context_create'):
+        L.tc_context_create.argtypes = [it]
+    if hasattr(L, 'tc_context_create'):
+        L.tc_context_create.restype = _P
+    if hasattr(L, 'tc_context_destroy'):
+        L.tc_context_destroy.argtypes = [_P]
+    if hasattr(L, 'tc_set_option'):
+        L.tc_set_option.argtypes = [_P, it, it]
+    if hasattr(L, 'tc_context_stream'):
+        L.tc_context_stream.argtypes = [_P]
+    if hasattr(L, 'tc_context_stream'):
+        L.tc_context_stream.restype = _P
+    if hasattr(L, 'tc_potential'):
+        L.tc_potential.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P, it]
+    if hasattr(L, 'tc_force_mrag'):
+        L.tc_force_mrag.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, _P, it, _P, _P, it]
+    if hasattr(L, 'tc_potential_resident'):
+        L.tc_potential_resident.argtypes = [_P, dbl, _P, _P, it, _P, it]
+    if hasattr(L, 'tc_force_mrag_resident'):
+        L.tc_force_mrag_resident.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it]
+    if hasattr(L, 'tc_synchronize'):
+        L.tc_synchronize.argtypes = [_P]
+    if hasattr(L, 'tc_get_stats'):
+        L.tc_get_stats.argtypes = [_P, _D, ctypes.POINTER(ctypes.c_ulonglong), _I]
+    if hasattr(L, 'tc_tree_export'):
+        L.tc_tree_export.argtypes = [_P, _I, _I, _D, _D, _I, _D]
+    if hasattr(L, 'tc_morton_sort_device'):
+        L.tc_morton_sort_device.argtypes = [_P, _P, _P, it, _P, _P, _P]
+    if hasattr(L, 'tc_device_alloc'):
+        L.tc_device_alloc.argtypes = [_P, ctypes.c_size_t]
+    if hasattr(L, 'tc_device_alloc'):
+        L.tc_device_alloc.restype = _P
+    if hasattr(L, 'tc_device_free'):
+        L.tc_device_free.argtypes = [_P, _P]
+    if hasattr(L, 'tc_copy_to_device'):
+        L.tc_copy_to_device.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    if hasattr(L, 'tc_copy_to_host'):
+        L.tc_copy_to_host.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    if hasattr(L, 'tc_server_create'):
+        L.tc_server_create.argtypes = [it, it, dbl]
+    if hasattr(L, 'tc_server_create'):
+        L.tc_server_create.restype = _P
+    if hasattr(L, 'tc_server_destroy'):
+        L.tc_server_destroy.argtypes = [_P]
+    if hasattr(L, 'tc_server_submit'):
+        L.tc_server_submit.argtypes = [_P, ctypes.POINTER(TcRequest)]
+    if hasattr(L, 'tc_server_submit'):
+        L.tc_server_submit.restype = ctypes.c_longlong
+    if hasattr(L, 'tc_server_wait'):
+        L.tc_server_wait.argtypes = [_P, ctypes.c_longlong, _D]
+    if hasattr(L, 'tc_server_stats'):
+        L.tc_server_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong)]
+    if hasattr(L, 'tc_copy_kernel'):
+        L.tc_copy_kernel.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    if hasattr(L, 'tc_direct_potential'):
+        L.tc_direct_potential.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P]
+    if hasattr(L, 'tc_direct_force'):
+        L.tc_direct_force.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P, _P]
     _lib = L
     return L
 
@@ -136,6 +181,50 @@ def treecode_force_mrag(theta, xs, ys, vs, x0s, y0s, hs):
     if nb and np.isnan(xr[0]) and last_error():
         raise TreecodeError(last_error())
     return xr, yr
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def treecode_potential_float(theta, xs, ys, vs, xt, yt):
+    """The reference's real=float build of treecode_potential (libraries.Makefile:13)."""
+    xs, ys, vs, xt, yt = map(_f32, (xs, ys, vs, xt, yt))
+    out = np.empty(len(xt), dtype=np.float32)
+    lib().treecode_potential_float(float(theta), _fp(xs), _fp(ys), _fp(vs), len(xs), _fp(xt), _fp(yt), len(xt),
+                                   _fp(out))
+    if len(out) and np.isnan(out[0]) and last_error():
+        raise TreecodeError(last_error())
+    return out
+
+
+def treecode_force_mrag_float(theta, xs, ys, vs, x0s, y0s, hs):
+    xs, ys, vs, x0s, y0s, hs = map(_f32, (xs, ys, vs, x0s, y0s, hs))
+    nb = len(x0s)
+    xr = np.empty(nb * 1024, dtype=np.float32)
+    yr = np.empty(nb * 1024, dtype=np.float32)
+    lib().treecode_force_mrag_float(float(theta), _fp(xs), _fp(ys), _fp(vs), len(xs), _fp(x0s), _fp(y0s), _fp(hs),
+                                    nb, _fp(xr), _fp(yr))
+    if nb and np.isnan(xr[0]) and last_error():
+        raise TreecodeError(last_error())
+    return xr, yr
+
+
+def treecode_velocity_mrag(theta, xs, ys, vorticity, x0s, y0s, hs):
+    """(u, v) = (grad_y, -grad_x) / (2 pi) of treecode_force_mrag (main.cpp:136-143)."""
+    xs, ys, vs, x0s, y0s, hs = map(_f64, (xs, ys, vorticity, x0s, y0s, hs))
+    nb = len(x0s)
+    u = np.empty(nb * 1024)
+    v = np.empty(nb * 1024)
+    lib().treecode_velocity_mrag(float(theta), _dp(xs), _dp(ys), _dp(vs), len(xs), _dp(x0s), _dp(y0s), _dp(hs), nb,
+                                 _dp(u), _dp(v))
+    if nb and np.isnan(u[0]) and last_error():
+        raise TreecodeError(last_error())
+    return u, v
 
 
 def _ptr(a):
@@ -290,6 +379,9 @@ class Context:
         self._check(lib().tc_direct_force(self.h, xs.ptr, ys.ptr, vs.ptr, len(xs), xt.ptr, yt.ptr, idx.ptr,
                                           len(idx), ox.ptr, oy.ptr))
         return ox.numpy(), oy.numpy()
+
+    def copy_kernel(self, dst, src):
+        self._check(lib().tc_copy_kernel(self.h, dst.ptr, src.ptr, min(dst.nbytes, src.nbytes)))
 
     def morton_sort_device(self, x, y, box, keys, perm):
         self._check(lib().tc_morton_sort_device(self.h, _ptr(x), _ptr(y), len(x), _ptr(box), _ptr(keys),

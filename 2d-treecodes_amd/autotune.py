@@ -1,0 +1,118 @@
+"""Autotuner for leaf capacity and expansion order.
+
+For one workload (sources, targets or blocks, theta, precision) it measures,
+on the GPU, every (leaf capacity, order) pair of a grid: the whole-solve time
+with inputs resident in HBM, and the relative L2 error against the FP64
+direct sum on a random sample of targets.  It returns the fastest pair whose
+error is within the target (default: no worse than the reference defaults,
+order 12 / leaf 512 for the potential and 24 / 192 for the force, and at
+most 1e-6), and caches the choice per (kind, precision, theta, log2 N,
+log2 targets) bucket in a JSON file so a server can look it up per request
+size.
+"""
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+DEFAULT = {"potential": (12, 512), "force": (24, 192)}
+ORDERS = (8, 12, 16, 20, 24, 32)
+LEAVES = (64, 128, 192, 256, 384, 512, 768, 1024)
+
+
+def bucket_key(kind, precision, theta, n, m):
+    return "%s/p%d/t%.2f/n%d/m%d" % (kind, precision, theta, int(round(math.log2(max(n, 1)))),
+                                     int(round(math.log2(max(m, 1)))))
+
+
+class Tuner:
+    def __init__(self, pkg, cache_path=None, device=0):
+        self.pkg = pkg
+        self.device = device
+        self.cache_path = cache_path
+        self.cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def lookup(self, kind, precision, theta, n, m):
+        return self.cache.get(bucket_key(kind, precision, theta, n, m))
+
+    def _save(self):
+        if self.cache_path:
+            tmp = self.cache_path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self.cache, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.cache_path)
+
+    def tune(self, kind, data, theta=0.5, precision=1, orders=ORDERS, leaves=LEAVES, err_target=None,
+             nsample=1024, reps=3, seed=0, log=None):
+        """data: potential {xs, ys, vs, xt, yt}; force {xs, ys, vs, x0s, y0s, hs}.
+        Returns {"order", "leaf", "ms", "err", "table": [...]} and records it in the cache."""
+        pkg = self.pkg
+        ctx = pkg.Context(device=self.device, precision=precision)
+        d = {k: ctx.to_device(v) for k, v in data.items()}
+        rng = np.random.default_rng(seed)
+        if kind == "potential":
+            m = len(data["xt"])
+            idx = np.sort(rng.choice(m, min(nsample, m), replace=False)).astype(np.int32)
+            ref = ctx.direct_potential(d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], ctx.to_device(idx, np.int32))
+            out = ctx.empty(m)
+        else:
+            m = len(data["x0s"]) * 1024
+            idx = np.sort(rng.choice(m, min(nsample, m), replace=False)).astype(np.int32)
+            b = idx // 1024
+            tx = data["x0s"][b] + (idx % 32) * data["hs"][b]
+            ty = data["y0s"][b] + ((idx // 32) % 32) * data["hs"][b]
+            rx, ry = ctx.direct_force(d["xs"], d["ys"], d["vs"], ctx.to_device(tx), ctx.to_device(ty),
+                                      ctx.to_device(np.arange(len(idx), dtype=np.int32), np.int32))
+            xr, yr = ctx.empty(m), ctx.empty(m)
+
+        def run():
+            if kind == "potential":
+                ctx.potential(theta, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
+            else:
+                ctx.force(theta, d["xs"], d["ys"], d["vs"], d["x0s"], d["y0s"], d["hs"], xr=xr, yr=yr, device=True)
+            ctx.synchronize()
+
+        def error():
+            if kind == "potential":
+                r = out.numpy()[idx]
+                return float(np.linalg.norm(r - ref) / np.linalg.norm(ref))
+            fx, fy = xr.numpy()[idx], yr.numpy()[idx]
+            return float(np.sqrt(np.sum((fx - rx) ** 2) + np.sum((fy - ry) ** 2)) /
+                         np.sqrt(np.sum(rx ** 2) + np.sum(ry ** 2)))
+
+        okey, lkey = (pkg.OPT_POT_ORDER, pkg.OPT_POT_LEAF) if kind == "potential" else \
+            (pkg.OPT_FORCE_ORDER, pkg.OPT_FORCE_LEAF)
+        table = []
+        for p in orders:
+            for leaf in leaves:
+                ctx.set(okey, p)
+                ctx.set(lkey, leaf)
+                run()
+                err = error()
+                ts = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    run()
+                    ts.append(time.perf_counter() - t0)
+                table.append({"order": p, "leaf": leaf, "ms": 1e3 * min(ts), "err": err})
+                if log:
+                    log("tune %s p=%d leaf=%d: %.3f ms err %.2e" % (kind, p, leaf, table[-1]["ms"], err))
+        p0, l0 = DEFAULT[kind]
+        base = [t for t in table if t["order"] == p0 and t["leaf"] == l0]
+        if err_target is None:
+            err_target = min(1e-6, max(base[0]["err"], 1e-15) * 1.01) if base else 1e-6
+        ok = [t for t in table if t["err"] <= err_target] or table
+        best = min(ok, key=lambda t: t["ms"])
+        res = {"order": best["order"], "leaf": best["leaf"], "ms": best["ms"], "err": best["err"],
+               "err_target": err_target, "table": table}
+        n = len(data["xs"])
+        self.cache[bucket_key(kind, precision, theta, n, m)] = {k: res[k] for k in ("order", "leaf", "ms", "err",
+                                                                                     "err_target")}
+        self._save()
+        ctx.close()
+        return res

@@ -249,6 +249,26 @@ tc_context *default_context()
     return g_default;
 }
 
+tc_context *g_default_f32 = nullptr;
+
+tc_context *default_context_f32()
+{
+    if (!g_default_f32) {
+        g_default_f32 = tc_context_create(0);
+        if (g_default_f32)
+            tc_set_option(g_default_f32, TC_OPT_PRECISION, 1);
+    }
+    return g_default_f32;
+}
+
+std::vector<double> widen(const float *p, size_t n)
+{
+    std::vector<double> v(n);
+    for (size_t i = 0; i < n; ++i)
+        v[i] = (double)p[i];
+    return v;
+}
+
 void report_dropin_failure(const char *fn)
 {
     static bool printed = false;
@@ -504,6 +524,16 @@ TC_API int tc_direct_force(tc_context *c, const double *xs, const double *ys, co
     });
 }
 
+TC_API int tc_copy_kernel(tc_context *c, void *dst, const void *src, size_t bytes)
+{
+    return guarded(c, [&] {
+        if (bytes % 16)
+            throw TcError(-3, "tc_copy_kernel: size must be a multiple of 16 bytes");
+        tc_copy16(src, dst, bytes, c->stream);
+        TC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 TC_API int tc_morton_sort_device(tc_context *c, const double *x, const double *y, int n, double *box,
                                  unsigned int *keys, unsigned int *perm)
 {
@@ -529,6 +559,62 @@ TC_API void treecode_potential(const double theta, const double *const xsources,
         report_dropin_failure("treecode_potential");
         for (int i = 0; i < ntargets; ++i)
             targetvalues[i] = std::numeric_limits<double>::quiet_NaN();
+    }
+}
+
+TC_API void treecode_potential_float(const float theta, const float *const xsources, const float *const ysources,
+                                     const float *const sourcevalues, const int nsources,
+                                     const float *const xtargets, const float *const ytargets,
+                                     const int ntargets, float *const targetvalues)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    const size_t n = nsources > 0 ? (size_t)nsources : 0, m = ntargets > 0 ? (size_t)ntargets : 0;
+    std::vector<double> xs = widen(xsources, n), ys = widen(ysources, n), vs = widen(sourcevalues, n);
+    std::vector<double> xt = widen(xtargets, m), yt = widen(ytargets, m), out(m);
+    tc_context *c = default_context_f32();
+    int rc = c ? tc_potential(c, (double)theta, xs.data(), ys.data(), vs.data(), nsources, xt.data(), yt.data(),
+                              ntargets, out.data(), 0)
+               : -1;
+    if (rc)
+        report_dropin_failure("treecode_potential_float");
+    for (size_t i = 0; i < m; ++i)
+        targetvalues[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)out[i];
+}
+
+TC_API void treecode_force_mrag_float(const float theta, const float *const xsources,
+                                      const float *const ysources, const float *const sourcevalues,
+                                      const int nsources, const float *const x0s, const float *const y0s,
+                                      const float *const hs, const int nblocks, float *const xresults,
+                                      float *const yresults)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    const size_t n = nsources > 0 ? (size_t)nsources : 0, nb = nblocks > 0 ? (size_t)nblocks : 0;
+    std::vector<double> xs = widen(xsources, n), ys = widen(ysources, n), vs = widen(sourcevalues, n);
+    std::vector<double> bx = widen(x0s, nb), by = widen(y0s, nb), bh = widen(hs, nb);
+    std::vector<double> xr(nb * 1024), yr(nb * 1024);
+    tc_context *c = default_context_f32();
+    int rc = c ? tc_force_mrag(c, (double)theta, xs.data(), ys.data(), vs.data(), nsources, bx.data(), by.data(),
+                               bh.data(), nblocks, xr.data(), yr.data(), 0)
+               : -1;
+    if (rc)
+        report_dropin_failure("treecode_force_mrag_float");
+    for (size_t i = 0; i < nb * 1024; ++i) {
+        xresults[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)xr[i];
+        yresults[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)yr[i];
+    }
+}
+
+TC_API void treecode_velocity_mrag(const double theta, const double *const xsources,
+                                   const double *const ysources, const double *const vorticity,
+                                   const int nsources, const double *const x0s, const double *const y0s,
+                                   const double *const hs, const int nblocks, double *const u, double *const v)
+{
+    treecode_force_mrag(theta, xsources, ysources, vorticity, nsources, x0s, y0s, hs, nblocks, v, u);
+    const double inv2pi = 1.0 / (2.0 * M_PI);
+    for (size_t i = 0; i < (size_t)(nblocks > 0 ? nblocks : 0) * 1024; ++i) {
+        const double gx = v[i], gy = u[i];
+        u[i] = gy * inv2pi;
+        v[i] = -gx * inv2pi;
     }
 }
 

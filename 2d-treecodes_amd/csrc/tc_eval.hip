@@ -541,6 +541,14 @@ __global__ __launch_bounds__(256) void k_direct(const double *__restrict__ xs, c
     }
 }
 
+// Plain 16-byte-per-thread copy: the reference kernel for scaling HBM counters
+// (FETCH_SIZE / WRITE_SIZE of a copy of known size).
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 int tiles_of(int n, int t) { return (n + t - 1) / t; }
 
 template <int P>
@@ -592,6 +600,15 @@ void tc_upload_binomials_eval(hipStream_t st)
     TC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_binomT), tab.data(), sizeof(double) * tab.size(), 0,
                                   hipMemcpyHostToDevice, st));
     TC_HIP(hipStreamSynchronize(st));
+}
+
+void tc_copy16(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (!n)
+        return;
+    k_copy16<<<2048, 256, 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+    TC_HIP(hipGetLastError());
 }
 
 void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n, const double *xt,

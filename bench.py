@@ -222,11 +222,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64"], default=None, help="time one precision only (profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not supported (--gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        cctx = pkg.Context()
+        nb = args.copy_calibration << 20
+        a, b = cctx.empty(nb // 8), cctx.empty(nb // 8)
+        for _ in range(3):
+            cctx.copy_kernel(b, a)
+        log("copy kernel: 3 x %d MiB" % args.copy_calibration)
+        a.free()
+        b.free()
+        cctx.close()
 
     log("generating workload", args.config)
     kind, data, desc = make_inputs(args.config, args.n)

@@ -41,6 +41,28 @@ TC_API void treecode_force_mrag(const double theta, const double *const xsources
                                 const double *const y0s, const double *const hs, const int nblocks,
                                 double *const xresults, double *const yresults);
 
+/* The reference's `real=float` build (libraries.Makefile:13, realtype substituted by m4)
+ * has the same two functions over float arrays.  Here they carry a _float suffix so both
+ * variants live in one library; they run the FP32-bulk path (FP64 tree geometry and
+ * translations, FP32 near field and multipole evaluation). */
+TC_API void treecode_potential_float(const float theta, const float *const xsources, const float *const ysources,
+                                     const float *const sourcevalues, const int nsources,
+                                     const float *const xtargets, const float *const ytargets,
+                                     const int ntargets, float *const targetvalues);
+TC_API void treecode_force_mrag_float(const float theta, const float *const xsources,
+                                      const float *const ysources, const float *const sourcevalues,
+                                      const int nsources, const float *const x0s, const float *const y0s,
+                                      const float *const hs, const int nblocks, float *const xresults,
+                                      float *const yresults);
+
+/* Velocities of an MRAG vortex solver from vorticity sources: the rotation and 2 pi
+ * scaling that main.cpp:136-143 applies to compare against stored velocities,
+ * u = grad_y / (2 pi), v = -grad_x / (2 pi) of treecode_force_mrag, same layout. */
+TC_API void treecode_velocity_mrag(const double theta, const double *const xsources,
+                                   const double *const ysources, const double *const vorticity,
+                                   const int nsources, const double *const x0s, const double *const y0s,
+                                   const double *const hs, const int nblocks, double *const u, double *const v);
+
 /* ---- extensions (no counterpart in the reference) ---- */
 
 /* Message of the last failure on the calling thread ("" if none).  The void
@@ -118,6 +140,10 @@ TC_API int tc_direct_potential(tc_context *ctx, const double *xs, const double *
 TC_API int tc_direct_force(tc_context *ctx, const double *xs, const double *ys, const double *vs, int n,
                            const double *xt, const double *yt, const int *idx, int nidx, double *outx,
                            double *outy);
+
+/* Device-to-device copy by a plain kernel (16 B per thread, bytes a multiple of 16); the
+ * known-size reference for scaling hardware memory counters.  Synchronous. */
+TC_API int tc_copy_kernel(tc_context *ctx, void *dst, const void *src, size_t bytes);
 
 /* Morton keys + stable radix sort of n points in device memory (the source-sort
  * stage on its own): box[4] out (xmin, ymin, ext, scale), keys/perm out. */

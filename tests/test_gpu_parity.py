@@ -258,3 +258,35 @@ def test_gpu_direct_sums_vs_oracle(pkg, oracle, wl, ctx64):
     gx, gy = ctx64.direct_force(*d, di)
     rx, ry = oracle.direct_force(xs, ys, vs, xt, yt, idx)
     assert rel_l2(gx, rx) <= 1e-14 and rel_l2(gy, ry) <= 1e-14
+
+
+def test_float_abi(pkg, oracle, wl):
+    # the reference's real=float build: float arrays in and out, FP32-bulk path inside
+    xs, ys, vs, xt, yt = (a.astype(np.float32) for a in wl.uniform_potential(20000, seed=61))
+    out = pkg.treecode_potential_float(0.5, xs, ys, vs, xt, yt)
+    assert out.dtype == np.float32
+    ref, _ = oracle.potential(0.5, *(a.astype(np.float64) for a in (xs, ys, vs, xt, yt)))
+    assert rel_l2(out, ref) <= FP32_TOL
+    xs, ys, vs, x0s, y0s, hs = (a.astype(np.float32) for a in wl.gaussian4_force(20000, nbx=4, h=1.0 / 128, seed=62))
+    fx, fy = pkg.treecode_force_mrag_float(0.5, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, _ = oracle.force_mrag(0.5, *(a.astype(np.float64) for a in (xs, ys, vs, x0s, y0s, hs)))
+    assert rel_l2(fx, rx) <= FP32_TOL and rel_l2(fy, ry) <= FP32_TOL
+
+
+def test_velocity_wrapper(pkg, wl):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(20000, nbx=4, h=1.0 / 128, seed=63)
+    u, v = pkg.treecode_velocity_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    fx, fy = pkg.treecode_force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert np.array_equal(u, fy * (1.0 / (2 * np.pi))) and np.array_equal(v, -fx * (1.0 / (2 * np.pi)))
+
+
+def test_log64_accuracy(pkg, ctx64):
+    # theta = 0 with one source of value 2 at the origin: out = log(r^2 + 10 eps) exactly as
+    # computed by the P2P loop, whose log is the library's own FP64 routine
+    r2 = np.concatenate([np.logspace(-14, 3, 4000), 1 + np.linspace(-0.3, 0.4, 3000)])
+    xt = np.sqrt(r2)
+    yt = np.zeros_like(xt)
+    out = ctx64.potential(0.0, np.zeros(1), np.zeros(1), np.full(1, 2.0), xt, yt)
+    ref = np.log(xt * xt + 10 * np.finfo(float).eps)
+    ulp = np.abs(out - ref) / np.spacing(np.abs(ref))
+    assert ulp.max() <= 4, ulp.max()
