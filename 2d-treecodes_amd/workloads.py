@@ -161,15 +161,34 @@ def request_stream(nreq=10000, rate_hz=2000.0, seed=1454, pot_frac=0.7, reuse_fr
             n = int(round(math.exp(math.log(force_n[0]) + ul[i] * (math.log(force_n[1]) - math.log(force_n[0])))))
             nb = int(round(math.exp(math.log(force_nb[0]) + ub[i] * (math.log(force_nb[1]) - math.log(force_nb[0])))))
         src_seed = 100000 + i
+        g = "uniform" if gen[i] == 0 else "gauss4"
         if reuse[i] and prev is not None:
-            n, src_seed = prev["n"], prev["src_seed"]
+            n, src_seed, g = prev["n"], prev["src_seed"], prev["gen"]
         r = {"id": i, "arrival": float(arrival[i]), "kind": "potential" if kind[i] == 0 else "force",
              "n": n, "nblocks": nb, "theta": float(thetas[th[i]]), "src_seed": src_seed,
-             "gen": "uniform" if gen[i] == 0 else "gauss4", "tgt_seed": 200000 + i,
+             "gen": g, "tgt_seed": 200000 + i,
              "reuses_sources": bool(reuse[i] and prev is not None)}
         reqs.append(r)
         prev = r
     return reqs
+
+
+def request_pools(seed=1454, size=2_000_000):
+    """Pre-generated source pools (uniform and 4-Gaussian) of the served stream."""
+    st = Stream(seed + 7)
+    uni = (st.uniform(size), st.uniform(size), st.uniform(size, -1.0, 1.0))
+    centres = np.array([[0.3, 0.3], [0.7, 0.3], [0.3, 0.7], [0.7, 0.7]])
+    gau = _gaussian_mixture(st, size, centres, np.full(4, 0.05))
+    return {"uniform": uni, "gauss4": gau, "size": size}
+
+
+def pool_sources(pools, r):
+    """Sources of request r as a window of the pool chosen by its source seed."""
+    size = pools["size"]
+    n = min(r["n"], size)
+    off = (r["src_seed"] * 2654435761) % (size - n + 1)
+    xs, ys, vs = pools[r["gen"]]
+    return xs[off:off + n], ys[off:off + n], vs[off:off + n]
 
 
 def request_sources(r):

@@ -208,6 +208,56 @@ def cpu_baseline(kind, data, theta, order, leaf, budget_s):
             "sample": sample + "; extrapolated to %d targets" % m}
 
 
+def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454):
+    """Replay the served mixed-size stream: Poisson arrivals, 70% potential / 30% force,
+    25% of requests reuse the previous request's sources (the server keeps the tree).
+    Latency is measured by the server from submission to completion."""
+    import threading
+    reqs = workloads.request_stream(nreq, rate_hz=rate_hz, seed=seed)
+    pools = workloads.request_pools(seed=seed)
+    prepared = []
+    for r in reqs:
+        xs, ys, vs = workloads.pool_sources(pools, r)
+        tg = workloads.request_targets(r, xs, ys)
+        prepared.append((r, xs, ys, vs, tg))
+    srv = pkg.Server(slots=slots, hbm_budget_gib=budget_gib)
+    info = [None] * nreq
+    tickets = [None] * nreq
+    done = threading.Event()
+
+    def waiter():
+        for i in range(nreq):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            _, info[i] = srv.wait(tickets[i])
+        done.set()
+
+    th = threading.Thread(target=waiter, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    for i, (r, xs, ys, vs, tg) in enumerate(prepared):
+        now = time.perf_counter() - t0
+        if r["arrival"] > now:
+            time.sleep(r["arrival"] - now)
+        sid = r["src_seed"]
+        if r["kind"] == "potential":
+            tickets[i] = srv.submit_potential(r["theta"], xs, ys, vs, *tg, precision=precision, source_id=sid)
+        else:
+            tickets[i] = srv.submit_force(r["theta"], xs, ys, vs, *tg, precision=precision, source_id=sid)
+    done.wait()
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([x["latency_ms"] for x in info])
+    svc = np.array([x["service_ms"] for x in info])
+    targets = sum(len(p[4][0]) * (1 if p[0]["kind"] == "potential" else 1024) for p in prepared)
+    return {"requests": nreq, "rate_hz": rate_hz, "slots": slots, "precision": "f32" if precision else "f64",
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_service_ms": float(svc.mean()), "makespan_s": makespan,
+            "targets_per_s": targets / makespan, "reused_trees": int(st["reused"]),
+            "max_inflight": int(st["max_inflight"])}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +272,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64"], default=None, help="time one precision only (profiling)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the served stream replayed after the solve timing (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate of the stream (1/s)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -268,6 +322,13 @@ def main():
     if not args.no_error and "f32" in cases and "f64" in cases:
         log("direct-sum error check")
         errs = sampled_error(cases["f32"], cases["f64"], kind, data)
+    serving = None
+    if args.serve_requests > 0:
+        log("served stream: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
+        for c in cases.values():
+            c.ctx.close()
+        serving = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, 64.0)
+        log("  p50 %.2f ms p99 %.2f ms" % (serving["p50_ms"], serving["p99_ms"]))
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -297,6 +358,7 @@ def main():
                      "kernel": "k_potential (traversal)" if kind == "potential" else "k_force (traversal)",
                      "note": "VALU-bound: peak is the vector %s rate (no MFMA in this path)" % head.upper()},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     if "f32" in timings and "f64" in timings:
         res["fp64_value"] = m / timings["f64"][0] / 1e6

@@ -97,3 +97,17 @@ def test_workloads_deterministic(wl):
     reqs = wl.request_stream(200)
     assert len(reqs) == 200 and all(r["arrival"] > 0 for r in reqs)
     assert any(r["reuses_sources"] for r in reqs)
+
+
+def test_request_stream_reuse_is_consistent(wl):
+    reqs = wl.request_stream(500, seed=3)
+    pools = wl.request_pools(seed=3, size=100000)
+    by_id = {}
+    for r in reqs:
+        key = r["src_seed"]
+        src = wl.pool_sources(pools, dict(r, n=min(r["n"], 100000)))
+        if key in by_id:
+            # same source id -> identical sources (the server may reuse the tree)
+            assert all(np.array_equal(a, b) for a, b in zip(by_id[key], src))
+        by_id[key] = src
+    assert sum(r["reuses_sources"] for r in reqs) > 50
