@@ -776,8 +776,10 @@ __global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int
     }
 }
 
-// FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, tiny), so
-// that the FP32 E2P evaluates sum b_j (s/z)^j without over/underflow.
+// FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, 1e-30), so that
+// the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta.  Since
+// |a_j| <= sum|v| r^j / j, dividing the coefficient itself j times keeps every
+// intermediate bounded (no overflow of s^-j for tiny radii).
 template <int P>
 __global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const double *exps, int nnodes, float2 *ex32)
 {
@@ -786,13 +788,14 @@ __global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const doubl
     if (id >= nnodes || lane >= P)
         return;
     const double r = nodes[id].r;
-    const double sc = r > 1e-300 ? r : 1e-300;
-    // s^-(j) by repeated division keeps the scale exact enough in FP64
-    double inv = 1.0;
-    for (int j = 0; j <= lane; ++j)
-        inv /= sc;
+    const double sc = r > 1e-30 ? r : 1e-30;
     const double *ex = exps + (size_t)2 * P * id;
-    ex32[(size_t)P * id + lane] = make_float2((float)(ex[lane] * inv), (float)(ex[P + lane] * inv));
+    double a = ex[lane], b = ex[P + lane];
+    for (int j = 0; j <= lane; ++j) {
+        a /= sc;
+        b /= sc;
+    }
+    ex32[(size_t)P * id + lane] = make_float2((float)a, (float)b);
 }
 
 int tiles_of(int n, int t) { return (n + t - 1) / t; }

@@ -207,16 +207,23 @@ def test_empty_and_error_paths(pkg, wl, ctx64):
     ctx64.potential(0.5, xs, ys, vs, xt, yt)
 
 
-def test_coincident_sources(pkg, oracle, ctx64):
+def test_coincident_sources(pkg, oracle, ctx64, ctx32):
+    # zero-radius nodes (all sources at one point) next to ordinary ones
     n = 2000
-    xs = np.full(n, 0.25)
-    ys = np.full(n, 0.75)
-    vs = np.linspace(-1, 1, n) + 0.3
-    xt = np.array([0.0, 0.25, 1.0, 0.3])
-    yt = np.array([0.0, 0.75, 1.0, 0.7])
+    xs = np.concatenate([np.full(n, 0.25), np.linspace(0.0, 1.0, 3000)])
+    ys = np.concatenate([np.full(n, 0.75), np.linspace(1.0, 0.0, 3000) ** 2])
+    vs = np.concatenate([np.linspace(-1, 1, n) + 0.3, np.full(3000, -0.2)])
+    xt = np.array([0.0, 0.25, 1.0, 0.3, 0.6])
+    yt = np.array([0.0, 0.75, 1.0, 0.7, 0.2])
     out = ctx64.potential(0.5, xs, ys, vs, xt, yt)
     ref, _ = oracle.potential(0.5, xs, ys, vs, xt, yt)
     assert rel_l2(out, ref) <= FP64_TOL
+    out32 = ctx32.potential(0.5, xs, ys, vs, xt, yt)
+    assert np.all(np.isfinite(out32)) and rel_l2(out32, ref) <= FP32_TOL
+    x0s, y0s, hs = np.array([0.2, 0.5]), np.array([0.7, 0.1]), np.array([1 / 512, 1 / 256])
+    fx, fy = ctx32.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, _ = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert np.all(np.isfinite(fx)) and rel_l2(fx, rx) <= FP32_TOL and rel_l2(fy, ry) <= FP32_TOL
 
 
 # ------------------------------------------------------------------ FP32 bulk
