@@ -32,7 +32,7 @@ EXPORTED = [
     "tc_morton_sort_device", "tc_device_alloc", "tc_device_free", "tc_copy_to_device", "tc_copy_to_host",
     "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
     "tc_server_wait", "tc_server_stats", "treecode_potential_float", "treecode_force_mrag_float",
-    "treecode_velocity_mrag", "tc_copy_kernel",
+    "treecode_velocity_mrag", "tc_copy_kernel", "treecode_force", "tc_force_targets",
 ]
 
 
@@ -137,6 +137,11 @@ def lib():
         L.tc_server_wait.argtypes = [_P, ctypes.c_longlong, _D]
     if hasattr(L, 'tc_server_stats'):
         L.tc_server_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong)]
+    if hasattr(L, 'treecode_force'):
+        L.treecode_force.argtypes = [dbl, _D, _D, _D, it, _D, _D, it, _D, _D]
+        L.treecode_force.restype = None
+    if hasattr(L, 'tc_force_targets'):
+        L.tc_force_targets.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P, _P, it]
     if hasattr(L, 'tc_copy_kernel'):
         L.tc_copy_kernel.argtypes = [_P, _P, _P, ctypes.c_size_t]
     if hasattr(L, 'tc_direct_potential'):
@@ -210,6 +215,18 @@ def treecode_force_mrag_float(theta, xs, ys, vs, x0s, y0s, hs):
     lib().treecode_force_mrag_float(float(theta), _fp(xs), _fp(ys), _fp(vs), len(xs), _fp(x0s), _fp(y0s), _fp(hs),
                                     nb, _fp(xr), _fp(yr))
     if nb and np.isnan(xr[0]) and last_error():
+        raise TreecodeError(last_error())
+    return xr, yr
+
+
+def treecode_force(theta, xs, ys, vs, xt, yt):
+    """Gradient at arbitrary targets (extension; the reference only has the MRAG layout)."""
+    xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
+    xr = np.empty(len(xt))
+    yr = np.empty(len(xt))
+    lib().treecode_force(float(theta), _dp(xs), _dp(ys), _dp(vs), len(xs), _dp(xt), _dp(yt), len(xt), _dp(xr),
+                         _dp(yr))
+    if len(xr) and np.isnan(xr[0]) and last_error():
         raise TreecodeError(last_error())
     return xr, yr
 
@@ -331,6 +348,15 @@ class Context:
             yr = np.empty(len(x0s) * 1024) if yr is None else yr
         self._check(lib().tc_force_mrag_resident(self.h, float(theta), _ptr(x0s), _ptr(y0s), _ptr(hs), len(x0s),
                                                  _ptr(xr), _ptr(yr), int(device)))
+        return xr, yr
+
+    def force_targets(self, theta, xs, ys, vs, xt, yt, xr=None, yr=None, device=False):
+        if not device:
+            xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
+            xr = np.empty(len(xt)) if xr is None else xr
+            yr = np.empty(len(xt)) if yr is None else yr
+        self._check(lib().tc_force_targets(self.h, float(theta), _ptr(xs), _ptr(ys), _ptr(vs), len(xs), _ptr(xt),
+                                           _ptr(yt), len(xt), _ptr(xr), _ptr(yr), int(device)))
         return xr, yr
 
     def synchronize(self):

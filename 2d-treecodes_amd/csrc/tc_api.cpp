@@ -242,6 +242,59 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
     });
 }
 
+int run_force_targets(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
+                      const double *xt, const double *yt, int m, double *xr, double *yr, int device_ptrs)
+{
+    return guarded(ctx, [&] {
+        tc_context *c = ctx;
+        c->ev_valid = false;
+        if (m <= 0)
+            return;
+        if (n <= 0) {
+            if (device_ptrs) {
+                TC_HIP(hipMemsetAsync(xr, 0, sizeof(double) * (size_t)m, c->stream));
+                TC_HIP(hipMemsetAsync(yr, 0, sizeof(double) * (size_t)m, c->stream));
+            } else {
+                std::memset(xr, 0, sizeof(double) * (size_t)m);
+                std::memset(yr, 0, sizeof(double) * (size_t)m);
+            }
+            c->tree_kind = TC_KIND_NONE;
+            return;
+        }
+        const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
+        double *dxr = xr, *dyr = yr;
+        if (!device_ptrs) {
+            for (int i = 0; i < m; ++i)
+                if (!std::isfinite(xt[i]) || !std::isfinite(yt[i]))
+                    throw TcError(-4, "treecode_force: non-finite target coordinate");
+            c->io.reserve(sizeof(double) * (3 * (size_t)n + 4 * (size_t)m));
+            double *base = c->io.as<double>();
+            double *bxs = base, *bys = base + n, *bvs = base + 2 * (size_t)n;
+            double *bxt = base + 3 * (size_t)n, *byt = bxt + m;
+            dxr = byt + m;
+            dyr = dxr + m;
+            copy_in(xs, bxs, n, c->stream);
+            copy_in(ys, bys, n, c->stream);
+            copy_in(vs, bvs, n, c->stream);
+            copy_in(xt, bxt, m, c->stream);
+            copy_in(yt, byt, m, c->stream);
+            dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
+        }
+        unsigned long long *cnt = prepare_counters(c);
+        record(c, 0);
+        build(c, dxs, dys, dvs, n, TC_KIND_FORCE);
+        record(c, 1);
+        tc_eval_force_targets(c->tree, theta, dxt, dyt, m, dxr, dyr, c->precision, cnt, c->ws, c->stream,
+                              c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+        c->ev_valid = c->profile;
+        if (!device_ptrs) {
+            TC_HIP(hipMemcpyAsync(xr, dxr, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipMemcpyAsync(yr, dyr, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
 tc_context *default_context()
 {
     if (!g_default)
@@ -393,6 +446,13 @@ TC_API int tc_force_mrag_resident(tc_context *c, double theta, const double *x0s
                                   const double *hs, int nblocks, double *xr, double *yr, int device_ptrs)
 {
     return run_force(c, theta, nullptr, nullptr, nullptr, 0, x0s, y0s, hs, nblocks, xr, yr, device_ptrs, false);
+}
+
+TC_API int tc_force_targets(tc_context *c, double theta, const double *xs, const double *ys, const double *vs,
+                            int n, const double *xt, const double *yt, int m, double *xr, double *yr,
+                            int device_ptrs)
+{
+    return run_force_targets(c, theta, xs, ys, vs, n, xt, yt, m, xr, yr, device_ptrs);
 }
 
 TC_API int tc_synchronize(tc_context *c)
@@ -559,6 +619,23 @@ TC_API void treecode_potential(const double theta, const double *const xsources,
         report_dropin_failure("treecode_potential");
         for (int i = 0; i < ntargets; ++i)
             targetvalues[i] = std::numeric_limits<double>::quiet_NaN();
+    }
+}
+
+TC_API void treecode_force(const double theta, const double *const xsources, const double *const ysources,
+                           const double *const sourcevalues, const int nsources, const double *const xtargets,
+                           const double *const ytargets, const int ntargets, double *const xresults,
+                           double *const yresults)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    tc_context *c = default_context();
+    int rc = c ? tc_force_targets(c, theta, xsources, ysources, sourcevalues, nsources, xtargets, ytargets,
+                                  ntargets, xresults, yresults, 0)
+               : -1;
+    if (rc) {
+        report_dropin_failure("treecode_force");
+        for (int i = 0; i < ntargets; ++i)
+            xresults[i] = yresults[i] = std::numeric_limits<double>::quiet_NaN();
     }
 }
 

@@ -202,6 +202,9 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
                    hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+void tc_eval_force_targets(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *xr,
+                           double *yr, int precision, unsigned long long *counters, TcScratch &ws,
+                           hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 void tc_copy16(const void *src, void *dst, size_t bytes, hipStream_t st);
 void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n, const double *xt,
                     const double *yt, const int *idx, int nidx, int force, double *out0, double *out1,

@@ -126,6 +126,53 @@ TC_DEV float e2p_pot32(float rz, float iz, float mass, float sc, const float2 *_
     return mass * fast_log2(r2) * LN2_HALF + sum;
 }
 
+// force_e2p_8x8 (force-kernels.m4:57-98) for one target: fx += Re dphi/dz, fy -= Im dphi/dz
+template <int P>
+TC_DEV void e2p_force64(double rz, double iz, double mass, const double *__restrict__ ex, double &fx, double &fy)
+{
+#pragma clang fp contract(fast)
+    const double inv = 1.0 / (rz * rz + iz * iz);
+    const double ri1 = rz * inv, ii1 = -iz * inv;
+    double rsum = mass * ri1, isum = mass * ii1;
+    double pr = ri1, pi = ii1;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const double rn = pr * ri1 - pi * ii1;
+        const double in = pr * ii1 + pi * ri1;
+        pr = rn;
+        pi = in;
+        rsum -= (j + 1) * (ex[j] * pr - ex[P + j] * pi);
+        isum -= (j + 1) * (ex[j] * pi + ex[P + j] * pr);
+    }
+    fx += rsum;
+    fy -= isum;
+}
+
+// FP32 force E2P on scaled coefficients: dphi/dz = (Q - sum j b_j w^j) / z, w = s/z.
+template <int P>
+TC_DEV void e2p_force32(float zr, float zi, float mass, float sc, const float2 *__restrict__ ex, float &gx, float &gy)
+{
+#pragma clang fp contract(fast)
+    const float inv = fast_rcp(zr * zr + zi * zi);
+    const float ir = zr * inv, iim = -zi * inv;
+    const float wr = sc * ir, wi = sc * iim;
+    float pr = 1.f, pi = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const float rn = pr * wr - pi * wi;
+        const float in = pr * wi + pi * wr;
+        pr = rn;
+        pi = in;
+        const float cr = ex[j].x * pr - ex[j].y * pi;
+        const float ci = ex[j].x * pi + ex[j].y * pr;
+        sr += (float)(j + 1) * cr;
+        si += (float)(j + 1) * ci;
+    }
+    const float qr = mass - sr, qi = -si;
+    gx = qr * ir - qi * iim;
+    gy = -(qr * iim + qi * ir);
+}
+
 template <int P, bool F32>
 __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                    const float2 *__restrict__ exps32,
@@ -373,47 +420,13 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
             ne2p += 64;
             const double rz = (xb - xcom) + ix * h, iz = (yb - ycom) + iy * h;
             if (F32) {
-#pragma clang fp contract(fast)
-                const float2 *ex = exps32 + (size_t)P * id;
-                const float zr = (float)rz, zi = (float)iz;
-                const float inv = fast_rcp(zr * zr + zi * zi);
-                const float ir = zr * inv, iim = -zi * inv;
-                const float sc = (float)(r > 1e-30 ? r : 1e-30);
-                const float wr = sc * ir, wi = sc * iim;
-                float pr = 1.f, pi = 0.f, sr = 0.f, si = 0.f;
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const float rn = pr * wr - pi * wi;
-                    const float in = pr * wi + pi * wr;
-                    pr = rn;
-                    pi = in;
-                    const float cr = ex[j].x * pr - ex[j].y * pi;
-                    const float ci = ex[j].x * pi + ex[j].y * pr;
-                    sr += (float)(j + 1) * cr;
-                    si += (float)(j + 1) * ci;
-                }
-                // dphi/dz = (Q - sum j b_j w^j) / z
-                const float qr = (float)nd.mass - sr, qi = -si;
-                fx += (double)(qr * ir - qi * iim);
-                fy -= (double)(qr * iim + qi * ir);
+                float gx, gy;
+                e2p_force32<P>((float)rz, (float)iz, (float)nd.mass, (float)(r > 1e-30 ? r : 1e-30),
+                               exps32 + (size_t)P * id, gx, gy);
+                fx += (double)gx;
+                fy += (double)gy;
             } else {
-#pragma clang fp contract(fast)
-                const double *ex = exps + (size_t)2 * P * id;
-                const double inv = 1.0 / (rz * rz + iz * iz);
-                const double ri1 = rz * inv, ii1 = -iz * inv;
-                double rsum = nd.mass * ri1, isum = nd.mass * ii1;
-                double pr = ri1, pi = ii1;
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const double rn = pr * ri1 - pi * ii1;
-                    const double in = pr * ii1 + pi * ri1;
-                    pr = rn;
-                    pi = in;
-                    rsum -= (j + 1) * (ex[j] * pr - ex[P + j] * pi);
-                    isum -= (j + 1) * (ex[j] * pi + ex[P + j] * pr);
-                }
-                fx += rsum;
-                fy -= isum;
+                e2p_force64<P>(rz, iz, nd.mass, exps + (size_t)2 * P * id, fx, fy);
             }
             continue;
         }
@@ -500,6 +513,131 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
     }
 }
 
+// Force at arbitrary targets (no counterpart in the reference: main.cpp:191-195
+// rejects non-block targets for velocities).  Same masked wave traversal as
+// k_potential -- per-target accept test r^2 < theta^2 d^2 -- with the force
+// E2P (force-kernels.m4:57-98 per target) and the force P2P pair term
+// (force-kernels.m4:46-52).
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void k_force_targets(const TcNode *__restrict__ nodes,
+                                                       const double *__restrict__ exps,
+                                                       const float2 *__restrict__ exps32,
+                                                       const double4 *__restrict__ src,
+                                                       const float4 *__restrict__ src32,
+                                                       const double *__restrict__ xt, const double *__restrict__ yt,
+                                                       const uint32_t *__restrict__ tperm, int m, double th2,
+                                                       double *__restrict__ xres, double *__restrict__ yres,
+                                                       unsigned long long *counters)
+{
+    __shared__ int s_node[4][TC_STACK];
+    __shared__ unsigned long long s_mask[4][TC_STACK];
+    __shared__ double4 s_src[4][64];
+    const int wid = threadIdx.x >> 6, lane = tc_lane();
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < m;
+    const uint32_t ti = valid ? tperm[k] : 0u;
+    const double x = valid ? xt[ti] : 0.0, y = valid ? yt[ti] : 0.0;
+    const unsigned long long live = tc_ballot(valid);
+    if (!live)
+        return;
+    double fx = 0, fy = 0;
+    unsigned long long np2p = 0, ne2p = 0, nslots = 0;
+    s_node[wid][0] = 0;
+    s_mask[wid][0] = live;
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        tc_wave_sync();
+        const int id = tc_uniform(s_node[wid][sp]);
+        const unsigned long long mask = tc_uniform64(s_mask[wid][sp]);
+        const TcNode nd = nodes[id];
+        if (nd.s == nd.e)
+            continue;
+        const bool in = (mask >> lane) & 1ull;
+        const double dx = x - nd.xcom, dy = y - nd.ycom;
+        const double r2 = dx * dx + dy * dy;
+        const bool acc_ok = in && (nd.r * nd.r < th2 * r2);
+        const unsigned long long amask = tc_ballot(acc_ok);
+        if (acc_ok) {
+            if (F32) {
+                const float sc = (float)(nd.r > 1e-30 ? nd.r : 1e-30);
+                float gx, gy;
+                e2p_force32<P>((float)dx, (float)dy, (float)nd.mass, sc, exps32 + (size_t)P * id, gx, gy);
+                fx += (double)gx;
+                fy += (double)gy;
+            } else {
+                e2p_force64<P>(dx, dy, nd.mass, exps + (size_t)2 * P * id, fx, fy);
+            }
+        }
+        ne2p += __popcll(amask);
+        const unsigned long long omask = mask & ~amask;
+        if (!omask)
+            continue;
+        if (nd.child == 0) {
+            const bool act = (omask >> lane) & 1ull;
+            np2p += (unsigned long long)__popcll(omask) * (unsigned long long)(nd.e - nd.s);
+            nslots += 64ull * (unsigned long long)(nd.e - nd.s);
+            if (F32) {
+                const float tx = (float)(x - nd.xcom), ty = (float)(y - nd.ycom);
+                float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
+                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
+                    const int cnt = min(64, nd.e - b0);
+                    if (lane < cnt)
+                        tile[lane] = src32[b0 + lane];
+                    tc_wave_sync();
+                    if (act) {
+#pragma clang fp contract(fast)
+                        float ax32 = 0.f, ay32 = 0.f;
+                        for (int j = 0; j < cnt; ++j) {
+                            const float4 q = tile[j];
+                            const float ddx = tx - q.x, ddy = ty - q.y;
+                            const float f = q.z * fast_rcp(ddx * ddx + (ddy * ddy + EPS_REG32));
+                            ax32 += ddx * f;
+                            ay32 += ddy * f;
+                        }
+                        fx += (double)ax32;
+                        fy += (double)ay32;
+                    }
+                    tc_wave_sync();
+                }
+            } else {
+                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
+                    const int cnt = min(64, nd.e - b0);
+                    if (lane < cnt)
+                        s_src[wid][lane] = src[b0 + lane];
+                    tc_wave_sync();
+                    if (act) {
+#pragma clang fp contract(fast)
+                        for (int j = 0; j < cnt; ++j) {
+                            const double4 q = s_src[wid][j];
+                            const double ddx = x - q.x, ddy = y - q.y;
+                            const double f = q.z * rcp64(ddx * ddx + (ddy * ddy + EPS_REG));
+                            fx += ddx * f;
+                            fy += ddy * f;
+                        }
+                    }
+                    tc_wave_sync();
+                }
+            }
+        } else {
+            if (lane < 4) {
+                s_node[wid][sp + lane] = nd.child + lane;
+                s_mask[wid][sp + lane] = omask;
+            }
+            sp += 4;
+        }
+    }
+    if (valid) {
+        xres[ti] = fx;
+        yres[ti] = fy;
+    }
+    if (counters && lane == 0) {
+        atomicAdd(&counters[TC_CNT_P2P], np2p);
+        atomicAdd(&counters[TC_CNT_E2P], ne2p);
+        atomicAdd(&counters[TC_CNT_P2P_SLOTS], nslots);
+    }
+}
+
 // ------------------------------------------------------------------ direct sums
 // The FP64 O(N) sums of main.cpp:206-245 at selected targets, one block per
 // target: potential sum_j v_j log(sqrt(r^2 + 10 eps)), force sum_j v_j d/(r^2 + 10 eps).
@@ -564,6 +702,24 @@ void launch_potential(const TcTree &t, double th2, const double *xt, const doubl
         k_potential<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
                                                     t.src.as<double4>(), nullptr, xt, yt, tperm, m, th2, out,
                                                     counters);
+    TC_HIP(hipGetLastError());
+}
+
+template <int P>
+void launch_force_targets(const TcTree &t, double th2, const double *xt, const double *yt, const uint32_t *tperm,
+                          int m, double *xr, double *yr, int precision, unsigned long long *counters,
+                          hipStream_t st)
+{
+    const int grid = tiles_of(m, 256);
+    if (precision == 1)
+        k_force_targets<P, true><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(),
+                                                       t.exps32.as<float2>(), t.src.as<double4>(),
+                                                       t.src32.as<float4>(), xt, yt, tperm, m, th2, xr, yr,
+                                                       counters);
+    else
+        k_force_targets<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
+                                                        t.src.as<double4>(), nullptr, xt, yt, tperm, m, th2, xr,
+                                                        yr, counters);
     TC_HIP(hipGetLastError());
 }
 
@@ -665,6 +821,32 @@ void tc_eval_force(const TcTree &t, double theta, const double *x0s, const doubl
     case 24: launch_force<24>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
     case 32: launch_force<32>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
     default: throw TcError(-3, "tc_eval_force: unsupported order");
+    }
+    if (ev1)
+        TC_HIP(hipEventRecord(ev1, st));
+}
+
+void tc_eval_force_targets(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *xr,
+                           double *yr, int precision, unsigned long long *counters, TcScratch &ws,
+                           hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (m <= 0)
+        return;
+    if (precision == 1 && !t.has_fp32)
+        throw TcError(-3, "tc_eval_force_targets: tree was built without FP32 mirrors");
+    tc_sort_targets(xt, yt, m, ws, st);
+    const double th2 = theta * theta;
+    const uint32_t *tperm = ws.tperm.as<uint32_t>();
+    if (ev0)
+        TC_HIP(hipEventRecord(ev0, st));
+    switch (t.order) {
+    case 8: launch_force_targets<8>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+    case 12: launch_force_targets<12>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+    case 16: launch_force_targets<16>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+    case 20: launch_force_targets<20>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+    case 24: launch_force_targets<24>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+    case 32: launch_force_targets<32>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+    default: throw TcError(-3, "tc_eval_force_targets: unsupported order");
     }
     if (ev1)
         TC_HIP(hipEventRecord(ev1, st));

@@ -55,6 +55,15 @@ TC_API void treecode_force_mrag_float(const float theta, const float *const xsou
                                       const float *const hs, const int nblocks, float *const xresults,
                                       float *const yresults);
 
+/* Gradient at an arbitrary list of targets (the reference supports velocities only on
+ * 32x32 blocks, main.cpp:191-195): (xr, yr)[i] = sum_j v_j (t_i - s_j)/(|t_i - s_j|^2 + 10 eps).
+ * Order 24 / leaf 192 as the force build; per-target opening test r^2 < theta^2 d^2 of
+ * treecode-potential.cpp:48-50 with the force E2P and P2P kernels. */
+TC_API void treecode_force(const double theta, const double *const xsources, const double *const ysources,
+                           const double *const sourcevalues, const int nsources, const double *const xtargets,
+                           const double *const ytargets, const int ntargets, double *const xresults,
+                           double *const yresults);
+
 /* Velocities of an MRAG vortex solver from vorticity sources: the rotation and 2 pi
  * scaling that main.cpp:136-143 applies to compare against stored velocities,
  * u = grad_y / (2 pi), v = -grad_x / (2 pi) of treecode_force_mrag, same layout. */
@@ -101,6 +110,9 @@ TC_API int tc_potential(tc_context *ctx, double theta, const double *xs, const d
 TC_API int tc_force_mrag(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs,
                          int n, const double *x0s, const double *y0s, const double *hs, int nblocks, double *xr,
                          double *yr, int device_ptrs);
+TC_API int tc_force_targets(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs,
+                            int n, const double *xt, const double *yt, int m, double *xr, double *yr,
+                            int device_ptrs);
 TC_API int tc_synchronize(tc_context *ctx);
 
 /* Evaluate new targets against the tree left resident by the last tc_potential /

@@ -47,6 +47,8 @@ def lib():
                                      _D, _D, ctypes.c_int, ctypes.c_int, _LL, ctypes.c_int]
         L.orc_force_tree.argtypes = [ctypes.c_void_p, ctypes.c_double, _D, _D, _D, ctypes.c_int, ctypes.c_int,
                                      _D, _D, _LL, ctypes.c_int]
+        L.orc_force_targets.argtypes = [ctypes.c_double, _D, _D, _D, ctypes.c_int, _D, _D, ctypes.c_int, _D, _D,
+                                        ctypes.c_int, ctypes.c_int, _LL, ctypes.c_int]
         L.orc_direct_potential.argtypes = [_D, _D, _D, ctypes.c_int, _D, _D, _I, ctypes.c_int, _D, ctypes.c_int]
         L.orc_direct_force.argtypes = [_D, _D, _D, ctypes.c_int, _D, _D, _I, ctypes.c_int, _D, _D, ctypes.c_int]
         L.orc_p2e.argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D, _D]
@@ -171,6 +173,16 @@ def force_mrag(theta, xs, ys, vs, x0s, y0s, hs, order=24, leafcap=192, nthreads=
     cnt = np.zeros(4, np.int64)
     lib().orc_force_mrag(theta, _d(xs), _d(ys), _d(vs), len(xs), _d(x0s), _d(y0s), _d(hs), nb, _d(xr), _d(yr),
                          order, leafcap, cnt.ctypes.data_as(_LL), nthreads)
+    return xr, yr, cnt
+
+
+def force_targets(theta, xs, ys, vs, xt, yt, order=24, leafcap=192, nthreads=0):
+    xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
+    xr = np.zeros(len(xt))
+    yr = np.zeros(len(xt))
+    cnt = np.zeros(4, np.int64)
+    lib().orc_force_targets(theta, _d(xs), _d(ys), _d(vs), len(xs), _d(xt), _d(yt), len(xt), _d(xr), _d(yr),
+                            order, leafcap, cnt.ctypes.data_as(_LL), nthreads)
     return xr, yr, cnt
 
 

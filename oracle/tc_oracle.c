@@ -859,6 +859,82 @@ void orc_force_mrag(double theta, const double *xs, const double *ys, const doub
     orc_tree_free(t);
 }
 
+/* ---------------------------------------------------------------- force at arbitrary targets */
+
+/* The gradient at arbitrary targets (no counterpart in the reference, whose
+ * velocities exist only on 32x32 blocks, main.cpp:191-195): the per-target DFS
+ * and opening test of treecode-potential.cpp:25-86 with the force E2P of
+ * force-kernels.m4:57-98 and the pair term of force-kernels.m4:46-52. */
+static void eval_force_target(const orc_tree *t, double th2, double xt, double yt, double *fx, double *fy,
+                              long long *cnt)
+{
+    int stack[4 * LMAX + 8];
+    int sp = 0;
+    stack[0] = 0;
+    const int p = t->order;
+    double ax = 0, ay = 0;
+    while (sp > -1) {
+        const int id = stack[sp--];
+        if (t->s[id] == t->e[id])
+            continue; /* C7 */
+        const double dx = xt - t->xcom[id], dy = yt - t->ycom[id];
+        const double r2 = dx * dx + dy * dy;
+        if (t->r[id] * t->r[id] < th2 * r2) {
+            const double *ex = t->exp + (size_t)2 * p * id;
+            orc_e2p_force(t->mass[id], dx, dy, ex, ex + p, p, &ax, &ay);
+            cnt[ORC_CNT_E2P] += 1;
+        } else if (!t->child[id]) {
+            for (int i = t->s[id]; i < t->e[id]; ++i) {
+                const double xr = xt - t->xs[i], yr = yt - t->ys[i];
+                const double f = t->vs[i] / (xr * xr + (yr * yr + EPS_REG));
+                ax += xr * f;
+                ay += yr * f;
+            }
+            cnt[ORC_CNT_P2P_PAIRS] += t->e[id] - t->s[id];
+        } else {
+            for (int c = 0; c < 4; ++c)
+                stack[++sp] = t->child[id] + c;
+        }
+    }
+    *fx = ax;
+    *fy = ay;
+}
+
+void orc_force_targets_tree(const orc_tree *t, double theta, const double *xt, const double *yt, int t0, int t1,
+                            double *xr, double *yr, long long *counters, int nthreads)
+{
+    set_threads(nthreads);
+    const double th2 = theta * theta;
+    long long c0 = 0, c1 = 0;
+#pragma omp parallel for schedule(static, 1) reduction(+ : c0, c1)
+    for (int i = t0; i < t1; ++i) {
+        long long cnt[ORC_NCOUNTERS] = {0, 0, 0, 0};
+        eval_force_target(t, th2, xt[i], yt[i], &xr[i], &yr[i], cnt);
+        c0 += cnt[0];
+        c1 += cnt[1];
+    }
+    if (counters) {
+        counters[ORC_CNT_P2P_PAIRS] += c0;
+        counters[ORC_CNT_E2P] += c1;
+    }
+}
+
+void orc_force_targets(double theta, const double *xs, const double *ys, const double *vs, int n,
+                       const double *xt, const double *yt, int m, double *xr, double *yr, int order, int leafcap,
+                       long long *counters, int nthreads)
+{
+    if (order <= 0) order = 24;
+    if (leafcap <= 0) leafcap = 192;
+    if (n <= 0) {
+        memset(xr, 0, sizeof(double) * (size_t)m);
+        memset(yr, 0, sizeof(double) * (size_t)m);
+        return;
+    }
+    orc_tree *t = orc_tree_build(xs, ys, vs, n, leafcap, order);
+    orc_force_targets_tree(t, theta, xt, yt, 0, m, xr, yr, counters, nthreads);
+    orc_tree_free(t);
+}
+
 /* ---------------------------------------------------------------- direct sums */
 
 /* main.cpp:206-224 */

@@ -85,6 +85,14 @@ void orc_force_tree(const orc_tree *t, double theta, const double *x0s, const do
                     const double *hs, int b0, int b1, double *xr, double *yr,
                     long long *counters, int nthreads);
 
+/* Gradient at arbitrary targets: per-target DFS and opening test of the potential
+ * traversal with the force E2P / P2P terms (order 24, leaf 192 by default). */
+void orc_force_targets(double theta, const double *xs, const double *ys, const double *vs, int n,
+                       const double *xt, const double *yt, int m, double *xr, double *yr, int order, int leafcap,
+                       long long *counters, int nthreads);
+void orc_force_targets_tree(const orc_tree *t, double theta, const double *xt, const double *yt, int t0, int t1,
+                            double *xr, double *yr, long long *counters, int nthreads);
+
 /* FP64 direct sums at selected targets (main.cpp:206-245): out[k] for target idx[k]. */
 void orc_direct_potential(const double *xs, const double *ys, const double *vs, int n,
                           const double *xt, const double *yt, const int *idx, int nidx,

@@ -297,3 +297,20 @@ def test_log64_accuracy(pkg, ctx64):
     ref = np.log(xt * xt + 10 * np.finfo(float).eps)
     ulp = np.abs(out - ref) / np.spacing(np.abs(ref))
     assert ulp.max() <= 4, ulp.max()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_force_targets(pkg, oracle, wl, ctx64, ctx32, precision):
+    xs, ys, vs, *_ = wl.gaussian4_force(60000, seed=64)
+    st = wl.Stream(65)
+    xt, yt = st.uniform(9000, 0.1, 0.9), st.uniform(9000, 0.1, 0.9)
+    ctx = ctx32 if precision else ctx64
+    fx, fy = ctx.force_targets(0.5, xs, ys, vs, xt, yt)
+    rx, ry, cnt = oracle.force_targets(0.5, xs, ys, vs, xt, yt)
+    tol = FP32_TOL if precision else FP64_TOL
+    assert rel_l2(fx, rx) <= tol and rel_l2(fy, ry) <= tol
+    c = ctx.stats()["counters"]
+    assert (c["p2p"], c["e2p"]) == (cnt[0], cnt[1])
+    if not precision:
+        gx, gy = pkg.treecode_force(0.5, xs, ys, vs, xt, yt)
+        assert np.array_equal(gx, fx) and np.array_equal(gy, fy)

@@ -374,3 +374,16 @@ def test_bc_constants_round_to_nearest(oracle):
     # -1/l printed by `bc -l` with 20 digits parses to the nearest double of -1/l
     fx = load_fixture("neg_inv")
     assert np.array_equal(fx["bc20"], fx["nearest"])
+
+
+def test_force_targets_vs_direct(oracle, wl):
+    xs, ys, vs, xt, yt = wl.uniform_potential(4000, seed=19)
+    st = wl.Stream(20)
+    xt, yt = st.uniform(700, -0.2, 1.2), st.uniform(700, -0.2, 1.2)
+    idx = np.arange(len(xt), dtype=np.int32)
+    dx, dy = oracle.direct_force(xs, ys, vs, xt, yt, idx)
+    fx, fy, cnt = oracle.force_targets(0.0, xs, ys, vs, xt, yt)
+    assert rel_l2(fx, dx) <= 1e-13 and rel_l2(fy, dy) <= 1e-13
+    assert cnt[0] == len(xs) * len(xt)
+    fx, fy, cnt = oracle.force_targets(0.5, xs, ys, vs, xt, yt)
+    assert rel_l2(fx, dx) <= 1e-9 and rel_l2(fy, dy) <= 1e-9 and cnt[1] > 0
