@@ -364,11 +364,11 @@ class Context:
 
     def stats(self):
         t = np.zeros(4)
-        c = (ctypes.c_ulonglong * 5)()
+        c = (ctypes.c_ulonglong * 6)()
         info = np.zeros(4, np.int32)
         self._check(lib().tc_get_stats(self.h, _dp(t), c, info.ctypes.data_as(_I)))
         return {"times_ms": {"build": t[0], "tsort": t[1], "eval": t[2], "total": t[3]},
-                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3], "p2p_slots": c[4]},
+                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3], "p2p_slots": c[4], "e2p_slots": c[5]},
                 "n": int(info[0]), "nnodes": int(info[1]), "ninternal": int(info[2]), "order": int(info[3])}
 
     def tree(self):

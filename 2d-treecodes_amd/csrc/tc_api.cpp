@@ -34,7 +34,7 @@ struct tc_context {
     DevBuf io, cnt;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_valid = false, eval_has_tsort = false;
-    unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0, 0};
+    unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -601,7 +601,7 @@ TC_API int tc_morton_sort_device(tc_context *c, const double *x, const double *y
         if (n <= 0)
             throw TcError(-3, "tc_morton_sort_device: n must be positive");
         tc_bbox(x, y, n, box, c->ws, c->stream);
-        tc_morton_sort(x, y, n, box, keys, perm, c->ws, c->stream);
+        tc_morton_sort(x, y, nullptr, n, box, keys, perm, nullptr, c->ws, c->stream);
     });
 }
 

@@ -216,16 +216,57 @@ TC_DEV uint32_t quantise(double v, double vmin, double scale)
     return (uint32_t)q;
 }
 
-// sort-sources.cpp:87-110 (quantise + interleave, x in even bits)
-__global__ __launch_bounds__(256) void k_keys(const double *x, const double *y, int n, const double *box,
-                                               uint32_t *keys)
+// sort-sources.cpp:87-110 (quantise + interleave, x in even bits).  With
+// `packed`, also writes the source record {x, y, v, 0} in input order, so the
+// gather after the sort reads one 32-byte record per source.
+__global__ __launch_bounds__(256) void k_keys(const double *x, const double *y, const double *v, int n,
+                                               const double *box, uint32_t *keys, double4 *packed)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n)
         return;
     const double xmin = box[0], ymin = box[1], scale = box[3];
-    const uint32_t qx = quantise(x[i], xmin, scale), qy = quantise(y[i], ymin, scale);
+    const double xi = x[i], yi = y[i];
+    const uint32_t qx = quantise(xi, xmin, scale), qy = quantise(yi, ymin, scale);
     keys[i] = tc_spread15(qx) | (tc_spread15(qy) << 1);
+    if (packed)
+        packed[i] = make_double4(xi, yi, v[i], 0.0);
+}
+
+// Hilbert index of the quantised point (the standard quadrant-rotation
+// construction on a 2^15 x 2^15 grid, 30-bit result).  Used to order targets:
+// a Hilbert run of 64 points is more compact than a Morton run (no jumps
+// between distant quadrants), so the 64 lanes of a traversal wave agree on
+// more accept/open decisions.
+TC_DEV uint32_t hilbert15(uint32_t x, uint32_t y)
+{
+    const uint32_t n1 = (1u << TC_LMAX) - 1u;
+    uint32_t d = 0;
+    for (uint32_t s = 1u << (TC_LMAX - 1); s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u;
+        const uint32_t ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n1 - x;
+                y = n1 - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+__global__ __launch_bounds__(256) void k_keys_hilbert(const double *x, const double *y, int n, const double *box,
+                                                       uint32_t *keys)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const double xmin = box[0], ymin = box[1], scale = box[3];
+    keys[i] = hilbert15(quantise(x[i], xmin, scale), quantise(y[i], ymin, scale));
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -334,14 +375,13 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     }
 }
 
-__global__ __launch_bounds__(256) void k_gather_src(const double *x, const double *y, const double *v,
-                                                     const uint32_t *perm, int n, double4 *src)
+__global__ __launch_bounds__(256) void k_gather_src(const double4 *packed, const uint32_t *perm, int n,
+                                                     double4 *src)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n)
         return;
-    const uint32_t j = perm[i];
-    src[i] = make_double4(x[j], y[j], v[j], 0.0);
+    src[i] = packed[perm[i]];
 }
 
 // ------------------------------------------------------------------ topology
@@ -865,12 +905,16 @@ static const uint32_t *tc_bbox_bad_flag(TcScratch &ws, int n)
     return reinterpret_cast<const uint32_t *>(ws.bbox_part.as<double4>() + np);
 }
 
-void tc_morton_sort(const double *x, const double *y, int n, const double *box_d, uint32_t *keys_out,
-                    uint32_t *perm_out, TcScratch &ws, hipStream_t st)
+void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
+                    uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
+                    bool hilbert)
 {
     ws.k2.reserve(sizeof(uint32_t) * (size_t)n);
     ws.v2.reserve(sizeof(uint32_t) * (size_t)n);
-    k_keys<<<tiles_of(n, 256), 256, 0, st>>>(x, y, n, box_d, keys_out);
+    if (hilbert)
+        k_keys_hilbert<<<tiles_of(n, 256), 256, 0, st>>>(x, y, n, box_d, keys_out);
+    else
+        k_keys<<<tiles_of(n, 256), 256, 0, st>>>(x, y, v, n, box_d, keys_out, packed);
     TC_HIP(hipGetLastError());
     const int ntiles = tiles_of(n, RS_TILE);
     ws.hist.reserve(sizeof(uint32_t) * (size_t)RS_BINS * ntiles * 2);
@@ -942,8 +986,9 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     double *box = t.box_d.as<double>();
     tc_bbox(x, y, n, box, ws, st);
     uint32_t *keys = t.keys.as<uint32_t>(), *perm = t.perm.as<uint32_t>();
-    tc_morton_sort(x, y, n, box, keys, perm, ws, st);
-    k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(x, y, v, perm, n, t.src.as<double4>());
+    ws.packed.reserve(sizeof(double4) * (size_t)n);
+    tc_morton_sort(x, y, v, n, box, keys, perm, ws.packed.as<double4>(), ws, st, false);
+    k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(ws.packed.as<double4>(), perm, n, t.src.as<double4>());
     TC_HIP(hipGetLastError());
 
     // census of internal cells
@@ -1018,5 +1063,6 @@ void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, h
     ws.info.reserve(sizeof(int) * 32 + sizeof(double) * 4);
     double *box = reinterpret_cast<double *>(boxbuf.as<char>() + 128);
     tc_bbox(xt, yt, m, box, ws, st);
-    tc_morton_sort(xt, yt, m, box, ws.tkeys.as<uint32_t>(), ws.tperm.as<uint32_t>(), ws, st);
+    tc_morton_sort(xt, yt, nullptr, m, box, ws.tkeys.as<uint32_t>(), ws.tperm.as<uint32_t>(), nullptr, ws, st,
+                   true);
 }

@@ -29,9 +29,13 @@ struct alignas(16) TcNode {
 };
 static_assert(sizeof(TcNode) == 64, "TcNode must stay 64 B (one s_load_dwordx16)");
 
-// P2P_SLOTS = 64 x leaf size per wave-level leaf visit: the lane slots the masked
-// P2P loop occupies (P2P / P2P_SLOTS is its SIMD efficiency)
-enum TcCounter { TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_CNT_P2P_SLOTS = 4, TC_NCOUNTERS = 5 };
+// P2P_SLOTS = 64 x leaf size per wave-level leaf visit and E2P_SLOTS = 64 per
+// wave-level multipole evaluation: the lane slots the masked loops occupy
+// (P2P / P2P_SLOTS and E2P / E2P_SLOTS are their SIMD efficiencies)
+enum TcCounter {
+    TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_CNT_P2P_SLOTS = 4, TC_CNT_E2P_SLOTS = 5,
+    TC_NCOUNTERS = 6
+};
 
 struct TcError : std::runtime_error {
     int code;
@@ -180,19 +184,20 @@ struct TcTree {
 
 // Scratch reused across calls of one context.
 struct TcScratch {
-    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints;
+    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints, packed;
     void release()
     {
         for (DevBuf *b : {&k2, &v2, &hist, &part, &scan_tmp, &istart, &ilevel, &info, &tkeys, &tperm, &tsorted,
-                          &bbox_part, &ints})
+                          &bbox_part, &ints, &packed})
             b->release();
     }
 };
 
 // Internal entry points (tc_build.hip, tc_eval.hip)
 void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st);
-void tc_morton_sort(const double *x, const double *y, int n, const double *box_d, uint32_t *keys_out,
-                    uint32_t *perm_out, TcScratch &ws, hipStream_t st);
+void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
+                    uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
+                    bool hilbert = false);
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
                    bool fp32, TcScratch &ws, hipStream_t st);
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);

@@ -84,93 +84,88 @@ TC_DEV double log64(double x)
 }
 
 // ------------------------------------------------------------------ potential kernels
-// potential_e2p (potential-kernels.m4:59-89) for one expansion
+// potential_e2p (potential-kernels.m4:59-89) for one expansion:
+// Q log|z| + Re sum_{j=1..p} a_j u^j with u = 1/z, evaluated by Horner's rule
+// (4 FMAs per coefficient).
 template <int P>
 TC_DEV double e2p_pot64(double rz, double iz, double mass, const double *__restrict__ ex)
 {
-#pragma clang fp contract(fast)
-    const double r2 = rz * rz + iz * iz;
+    const double r2 = fma(rz, rz, iz * iz);
     const double inv = 1.0 / r2;
-    const double ri1 = rz * inv, ii1 = -iz * inv;
-    double ri = ri1, ii = ii1;
-    double sum = ex[0] * ri - ex[P] * ii;
+    const double ur = rz * inv, ui = -iz * inv;
+    double ar = ex[P - 1], ai = ex[2 * P - 1];
 #pragma unroll
-    for (int j = 1; j < P; ++j) {
-        const double rn = ri * ri1 - ii * ii1;
-        const double in = ri * ii1 + ii * ri1;
-        ri = rn;
-        ii = in;
-        sum += ex[j] * ri - ex[P + j] * ii;
+    for (int j = P - 2; j >= 0; --j) {
+        const double tr = fma(ur, ar, fma(-ui, ai, ex[j]));
+        const double ti = fma(ur, ai, fma(ui, ar, ex[P + j]));
+        ar = tr;
+        ai = ti;
     }
-    return mass * log64(r2) * 0.5 + sum;
+    return fma(mass * 0.5, log64(r2), fma(ur, ar, -ui * ai));
 }
 
-// FP32 E2P on scaled coefficients b_j = a_j / s^j:  Re sum b_j (s/z)^j.
+// FP32 E2P on scaled coefficients b_j = a_j / s^j: Re sum b_j w^j, w = s/z (Horner).
 template <int P>
 TC_DEV float e2p_pot32(float rz, float iz, float mass, float sc, const float2 *__restrict__ ex)
 {
-#pragma clang fp contract(fast)
-    const float r2 = rz * rz + iz * iz;
+    const float r2 = fmaf(rz, rz, iz * iz);
     const float inv = fast_rcp(r2);
     const float wr = sc * rz * inv, wi = -sc * iz * inv;
-    float pr = wr, pi = wi;
-    float sum = ex[0].x * pr - ex[0].y * pi;
+    float ar = ex[P - 1].x, ai = ex[P - 1].y;
 #pragma unroll
-    for (int j = 1; j < P; ++j) {
-        const float rn = pr * wr - pi * wi;
-        const float in = pr * wi + pi * wr;
-        pr = rn;
-        pi = in;
-        sum += ex[j].x * pr - ex[j].y * pi;
+    for (int j = P - 2; j >= 0; --j) {
+        const float tr = fmaf(wr, ar, fmaf(-wi, ai, ex[j].x));
+        const float ti = fmaf(wr, ai, fmaf(wi, ar, ex[j].y));
+        ar = tr;
+        ai = ti;
     }
-    return mass * fast_log2(r2) * LN2_HALF + sum;
+    return fmaf(mass * LN2_HALF, fast_log2(r2), fmaf(wr, ar, -wi * ai));
 }
 
-// force_e2p_8x8 (force-kernels.m4:57-98) for one target: fx += Re dphi/dz, fy -= Im dphi/dz
+// force_e2p_8x8 (force-kernels.m4:57-98) for one target: dphi/dz =
+// Q u - sum_k k a_k u^(k+1) = u (Q - u T), T = sum_k k a_k u^(k-1) by Horner;
+// fx += Re dphi/dz, fy -= Im dphi/dz.
 template <int P>
 TC_DEV void e2p_force64(double rz, double iz, double mass, const double *__restrict__ ex, double &fx, double &fy)
 {
-#pragma clang fp contract(fast)
-    const double inv = 1.0 / (rz * rz + iz * iz);
-    const double ri1 = rz * inv, ii1 = -iz * inv;
-    double rsum = mass * ri1, isum = mass * ii1;
-    double pr = ri1, pi = ii1;
+    const double inv = 1.0 / fma(rz, rz, iz * iz);
+    const double ur = rz * inv, ui = -iz * inv;
+    double ar = (double)P * ex[P - 1], ai = (double)P * ex[2 * P - 1];
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
-        const double rn = pr * ri1 - pi * ii1;
-        const double in = pr * ii1 + pi * ri1;
-        pr = rn;
-        pi = in;
-        rsum -= (j + 1) * (ex[j] * pr - ex[P + j] * pi);
-        isum -= (j + 1) * (ex[j] * pi + ex[P + j] * pr);
+    for (int j = P - 2; j >= 0; --j) {
+        const double k = (double)(j + 1);
+        const double tr = fma(ur, ar, fma(-ui, ai, k * ex[j]));
+        const double ti = fma(ur, ai, fma(ui, ar, k * ex[P + j]));
+        ar = tr;
+        ai = ti;
     }
-    fx += rsum;
-    fy -= isum;
+    const double vr = mass - fma(ur, ar, -ui * ai);
+    const double vi = -fma(ur, ai, ui * ar);
+    fx += fma(ur, vr, -ui * vi);
+    fy -= fma(ur, vi, ui * vr);
 }
 
-// FP32 force E2P on scaled coefficients: dphi/dz = (Q - sum j b_j w^j) / z, w = s/z.
+// FP32 force E2P on scaled coefficients: dphi/dz = u (Q - w T'), w = s u,
+// T' = sum_k k b_k w^(k-1) by Horner.
 template <int P>
 TC_DEV void e2p_force32(float zr, float zi, float mass, float sc, const float2 *__restrict__ ex, float &gx, float &gy)
 {
-#pragma clang fp contract(fast)
-    const float inv = fast_rcp(zr * zr + zi * zi);
-    const float ir = zr * inv, iim = -zi * inv;
-    const float wr = sc * ir, wi = sc * iim;
-    float pr = 1.f, pi = 0.f, sr = 0.f, si = 0.f;
+    const float inv = fast_rcp(fmaf(zr, zr, zi * zi));
+    const float ur = zr * inv, ui = -zi * inv;
+    const float wr = sc * ur, wi = sc * ui;
+    float ar = (float)P * ex[P - 1].x, ai = (float)P * ex[P - 1].y;
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
-        const float rn = pr * wr - pi * wi;
-        const float in = pr * wi + pi * wr;
-        pr = rn;
-        pi = in;
-        const float cr = ex[j].x * pr - ex[j].y * pi;
-        const float ci = ex[j].x * pi + ex[j].y * pr;
-        sr += (float)(j + 1) * cr;
-        si += (float)(j + 1) * ci;
+    for (int j = P - 2; j >= 0; --j) {
+        const float k = (float)(j + 1);
+        const float tr = fmaf(wr, ar, fmaf(-wi, ai, k * ex[j].x));
+        const float ti = fmaf(wr, ai, fmaf(wi, ar, k * ex[j].y));
+        ar = tr;
+        ai = ti;
     }
-    const float qr = mass - sr, qi = -si;
-    gx = qr * ir - qi * iim;
-    gy = -(qr * iim + qi * ir);
+    const float vr = mass - fmaf(wr, ar, -wi * ai);
+    const float vi = -fmaf(wr, ai, wi * ar);
+    gx = fmaf(ur, vr, -ui * vi);
+    gy = -fmaf(ur, vi, ui * vr);
 }
 
 template <int P, bool F32>
@@ -194,7 +189,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     if (!live)
         return;
     double acc = 0;
-    unsigned long long np2p = 0, ne2p = 0, nslots = 0;
+    unsigned long long np2p = 0, ne2p = 0, nslots = 0, neslots = 0;
     int sp = 0;
     s_node[wid][0] = 0;
     s_mask[wid][0] = live;
@@ -221,6 +216,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
             }
         }
         ne2p += __popcll(amask);
+        neslots += amask ? 64ull : 0ull;
         const unsigned long long omask = mask & ~amask;
         if (!omask)
             continue;
@@ -307,6 +303,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], nslots);
+        atomicAdd(&counters[TC_CNT_E2P_SLOTS], neslots);
     }
 }
 
@@ -510,6 +507,7 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
         atomicAdd(&counters[TC_CNT_E2L], ne2l);
         atomicAdd(&counters[TC_CNT_L2P], 64ull);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
+        atomicAdd(&counters[TC_CNT_E2P_SLOTS], ne2p);
     }
 }
 
@@ -541,7 +539,7 @@ __global__ __launch_bounds__(256) void k_force_targets(const TcNode *__restrict_
     if (!live)
         return;
     double fx = 0, fy = 0;
-    unsigned long long np2p = 0, ne2p = 0, nslots = 0;
+    unsigned long long np2p = 0, ne2p = 0, nslots = 0, neslots = 0;
     s_node[wid][0] = 0;
     s_mask[wid][0] = live;
     int sp = 1;
@@ -570,6 +568,7 @@ __global__ __launch_bounds__(256) void k_force_targets(const TcNode *__restrict_
             }
         }
         ne2p += __popcll(amask);
+        neslots += amask ? 64ull : 0ull;
         const unsigned long long omask = mask & ~amask;
         if (!omask)
             continue;
@@ -635,6 +634,7 @@ __global__ __launch_bounds__(256) void k_force_targets(const TcNode *__restrict_
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], nslots);
+        atomicAdd(&counters[TC_CNT_E2P_SLOTS], neslots);
     }
 }
 

@@ -89,7 +89,7 @@ enum {
     TC_OPT_FORCE_ORDER = 3,    /* force expansion order, default 24 */
     TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
     TC_OPT_PROFILE = 5,        /* 1 = record phase times with HIP events */
-    TC_OPT_COUNTERS = 6        /* 1 = count interactions (p2p pairs, e2p, e2l, l2p, p2p lane slots) */
+    TC_OPT_COUNTERS = 6        /* 1 = count interactions (see tc_get_stats) */
 };
 
 /* Phase times (tc_get_stats times_ms[]) */
@@ -123,10 +123,11 @@ TC_API int tc_potential_resident(tc_context *ctx, double theta, const double *xt
 TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x0s, const double *y0s,
                                   const double *hs, int nblocks, double *xr, double *yr, int device_ptrs);
 
-/* Times of the last call's phases (requires TC_OPT_PROFILE), counters[5] (requires
+/* Times of the last call's phases (requires TC_OPT_PROFILE), counters[6] (requires
  * TC_OPT_COUNTERS): {target-source pairs evaluated directly, (target, node) multipole
  * evaluations, (brick, node) multipole-to-local shifts, targets given a local expansion,
- * lane slots occupied by the direct loops}, and tree_info = {n, nnodes, ninternal, order}. */
+ * lane slots occupied by the direct loops, lane slots occupied by multipole evaluations},
+ * and tree_info = {n, nnodes, ninternal, order}. */
 TC_API int tc_get_stats(tc_context *ctx, double *times_ms, unsigned long long *counters, int *tree_info);
 
 /* Copy the resident tree to host memory (any pointer may be NULL):

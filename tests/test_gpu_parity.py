@@ -314,3 +314,14 @@ def test_force_targets(pkg, oracle, wl, ctx64, ctx32, precision):
     if not precision:
         gx, gy = pkg.treecode_force(0.5, xs, ys, vs, xt, yt)
         assert np.array_equal(gx, fx) and np.array_equal(gy, fy)
+
+
+def test_target_order_invariance(pkg, wl, ctx64, ctx32):
+    # each lane accumulates in its own target's DFS order, so results do not depend on
+    # how targets are grouped into waves: a permutation of the inputs permutes the outputs
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(50000, seed=66)
+    perm = np.random.default_rng(1).permutation(len(xt))
+    for ctx in (ctx64, ctx32):
+        a = ctx.potential(0.5, xs, ys, vs, xt, yt)
+        b = ctx.potential(0.5, xs, ys, vs, xt[perm], yt[perm])
+        assert np.array_equal(a[perm], b)
