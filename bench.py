@@ -86,6 +86,7 @@ class GpuCase:
             kw.update(force_order=order or 24, force_leaf=leaf or 192)
         self.ctx = pkg.Context(**kw)
         self.order = order or (12 if kind == "potential" else 24)
+        self.leaf = leaf or (512 if kind == "potential" else 192)
         self.d = {k: self.ctx.to_device(v) for k, v in data.items()}
         if kind == "potential":
             self.m = len(data["xt"])
@@ -276,6 +277,8 @@ def main():
                     help="requests of the served stream replayed after the solve timing (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate of the stream (1/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--tune", action="store_true",
+                    help="choose order and leaf capacity per precision with the autotuner (rel err <= 1e-6)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -300,15 +303,29 @@ def main():
     p = args.order or (12 if kind == "potential" else 24)
     cases = {}
     timings = {}
+    tuned = {}
     for prec in (["f32", "f64"] if args.only is None else [args.only]):
+        order, leaf = args.order, args.leaf
+        if args.tune:
+            from treecodes2d_amd import autotune
+            log("autotuning", prec)
+            grid = dict(orders=(8, 12, 16), leaves=(128, 256, 512, 1024)) if kind == "potential" else \
+                dict(orders=(16, 20, 24), leaves=(64, 128, 192, 384))
+            res = autotune.Tuner(pkg).tune(kind, data, theta=args.theta, precision=1 if prec == "f32" else 0,
+                                           err_target=5e-7, reps=2, **grid)
+            order, leaf = res["order"], res["leaf"]
+            tuned[prec] = {"order": order, "leaf": leaf, "err_sampled": res["err"],
+                           "table": [(t["order"], t["leaf"], round(t["ms"], 3), t["err"]) for t in res["table"]]}
+            log("  chose order %d leaf %d (%.3f ms, err %.2e)" % (order, leaf, res["ms"], res["err"]))
         log("timing", prec)
-        c = GpuCase(kind, data, 1 if prec == "f32" else 0, args.theta, args.order, args.leaf)
+        c = GpuCase(kind, data, 1 if prec == "f32" else 0, args.theta, order, leaf)
         dt, phases = c.timed(args.steps, args.warmup)
         cases[prec] = c
         timings[prec] = (dt, phases)
         log("  %s: %.3f ms/step, phases %s" % (prec, dt * 1e3, {k: round(v, 3) for k, v in phases.items()}))
 
     head = "f32" if "f32" in cases else "f64"
+    p = cases[head].order
     dt, phases = timings[head]
     m = cases[head].m
     st = cases[head].counters()
@@ -317,6 +334,13 @@ def main():
     eval_ms = phases["eval"]
     achieved = fl / (eval_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[head]
+    traffic, traffic_src = None, None
+    tfile = os.path.join(ROOT, "profiles", "potential_2e24_traffic.json")
+    if kind == "potential" and head == "f32" and os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        if tj["workload"] == desc and tj["order"] == p and tj["theta"] == args.theta and \
+                tj["leaf"] == (cases[head].leaf):
+            traffic, traffic_src = tj["hbm_bytes_per_launch"], "profiles/potential_2e24_hbm_traffic.md"
 
     errs = None
     if not args.no_error and "f32" in cases and "f64" in cases:
@@ -347,14 +371,13 @@ def main():
         "vs_baseline": None,
         "dtype": head,
         "data": "synthetic",
-        "config": {"workload": desc, "theta": args.theta, "order": p,
-                   "leaf": args.leaf or (512 if kind == "potential" else 192),
+        "config": {"workload": desc, "theta": args.theta, "order": p, "leaf": cases[head].leaf,
                    "precision": "FP32 P2P/E2P on leaf-relative offsets, FP64 decisions/P2E/E2E/E2L"
                    if head == "f32" else "plain FP64", "inputs": "resident in HBM"},
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "counters": {k: int(v) for k, v in cnt.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_potential (traversal)" if kind == "potential" else "k_force (traversal)",
                      "note": "VALU-bound: peak is the vector %s rate (no MFMA in this path)" % head.upper()},
         "cpu_baseline": cpu,
@@ -366,6 +389,10 @@ def main():
         res["fp64_phases_ms"] = {k: round(v, 4) for k, v in timings["f64"][1].items()}
     if errs:
         res["rel_err_vs_direct"] = errs
+    if tuned:
+        res["autotune"] = tuned
+        if "f64" in cases:
+            res["fp64_config"] = {"order": cases["f64"].order, "leaf": cases["f64"].leaf}
     print(json.dumps(res), flush=True)
 
 
