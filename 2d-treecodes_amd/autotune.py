@@ -18,6 +18,7 @@ import time
 import numpy as np
 
 DEFAULT = {"potential": (12, 512), "force": (24, 192)}
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 ORDERS = (8, 12, 16, 20, 24, 32)
 LEAVES = (64, 128, 192, 256, 384, 512, 768, 1024)
 
@@ -28,7 +29,7 @@ def bucket_key(kind, precision, theta, n, m):
 
 
 class Tuner:
-    def __init__(self, pkg, cache_path=None, device=0):
+    def __init__(self, pkg, cache_path=CACHE, device=0):
         self.pkg = pkg
         self.device = device
         self.cache_path = cache_path
@@ -38,7 +39,9 @@ class Tuner:
                 self.cache = json.load(f)
 
     def lookup(self, kind, precision, theta, n, m):
-        return self.cache.get(bucket_key(kind, precision, theta, n, m))
+        """Tuned (order, leaf) for this size bucket, or None."""
+        e = self.cache.get(bucket_key(kind, precision, theta, n, m))
+        return (e["order"], e["leaf"]) if e else None
 
     def _save(self):
         if self.cache_path:
@@ -48,7 +51,7 @@ class Tuner:
             os.replace(tmp, self.cache_path)
 
     def tune(self, kind, data, theta=0.5, precision=1, orders=ORDERS, leaves=LEAVES, err_target=None,
-             nsample=1024, reps=3, seed=0, log=None):
+             nsample=1024, reps=3, seed=0, log=None, save=True):
         """data: potential {xs, ys, vs, xt, yt}; force {xs, ys, vs, x0s, y0s, hs}.
         Returns {"order", "leaf", "ms", "err", "table": [...]} and records it in the cache."""
         pkg = self.pkg
@@ -111,8 +114,9 @@ class Tuner:
         res = {"order": best["order"], "leaf": best["leaf"], "ms": best["ms"], "err": best["err"],
                "err_target": err_target, "table": table}
         n = len(data["xs"])
-        self.cache[bucket_key(kind, precision, theta, n, m)] = {k: res[k] for k in ("order", "leaf", "ms", "err",
-                                                                                     "err_target")}
-        self._save()
+        if save:
+            self.cache[bucket_key(kind, precision, theta, n, m)] = {k: res[k] for k in ("order", "leaf", "ms", "err",
+                                                                                         "err_target")}
+            self._save()
         ctx.close()
         return res

@@ -308,6 +308,67 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 }
 
 // ------------------------------------------------------------------ force kernels
+// Multipole-to-local (downward_e2l, force-kernels.m4:102-155) for up to two
+// nodes at once: lanes 0-31 take node q[0], lanes 32-63 node q[1]; lane l of a
+// half owns local coefficient l+1.  z0 = com - brick centre, u = 1/z0; each
+// lane forms u^(l+1) by binary powering, publishes (-1)^k a_k u^k, then sums
+// C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
+template <int P>
+TC_DEV void e2l_pair(const TcNode *__restrict__ nodes, const double *__restrict__ exps, const int *q, int nq,
+                     double x0b, double y0b, double (*s_binom)[64], double2 *s_c, double &lre, double &lim)
+{
+#pragma clang fp contract(fast)
+    const int lane = tc_lane();
+    const int h = lane >> 5, l = lane & 31;
+    const bool on = h < nq && l < P;
+    double ur = 0, ui = 0, pr = 1, pi = 0, mass = 0;
+    if (h < nq) {
+        const int id = q[h];
+        const TcNode nd = nodes[id];
+        mass = nd.mass;
+        const double zx = nd.xcom - x0b, zy = nd.ycom - y0b;
+        const double inv = 1.0 / (zx * zx + zy * zy);
+        ur = zx * inv;
+        ui = -zy * inv;
+        double br = ur, bi = ui;
+        int e = l + 1;
+#pragma unroll
+        for (int bit = 0; bit < 6; ++bit) {
+            if (e & 1) {
+                const double tr = pr * br - pi * bi;
+                pi = pr * bi + pi * br;
+                pr = tr;
+            }
+            e >>= 1;
+            const double sr = br * br - bi * bi;
+            bi = 2.0 * br * bi;
+            br = sr;
+        }
+        if (on) {
+            const double *ex = exps + (size_t)2 * P * id;
+            const double ar = ex[l], ai = ex[P + l];
+            const double sgn = (l & 1) ? 1.0 : -1.0;  // (-1)^k, k = l + 1
+            s_c[h * 33 + l + 1] = make_double2(sgn * (ar * pr - ai * pi), sgn * (ar * pi + ai * pr));
+        }
+    }
+    tc_wave_sync();
+    if (on) {
+        const int L = l + 1;
+        double rt = (L == 1) ? -mass : mass * (-(1.0 / (double)L));
+        double it = 0;
+        const double2 *c = s_c + h * 33;
+        for (int kk = 1; kk <= P; ++kk) {
+            const double bb = s_binom[kk - 1][L + kk - 1];
+            const double2 ck = c[kk];
+            rt += bb * ck.x;
+            it += bb * ck.y;
+        }
+        lre += rt * pr - it * pi;
+        lim += rt * pi + it * pr;
+    }
+    tc_wave_sync();
+}
+
 template <int P, bool F32>
 __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
@@ -318,7 +379,8 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
 {
     __shared__ int s_node[4][TC_STACK];
     __shared__ double4 s_src[4][64];
-    __shared__ double2 s_coef[4][TC_MAXORDER + 1];
+    __shared__ double2 s_coef[4][2 * 33];
+    __shared__ int s_e2lq[4][2];
     __shared__ double s_binom[TC_MAXORDER][64];
 
     for (int i = threadIdx.x; i < P * 64; i += 256)
@@ -348,6 +410,7 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
     double fx = 0, fy = 0;
     double lre = 0, lim = 0;
     unsigned long long np2p = 0, ne2p = 0, ne2l = 0;
+    int nq = 0;
 
     int sp = 1;
     s_node[wid][0] = 0;
@@ -362,51 +425,15 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
         const double ddx = x0b - xcom, ddy = y0b - ycom;
         const double distance = sqrt(ddx * ddx + ddy * ddy);
         if ((distance / r - 1) > invtheta && rbrick <= r) {
-            // downward_e2l (force-kernels.m4:102-155), z0 = com - brick centre
+            // queued; downward_e2l runs on pairs of nodes (e2l_pair)
             ++ne2l;
-            const double zx = xcom - x0b, zy = ycom - y0b;
-            const double *ex = exps + (size_t)2 * P * id;
-            double ri, ii;
-            {
-#pragma clang fp contract(fast)
-                const double r2z0 = zx * zx + zy * zy;
-                const double inv = 1.0 / r2z0;
-                const double ri1 = zx * inv, ii1 = -zy * inv;
-                ri = ri1;
-                ii = ii1;
-                double pr = ri1, pi = ii1;
-                for (int j = 2; j <= P; ++j) {
-                    const double rn = pr * ri1 - pi * ii1;
-                    const double in = pr * ii1 + pi * ri1;
-                    pr = rn;
-                    pi = in;
-                    if (j == lane + 1) {
-                        ri = pr;
-                        ii = pi;
-                    }
-                }
-                if (lane < P) {
-                    const double ar = ex[lane], ai = ex[P + lane];
-                    const double sgn = (lane & 1) ? 1.0 : -1.0;  // (-1)^k with k = lane+1
-                    s_coef[wid][lane + 1] = make_double2(sgn * (ar * ri - ai * ii), sgn * (ar * ii + ai * ri));
-                }
+            s_e2lq[wid][nq] = id;
+            ++nq;
+            if (nq == 2) {
+                tc_wave_sync();
+                e2l_pair<P>(nodes, exps, s_e2lq[wid], 2, x0b, y0b, s_binom, s_coef[wid], lre, lim);
+                nq = 0;
             }
-            tc_wave_sync();
-            if (lane < P) {
-#pragma clang fp contract(fast)
-                const int L = lane + 1;
-                double rt = (L == 1) ? -nd.mass : nd.mass * (-(1.0 / (double)L));
-                double it = 0;
-                for (int kk = 1; kk <= P; ++kk) {
-                    const double bb = s_binom[kk - 1][L + kk - 1];
-                    const double2 c = s_coef[wid][kk];
-                    rt += bb * c.x;
-                    it += bb * c.y;
-                }
-                lre += rt * ri - it * ii;
-                lim += rt * ii + it * ri;
-            }
-            tc_wave_sync();
             continue;
         }
         const double cxt = xlo < (xcom < xhi ? xcom : xhi) ? (xcom < xhi ? xcom : xhi) : xlo;
@@ -475,6 +502,13 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
         sp += 4;
     }
 
+    if (nq) {
+        tc_wave_sync();
+        e2l_pair<P>(nodes, exps, s_e2lq[wid], nq, x0b, y0b, s_binom, s_coef[wid], lre, lim);
+    }
+    // fold the two half-wave partial local expansions
+    lre += __shfl_xor(lre, 32, 64);
+    lim += __shfl_xor(lim, 32, 64);
     // downward_l2p_8x8 (force-kernels.m4:157-191) about the brick centre
     if (lane < P)
         s_coef[wid][lane + 1] = make_double2(lre, lim);

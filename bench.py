@@ -278,7 +278,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate of the stream (1/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--tune", action="store_true",
-                    help="choose order and leaf capacity per precision with the autotuner (rel err <= 1e-6)")
+                    help="re-run the autotuner for order and leaf capacity per precision (rel err <= 5e-7)")
+    ap.add_argument("--reference-params", action="store_true",
+                    help="use the reference's order/leaf (12/512 potential, 24/192 force) instead of tuned ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -306,10 +308,18 @@ def main():
     tuned = {}
     for prec in (["f32", "f64"] if args.only is None else [args.only]):
         order, leaf = args.order, args.leaf
+        from treecodes2d_amd import autotune
+        if not (args.tune or args.reference_params or order or leaf):
+            m_t = len(data["xt"]) if kind == "potential" else len(data["x0s"]) * 1024
+            hit = autotune.Tuner(pkg).lookup(kind, 1 if prec == "f32" else 0, args.theta, len(data["xs"]), m_t)
+            if hit:
+                order, leaf = hit
+                tuned[prec] = {"order": order, "leaf": leaf, "source": "2d-treecodes_amd/tuned.json"}
+                log("  tuned parameters for %s: order %d leaf %d" % (prec, order, leaf))
         if args.tune:
             from treecodes2d_amd import autotune
             log("autotuning", prec)
-            grid = dict(orders=(8, 12, 16), leaves=(128, 256, 512, 1024)) if kind == "potential" else \
+            grid = dict(orders=(8, 12, 16), leaves=(64, 128, 256, 512)) if kind == "potential" else \
                 dict(orders=(16, 20, 24), leaves=(64, 128, 192, 384))
             res = autotune.Tuner(pkg).tune(kind, data, theta=args.theta, precision=1 if prec == "f32" else 0,
                                            err_target=5e-7, reps=2, **grid)
