@@ -83,6 +83,41 @@ TC_DEV double log64(double x)
     return fma(kd, 0.6931471803691238, fma(kd, 1.9082149292705877e-10, logm));
 }
 
+// The force traversal's stack holds node records in LDS: opening a node loads
+// the first 48 bytes (centre, radius, mass, source range, first child, level)
+// of its four children's records with one wave-wide vector load (lanes 0-11,
+// 16 bytes each), so a pop reads LDS instead of waiting on a dependent load.
+// That kernel runs 3-4 waves per SIMD and was latency-bound on its scalar node
+// loads; the potential traversals run 8 and hide those loads, and there the
+// extra LDS reads only cost issue slots (measured 7% slower), so they keep
+// scalar loads.
+struct alignas(16) TcStackRec {
+    double xcom, ycom, r, mass;
+    int s, e, child, level;
+};
+static_assert(sizeof(TcStackRec) == 48 && offsetof(TcNode, pad1) == 60, "stack record is TcNode's first 48 B");
+
+TC_DEV void stack_load(const TcNode *__restrict__ nodes, int first, int count, TcStackRec *stk)
+{
+    const int lane = tc_lane();
+    if (lane < 3 * count) {
+        const int c = lane / 3, part = lane - 3 * c;
+        const double2 v = reinterpret_cast<const double2 *>(nodes + first + c)[part];
+        reinterpret_cast<double2 *>(stk + c)[part] = v;
+    }
+}
+
+// A stack record read back; the fields that steer control flow are made
+// wave-uniform (scalar registers).
+TC_DEV TcStackRec stack_pop(const TcStackRec *stk)
+{
+    TcStackRec nd = *stk;
+    nd.s = tc_uniform(nd.s);
+    nd.e = tc_uniform(nd.e);
+    nd.child = tc_uniform(nd.child);
+    return nd;
+}
+
 // ------------------------------------------------------------------ potential kernels
 // potential_e2p (potential-kernels.m4:59-89) for one expansion:
 // Q log|z| + Re sum_{j=1..p} a_j u^j with u = 1/z, evaluated by Horner's rule
@@ -378,6 +413,7 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
                                                double *__restrict__ yres, unsigned long long *counters)
 {
     __shared__ int s_node[4][TC_STACK];
+    __shared__ TcStackRec s_stk[4][TC_STACK];
     __shared__ double4 s_src[4][64];
     __shared__ double2 s_coef[4][2 * 33];
     __shared__ int s_e2lq[4][2];
@@ -414,11 +450,12 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
 
     int sp = 1;
     s_node[wid][0] = 0;
+    stack_load(nodes, 0, 1, &s_stk[wid][0]);
     while (sp > 0) {
         --sp;
         tc_wave_sync();
         const int id = tc_uniform(s_node[wid][sp]);
-        const TcNode nd = nodes[id];
+        const TcStackRec nd = stack_pop(&s_stk[wid][sp]);
         if (nd.s == nd.e)
             continue;
         const double xcom = nd.xcom, ycom = nd.ycom, r = nd.r;
@@ -499,6 +536,7 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
         }
         if (lane < 4)
             s_node[wid][sp + lane] = nd.child + lane;
+        stack_load(nodes, nd.child, 4, &s_stk[wid][sp]);
         sp += 4;
     }
 
