@@ -103,7 +103,7 @@ void build(tc_context *c, const double *xs, const double *ys, const double *vs, 
 {
     const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
     const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
-    tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, c->ws, c->stream);
+    tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws, c->stream);
     c->tree_kind = kind;
     c->tree_precision = c->precision;
 }
@@ -114,6 +114,8 @@ void check_resident(tc_context *c, int kind)
     const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
     if (c->tree_kind == TC_KIND_NONE || c->tree.n <= 0)
         throw TcError(-5, "treecode: no resident tree in this context");
+    if (c->tree_kind != kind)
+        throw TcError(-5, "treecode: resident tree was built for the other kernel (potential vs force)");
     if (c->tree.order != order || c->tree.leafcap != leaf || (c->precision == 1 && !c->tree.has_fp32))
         throw TcError(-5, "treecode: resident tree was built with other order/leaf/precision");
 }
@@ -230,8 +232,9 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
         if (rebuild)
             build(c, dxs, dys, dvs, n, TC_KIND_FORCE);
         record(c, 1);
-        tc_eval_force(c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->stream,
-                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+        c->ws.work.reserve(256);
+        tc_eval_force(c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws.work.as<uint32_t>(),
+                      c->stream, c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
         c->eval_has_tsort = false;
         if (!device_ptrs) {

@@ -295,45 +295,38 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
                                                        uint32_t *vout, int n, int shift, const uint32_t *offs,
                                                        int ntiles)
 {
+    constexpr int NW = RS_NT / 64, WITEMS = 64 * RS_ITEMS;  // waves per tile, keys per wave
     __shared__ uint32_t s_k[RS_TILE];
     __shared__ uint32_t s_v[RS_TILE];
     __shared__ uint32_t s_tileoff[RS_BINS];
     __shared__ uint32_t s_gbase[RS_BINS];
-    __shared__ uint32_t s_run[RS_BINS];
-    __shared__ uint32_t s_wcnt[RS_NT / 64][RS_BINS];
-    __shared__ uint32_t s_w[RS_NT / 64 + 1];
+    __shared__ uint32_t s_wrun[NW][RS_BINS];  // per-wave digit counts, then per-wave tile offsets
+    __shared__ uint32_t s_w[NW + 1];
 
     const int tid = threadIdx.x, lane = tc_lane(), wid = tid >> 6;
     const int base = blockIdx.x * RS_TILE;
     const int cnt = min(RS_TILE, n - base);
-
-    s_run[tid] = 0;
-    for (int w = 0; w < RS_NT / 64; ++w)
-        s_wcnt[w][tid] = 0;
+    for (int d = lane; d < RS_BINS; d += 64)
+        s_wrun[wid][d] = 0;
     s_gbase[tid] = offs[tid * ntiles + blockIdx.x];
-    __syncthreads();
+    tc_wave_sync();
 
-    uint32_t k[RS_ITEMS], v[RS_ITEMS];
+    // Each wave ranks its own contiguous 1024 keys (item j of lane l is key
+    // base + wid*1024 + 64 j + l), so the order (wave, j, lane) is the input
+    // order and the ranks are stable without block barriers: peers with the
+    // same digit are found with 8 ballots, the wave's running count per digit
+    // lives in s_wrun[wid] (all lanes read it before the leader updates it).
+    uint32_t k[RS_ITEMS], v[RS_ITEMS], dr[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        const int i = base + j * RS_NT + tid;
+        const int i = base + wid * WITEMS + j * 64 + lane;
         const bool ok = i < n;
         k[j] = ok ? kin[i] : 0u;
         v[j] = ok ? (vin ? vin[i] : (uint32_t)i) : 0u;
-        if (ok)
-            atomicAdd(&s_run[(k[j] >> shift) & (RS_BINS - 1)], 1u);
     }
-    __syncthreads();
-    {
-        const uint32_t c = s_run[tid];
-        s_tileoff[tid] = block_scan_excl<RS_NT>(c, s_w, nullptr);
-        s_run[tid] = 0;
-    }
-    __syncthreads();
-
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        const bool ok = base + j * RS_NT + tid < n;
+        const bool ok = base + wid * WITEMS + j * 64 + lane < n;
         const uint32_t d = (k[j] >> shift) & (RS_BINS - 1);
         unsigned long long peers = tc_ballot(ok);
 #pragma unroll
@@ -342,29 +335,41 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
         const uint32_t lower = __popcll(peers & tc_lanemask_lt());
-        const bool leader = ok && lower == 0;
-        if (leader)
-            s_wcnt[wid][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (ok) {
-            uint32_t pre = s_run[d];
-            for (int w = 0; w < wid; ++w)
-                pre += s_wcnt[w][d];
-            const uint32_t pos = s_tileoff[d] + pre + lower;
+        const uint32_t run = ok ? s_wrun[wid][d] : 0u;
+        tc_wave_sync();
+        if (ok && lower == 0)
+            s_wrun[wid][d] = run + (uint32_t)__popcll(peers);
+        tc_wave_sync();
+        dr[j] = ((run + lower) << 8) | d;
+    }
+    __syncthreads();
+    {
+        // digit tid: tile offset = exclusive scan over digits of the tile
+        // counts; wave w's keys of that digit start after waves 0..w-1
+        uint32_t c[NW], tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            c[w] = s_wrun[w][tid];
+            tot += c[w];
+        }
+        uint32_t off = block_scan_excl<RS_NT>(tot, s_w, nullptr);
+        s_tileoff[tid] = off;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            s_wrun[w][tid] = off;
+            off += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        if (base + wid * WITEMS + j * 64 + lane < n) {
+            const uint32_t pos = s_wrun[wid][dr[j] & 255u] + (dr[j] >> 8);
             s_k[pos] = k[j];
             s_v[pos] = v[j];
         }
-        __syncthreads();
-        {
-            uint32_t add = 0;
-            for (int w = 0; w < RS_NT / 64; ++w) {
-                add += s_wcnt[w][tid];
-                s_wcnt[w][tid] = 0;
-            }
-            s_run[tid] += add;
-        }
-        __syncthreads();
     }
+    __syncthreads();
 
     for (int p = tid; p < cnt; p += RS_NT) {
         const uint32_t kk = s_k[p];
@@ -817,11 +822,13 @@ __global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int
 }
 
 // FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, 1e-30), so that
-// the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta.  Since
+// the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta (force trees: j b_j,
+// the derivative series' coefficients).  Since
 // |a_j| <= sum|v| r^j / j, dividing the coefficient itself j times keeps every
 // intermediate bounded (no overflow of s^-j for tiny radii).
 template <int P>
-__global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const double *exps, int nnodes, float2 *ex32)
+__global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const double *exps, int nnodes, int deriv,
+                                                float2 *ex32)
 {
     const int id = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = tc_lane();
@@ -830,7 +837,9 @@ __global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const doubl
     const double r = nodes[id].r;
     const double sc = r > 1e-30 ? r : 1e-30;
     const double *ex = exps + (size_t)2 * P * id;
-    double a = ex[lane], b = ex[P + lane];
+    // force trees store k b_k, the coefficients of the derivative series
+    const double k = deriv ? (double)(lane + 1) : 1.0;
+    double a = k * ex[lane], b = k * ex[P + lane];
     for (int j = 0; j <= lane; ++j) {
         a /= sc;
         b /= sc;
@@ -938,7 +947,7 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
 
 template <int P>
 static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, const int *lev, const int2 *cell,
-                          const int *selfid, bool fp32, hipStream_t st)
+                          const int *selfid, bool fp32, bool deriv, hipStream_t st)
 {
     const double *box = t.box_d.as<double>();
     k_leaf_upward<P><<<tiles_of(t.nnodes, 4), 256, 0, st>>>(child, se, lev, cell, t.src.as<double4>(), box,
@@ -956,13 +965,13 @@ static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, cons
     }
     if (fp32) {
         k_exps32<P><<<tiles_of(t.nnodes, 4), 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), t.nnodes,
-                                                            t.exps32.as<float2>());
+                                                            deriv ? 1 : 0, t.exps32.as<float2>());
         TC_HIP(hipGetLastError());
     }
 }
 
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
-                   bool fp32, TcScratch &ws, hipStream_t st)
+                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st)
 {
     if (n <= 0)
         throw TcError(-3, "tc_build_tree: no sources");
@@ -976,6 +985,7 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     t.order = order;
     t.leafcap = leafcap;
     t.has_fp32 = fp32;
+    t.deriv32 = fp32 && deriv32;
     t.box_d.reserve(sizeof(double) * 4);
     t.keys.reserve(sizeof(uint32_t) * (size_t)n);
     t.perm.reserve(sizeof(uint32_t) * (size_t)n);
@@ -1044,12 +1054,12 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
 
     const int *sid = t.selfid.as<int>();
     switch (order) {
-    case 8: upward_launch<8>(t, child, se, lev, cell, sid, fp32, st); break;
-    case 12: upward_launch<12>(t, child, se, lev, cell, sid, fp32, st); break;
-    case 16: upward_launch<16>(t, child, se, lev, cell, sid, fp32, st); break;
-    case 20: upward_launch<20>(t, child, se, lev, cell, sid, fp32, st); break;
-    case 24: upward_launch<24>(t, child, se, lev, cell, sid, fp32, st); break;
-    case 32: upward_launch<32>(t, child, se, lev, cell, sid, fp32, st); break;
+    case 8: upward_launch<8>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+    case 12: upward_launch<12>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+    case 16: upward_launch<16>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+    case 20: upward_launch<20>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+    case 24: upward_launch<24>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+    case 32: upward_launch<32>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
     }
 }
 

@@ -9,6 +9,7 @@
 //   expansions       double[2p] per node: [Re a_1..a_p][Im a_1..a_p]
 //   FP32 mirrors     float4 {x-xc, y-yc, v, 0} per source (leaf-relative) and
 //                    float2[p] per node of scaled coefficients a_j / r^j
+//                    (j a_j / r^j for force trees)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -173,6 +174,7 @@ struct TcTree {
     int level_start[TC_LMAX + 2] = {0};  // internal ranks of level L: [level_start[L], level_start[L+1])
     double box[4] = {0, 0, 0, 0};        // xmin, ymin, ext, scale (host copy)
     bool has_fp32 = false;
+    bool deriv32 = false;  // FP32 coefficient mirror holds k b_k
     DevBuf keys, perm, src, src32, nodes, exps, exps32, nodew, selfid;
     DevBuf box_d;
     void release()
@@ -184,11 +186,11 @@ struct TcTree {
 
 // Scratch reused across calls of one context.
 struct TcScratch {
-    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints, packed;
+    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints, packed, work;
     void release()
     {
         for (DevBuf *b : {&k2, &v2, &hist, &part, &scan_tmp, &istart, &ilevel, &info, &tkeys, &tperm, &tsorted,
-                          &bbox_part, &ints, &packed})
+                          &bbox_part, &ints, &packed, &work})
             b->release();
     }
 };
@@ -198,15 +200,17 @@ void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
                     bool hilbert = false);
+// fp32: also build the FP32 mirrors; deriv32: the coefficient mirror holds
+// k b_k (force trees) instead of b_k (potential trees).
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
-                   bool fp32, TcScratch &ws, hipStream_t st);
+                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st);
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
                        hipEvent_t ev0, hipEvent_t ev1);
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
-                   hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+                   uint32_t *work, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 void tc_eval_force_targets(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *xr,
                            double *yr, int precision, unsigned long long *counters, TcScratch &ws,
                            hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);

@@ -17,6 +17,8 @@
 // decisions, E2L and L2P stay FP64 (same interaction lists as FP64).
 #include "tc_common.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <vector>
 
@@ -83,41 +85,6 @@ TC_DEV double log64(double x)
     return fma(kd, 0.6931471803691238, fma(kd, 1.9082149292705877e-10, logm));
 }
 
-// The force traversal's stack holds node records in LDS: opening a node loads
-// the first 48 bytes (centre, radius, mass, source range, first child, level)
-// of its four children's records with one wave-wide vector load (lanes 0-11,
-// 16 bytes each), so a pop reads LDS instead of waiting on a dependent load.
-// That kernel runs 3-4 waves per SIMD and was latency-bound on its scalar node
-// loads; the potential traversals run 8 and hide those loads, and there the
-// extra LDS reads only cost issue slots (measured 7% slower), so they keep
-// scalar loads.
-struct alignas(16) TcStackRec {
-    double xcom, ycom, r, mass;
-    int s, e, child, level;
-};
-static_assert(sizeof(TcStackRec) == 48 && offsetof(TcNode, pad1) == 60, "stack record is TcNode's first 48 B");
-
-TC_DEV void stack_load(const TcNode *__restrict__ nodes, int first, int count, TcStackRec *stk)
-{
-    const int lane = tc_lane();
-    if (lane < 3 * count) {
-        const int c = lane / 3, part = lane - 3 * c;
-        const double2 v = reinterpret_cast<const double2 *>(nodes + first + c)[part];
-        reinterpret_cast<double2 *>(stk + c)[part] = v;
-    }
-}
-
-// A stack record read back; the fields that steer control flow are made
-// wave-uniform (scalar registers).
-TC_DEV TcStackRec stack_pop(const TcStackRec *stk)
-{
-    TcStackRec nd = *stk;
-    nd.s = tc_uniform(nd.s);
-    nd.e = tc_uniform(nd.e);
-    nd.child = tc_uniform(nd.child);
-    return nd;
-}
-
 // ------------------------------------------------------------------ potential kernels
 // potential_e2p (potential-kernels.m4:59-89) for one expansion:
 // Q log|z| + Re sum_{j=1..p} a_j u^j with u = 1/z, evaluated by Horner's rule
@@ -180,20 +147,20 @@ TC_DEV void e2p_force64(double rz, double iz, double mass, const double *__restr
     fy -= fma(ur, vi, ui * vr);
 }
 
-// FP32 force E2P on scaled coefficients: dphi/dz = u (Q - w T'), w = s u,
-// T' = sum_k k b_k w^(k-1) by Horner.
+// FP32 force E2P on scaled derivative coefficients d_k = k b_k (the force
+// tree's mirror): dphi/dz = u (Q - w T'), w = s u, T' = sum_k d_k w^(k-1) by
+// Horner.
 template <int P>
 TC_DEV void e2p_force32(float zr, float zi, float mass, float sc, const float2 *__restrict__ ex, float &gx, float &gy)
 {
     const float inv = fast_rcp(fmaf(zr, zr, zi * zi));
     const float ur = zr * inv, ui = -zi * inv;
     const float wr = sc * ur, wi = sc * ui;
-    float ar = (float)P * ex[P - 1].x, ai = (float)P * ex[P - 1].y;
+    float ar = ex[P - 1].x, ai = ex[P - 1].y;
 #pragma unroll
     for (int j = P - 2; j >= 0; --j) {
-        const float k = (float)(j + 1);
-        const float tr = fmaf(wr, ar, fmaf(-wi, ai, k * ex[j].x));
-        const float ti = fmaf(wr, ai, fmaf(wi, ar, k * ex[j].y));
+        const float tr = fmaf(wr, ar, fmaf(-wi, ai, ex[j].x));
+        const float ti = fmaf(wr, ai, fmaf(wi, ar, ex[j].y));
         ar = tr;
         ai = ti;
     }
@@ -349,8 +316,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 // lane forms u^(l+1) by binary powering, publishes (-1)^k a_k u^k, then sums
 // C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
 template <int P>
-TC_DEV void e2l_pair(const TcNode *__restrict__ nodes, const double *__restrict__ exps, const int *q, int nq,
-                     double x0b, double y0b, double (*s_binom)[64], double2 *s_c, double &lre, double &lim)
+TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double4 *qrec, int nq, double x0b,
+                     double y0b, double (*s_binom)[64], double2 *s_c, double &lre, double &lim)
 {
 #pragma clang fp contract(fast)
     const int lane = tc_lane();
@@ -359,9 +326,9 @@ TC_DEV void e2l_pair(const TcNode *__restrict__ nodes, const double *__restrict_
     double ur = 0, ui = 0, pr = 1, pi = 0, mass = 0;
     if (h < nq) {
         const int id = q[h];
-        const TcNode nd = nodes[id];
-        mass = nd.mass;
-        const double zx = nd.xcom - x0b, zy = nd.ycom - y0b;
+        const double4 nd = qrec[h];  // {xcom, ycom, mass, -}
+        mass = nd.z;
+        const double zx = nd.x - x0b, zy = nd.y - y0b;
         const double inv = 1.0 / (zx * zx + zy * zy);
         ur = zx * inv;
         ui = -zy * inv;
@@ -404,19 +371,81 @@ TC_DEV void e2l_pair(const TcNode *__restrict__ nodes, const double *__restrict_
     tc_wave_sync();
 }
 
+// Force stack records: the first 48 bytes of a node (centre, radius, mass,
+// source range, first child) plus the brick's decision for it, which lanes
+// 0-3 take for the four children at once when a node is opened (one node per
+// lane, brick-uniform inputs), so a pop only reads LDS and branches.
+enum TcAction { ACT_SKIP = 0, ACT_E2L = 1, ACT_E2P = 2, ACT_P2P = 3, ACT_OPEN = 4 };
+
+struct alignas(16) TcStackRec {
+    double xcom, ycom, r, mass;
+    int s, e, child, action;
+};
+static_assert(sizeof(TcStackRec) == 48 && offsetof(TcNode, level) == 44, "stack record is TcNode's first 48 B");
+
+struct TcBrick {
+    double x0b, y0b, rbrick, xlo, xhi, ylo, yhi, invtheta, th2;
+};
+
+// The per-brick decisions of treecode-force.cpp:101-199 for one node: empty
+// nodes are skipped; E2L when (|c - brick centre| / r - 1) > 1/theta and the
+// brick radius <= r; E2P when r^2 < theta^2 d^2, d the distance from the node
+// centre to the brick's closest point; else P2P at a leaf, else open.
+TC_DEV int force_action(const TcStackRec &nd, const TcBrick &bk)
+{
+    if (nd.s == nd.e)
+        return ACT_SKIP;
+    const double ddx = bk.x0b - nd.xcom, ddy = bk.y0b - nd.ycom;
+    const double distance = sqrt(ddx * ddx + ddy * ddy);
+    if ((distance / nd.r - 1) > bk.invtheta && bk.rbrick <= nd.r)
+        return ACT_E2L;
+    const double xm = nd.xcom < bk.xhi ? nd.xcom : bk.xhi, ym = nd.ycom < bk.yhi ? nd.ycom : bk.yhi;
+    const double cxt = bk.xlo < xm ? xm : bk.xlo;
+    const double cyt = bk.ylo < ym ? ym : bk.ylo;
+    const double r2 = (cxt - nd.xcom) * (cxt - nd.xcom) + (cyt - nd.ycom) * (cyt - nd.ycom);
+    if (nd.r * nd.r < bk.th2 * r2)
+        return ACT_E2P;
+    return nd.child == 0 ? ACT_P2P : ACT_OPEN;
+}
+
+// Lanes 0..count-1 load node first+lane, classify it and write its stack
+// record and id at stk[lane], ids[lane].
+TC_DEV void force_push(const TcNode *__restrict__ nodes, int first, int count, const TcBrick &bk, TcStackRec *stk,
+                       int *ids)
+{
+    const int lane = tc_lane();
+    if (lane < count) {
+        const double2 *g = reinterpret_cast<const double2 *>(nodes + first + lane);
+        const double2 a = g[0], b = g[1], c = g[2];
+        TcStackRec nd;
+        nd.xcom = a.x;
+        nd.ycom = a.y;
+        nd.r = b.x;
+        nd.mass = b.y;
+        nd.s = __double2loint(c.x);
+        nd.e = __double2hiint(c.x);
+        nd.child = __double2loint(c.y);
+        nd.action = force_action(nd, bk);
+        stk[lane] = nd;
+        ids[lane] = first + lane;
+    }
+}
+
 template <int P, bool F32>
 __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
                                                const double *__restrict__ y0s, const double *__restrict__ hs,
                                                int nbricks, double theta, double *__restrict__ xres,
-                                               double *__restrict__ yres, unsigned long long *counters)
+                                               double *__restrict__ yres, unsigned long long *counters,
+                                               uint32_t *__restrict__ work)
 {
     __shared__ int s_node[4][TC_STACK];
     __shared__ TcStackRec s_stk[4][TC_STACK];
     __shared__ double4 s_src[4][64];
     __shared__ double2 s_coef[4][2 * 33];
     __shared__ int s_e2lq[4][2];
+    __shared__ double4 s_e2lrec[4][2];
     __shared__ double s_binom[TC_MAXORDER][64];
 
     for (int i = threadIdx.x; i < P * 64; i += 256)
@@ -424,160 +453,212 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
     __syncthreads();
 
     const int wid = threadIdx.x >> 6, lane = tc_lane();
-    const int wave = tc_uniform((int)(blockIdx.x * 4 + wid));
-    if (wave >= nbricks)
-        return;
-    const int b = wave >> 4, brick = wave & 15;
-    const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
-    const double x0 = x0s[b], y0 = y0s[b], h = hs[b];
-    const int ix = lane & 7, iy = lane >> 3;
-    // targets as in force_p2p_8x8 (force-kernels.m4:28-32)
-    const double xb = x0 + (bx + 0) * h, yb = y0 + (by + 0) * h;
-    const double xt = xb + ix * h, yt = yb + iy * h;
-    // brick geometry (treecode-force.cpp:108,113-114,146-147)
-    const double x0b = x0 + h * (bx + 0.5 * (TILE - 1));
-    const double y0b = y0 + h * (by + 0.5 * (TILE - 1));
-    const double rbrick = 1.4142135623730951 * h * (TILE - 1) * 0.5;
-    const double xlo = x0 + bx * h, xhi = x0 + (bx + TILE - 1) * h;
-    const double ylo = y0 + by * h, yhi = y0 + (by + TILE - 1) * h;
-    const double invtheta = 1 / theta;
-    const double th2 = theta * theta;
+    unsigned long long np2p = 0, ne2p = 0, ne2l = 0, nl2p = 0;
+    // Bricks are dealt out one at a time from a device counter and each wave
+    // runs until the queue is empty: brick costs differ by orders of magnitude
+    // across block sizes, and a workgroup's LDS is only released when all four
+    // of its waves are done, so fixed brick-per-wave launches left SIMDs idle.
+    for (;;) {
+        uint32_t grab = 0;
+        if (lane == 0)
+            grab = atomicAdd(work, 1u);
+        const uint32_t wave = (uint32_t)__builtin_amdgcn_readlane((int)grab, 0);
+        if (wave >= (uint32_t)nbricks)
+            break;
+        const int b = wave >> 4, brick = wave & 15;
+        const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
+        const double x0 = x0s[b], y0 = y0s[b], h = hs[b];
+        const int ix = lane & 7, iy = lane >> 3;
+        // targets as in force_p2p_8x8 (force-kernels.m4:28-32)
+        const double xb = x0 + (bx + 0) * h, yb = y0 + (by + 0) * h;
+        const double xt = xb + ix * h, yt = yb + iy * h;
+        // brick geometry (treecode-force.cpp:108,113-114,146-147)
+        TcBrick bk;
+        bk.x0b = x0 + h * (bx + 0.5 * (TILE - 1));
+        bk.y0b = y0 + h * (by + 0.5 * (TILE - 1));
+        bk.rbrick = 1.4142135623730951 * h * (TILE - 1) * 0.5;
+        bk.xlo = x0 + bx * h;
+        bk.xhi = x0 + (bx + TILE - 1) * h;
+        bk.ylo = y0 + by * h;
+        bk.yhi = y0 + (by + TILE - 1) * h;
+        bk.invtheta = 1 / theta;
+        bk.th2 = theta * theta;
 
-    double fx = 0, fy = 0;
-    double lre = 0, lim = 0;
-    unsigned long long np2p = 0, ne2p = 0, ne2l = 0;
-    int nq = 0;
+        double fx = 0, fy = 0;
+        double lre = 0, lim = 0;
+        int nq = 0;
 
-    int sp = 1;
-    s_node[wid][0] = 0;
-    stack_load(nodes, 0, 1, &s_stk[wid][0]);
-    while (sp > 0) {
-        --sp;
-        tc_wave_sync();
-        const int id = tc_uniform(s_node[wid][sp]);
-        const TcStackRec nd = stack_pop(&s_stk[wid][sp]);
-        if (nd.s == nd.e)
-            continue;
-        const double xcom = nd.xcom, ycom = nd.ycom, r = nd.r;
-        const double ddx = x0b - xcom, ddy = y0b - ycom;
-        const double distance = sqrt(ddx * ddx + ddy * ddy);
-        if ((distance / r - 1) > invtheta && rbrick <= r) {
-            // queued; downward_e2l runs on pairs of nodes (e2l_pair)
-            ++ne2l;
-            s_e2lq[wid][nq] = id;
-            ++nq;
-            if (nq == 2) {
-                tc_wave_sync();
-                e2l_pair<P>(nodes, exps, s_e2lq[wid], 2, x0b, y0b, s_binom, s_coef[wid], lre, lim);
-                nq = 0;
-            }
-            continue;
-        }
-        const double cxt = xlo < (xcom < xhi ? xcom : xhi) ? (xcom < xhi ? xcom : xhi) : xlo;
-        const double cyt = ylo < (ycom < yhi ? ycom : yhi) ? (ycom < yhi ? ycom : yhi) : ylo;
-        const double r2 = (cxt - xcom) * (cxt - xcom) + (cyt - ycom) * (cyt - ycom);
-        if (r * r < th2 * r2) {
-            // force_e2p_8x8 (force-kernels.m4:57-98)
-            ne2p += 64;
-            const double rz = (xb - xcom) + ix * h, iz = (yb - ycom) + iy * h;
-            if (F32) {
-                float gx, gy;
-                e2p_force32<P>((float)rz, (float)iz, (float)nd.mass, (float)(r > 1e-30 ? r : 1e-30),
-                               exps32 + (size_t)P * id, gx, gy);
-                fx += (double)gx;
-                fy += (double)gy;
-            } else {
-                e2p_force64<P>(rz, iz, nd.mass, exps + (size_t)2 * P * id, fx, fy);
-            }
-            continue;
-        }
-        if (nd.child == 0) {
-            // force_p2p_8x8 (force-kernels.m4:15-55)
-            np2p += 64ull * (unsigned long long)(nd.e - nd.s);
-            if (F32) {
-                const float tx = (float)(xt - xcom), ty = (float)(yt - ycom);
-                float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
-                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
-                    const int cnt = min(64, nd.e - b0);
-                    if (lane < cnt)
-                        tile[lane] = src32[b0 + lane];
-                    tc_wave_sync();
-                    float ax32 = 0.f, ay32 = 0.f;
-                    for (int j = 0; j < cnt; ++j) {
-#pragma clang fp contract(fast)
-                        const float4 q = tile[j];
-                        const float dx = tx - q.x, dy = ty - q.y;
-                        const float f = q.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
-                        ax32 += dx * f;
-                        ay32 += dy * f;
-                    }
-                    fx += (double)ax32;
-                    fy += (double)ay32;
-                    tc_wave_sync();
+        int sp = 1;
+        force_push(nodes, 0, 1, bk, &s_stk[wid][0], &s_node[wid][0]);
+        while (sp > 0) {
+            --sp;
+            tc_wave_sync();
+            const int action = tc_uniform(s_stk[wid][sp].action);
+            if (action == ACT_SKIP)
+                continue;
+            const int id = tc_uniform(s_node[wid][sp]);
+            if (action == ACT_E2L) {
+                // queued; downward_e2l runs on pairs of nodes (e2l_pair)
+                ++ne2l;
+                if (lane == 0) {
+                    const TcStackRec &q = s_stk[wid][sp];
+                    s_e2lq[wid][nq] = id;
+                    s_e2lrec[wid][nq] = make_double4(q.xcom, q.ycom, q.mass, 0.0);
                 }
-            } else {
-                for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
-                    const int cnt = min(64, nd.e - b0);
-                    if (lane < cnt)
-                        s_src[wid][lane] = src[b0 + lane];
+                ++nq;
+                if (nq == 2) {
                     tc_wave_sync();
-                    for (int j = 0; j < cnt; ++j) {
-#pragma clang fp contract(fast)
-                        const double4 q = s_src[wid][j];
-                        const double dx = xt - q.x, dy = yt - q.y;
-                        const double f = q.z * rcp64(dx * dx + (dy * dy + EPS_REG));
-                        fx += dx * f;
-                        fy += dy * f;
-                    }
-                    tc_wave_sync();
+                    e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], 2, bk.x0b, bk.y0b, s_binom, s_coef[wid], lre,
+                                lim);
+                    nq = 0;
                 }
+                continue;
             }
-            continue;
-        }
-        if (lane < 4)
-            s_node[wid][sp + lane] = nd.child + lane;
-        stack_load(nodes, nd.child, 4, &s_stk[wid][sp]);
-        sp += 4;
-    }
-
-    if (nq) {
-        tc_wave_sync();
-        e2l_pair<P>(nodes, exps, s_e2lq[wid], nq, x0b, y0b, s_binom, s_coef[wid], lre, lim);
-    }
-    // fold the two half-wave partial local expansions
-    lre += __shfl_xor(lre, 32, 64);
-    lim += __shfl_xor(lim, 32, 64);
-    // downward_l2p_8x8 (force-kernels.m4:157-191) about the brick centre
-    if (lane < P)
-        s_coef[wid][lane + 1] = make_double2(lre, lim);
-    tc_wave_sync();
-    {
+            const TcStackRec &rec = s_stk[wid][sp];
+            const double xcom = rec.xcom, ycom = rec.ycom;
+            if (action == ACT_E2P) {
+                // force_e2p_8x8 (force-kernels.m4:57-98)
+                ne2p += 64;
+                const double r = rec.r, mass = rec.mass;
+                const double rz = (xb - xcom) + ix * h, iz = (yb - ycom) + iy * h;
+                if (F32) {
+                    float gx, gy;
+                    e2p_force32<P>((float)rz, (float)iz, (float)mass, (float)(r > 1e-30 ? r : 1e-30),
+                                   exps32 + (size_t)P * id, gx, gy);
+                    fx += (double)gx;
+                    fy += (double)gy;
+                } else {
+                    e2p_force64<P>(rz, iz, mass, exps + (size_t)2 * P * id, fx, fy);
+                }
+                continue;
+            }
+            if (action == ACT_P2P) {
+                // force_p2p_8x8 (force-kernels.m4:15-55)
+                const int ns = tc_uniform(rec.s), ne = tc_uniform(rec.e);
+                np2p += 64ull * (unsigned long long)(ne - ns);
+                if (F32) {
+                    const float tx = (float)(xt - xcom), ty = (float)(yt - ycom);
+                    float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
+                    for (int b0 = ns; b0 < ne; b0 += 64) {
+                        const int cnt = min(64, ne - b0);
+                        if (lane < cnt)
+                            tile[lane] = src32[b0 + lane];
+                        tc_wave_sync();
+                        float ax0 = 0.f, ay0 = 0.f, ax1 = 0.f, ay1 = 0.f;
+                        int j = 0;
+                        for (; j + 4 <= cnt; j += 4) {
 #pragma clang fp contract(fast)
-        const double zl = h * (0 - 0.5 * (TILE - 1));
-        const double zr1 = zl + ix * h, zi1 = zl + iy * h;
-        const double2 b1 = s_coef[wid][1];
-        double rres = b1.x, ires = b1.y;
-        double zr = zr1, zi = zi1;
-        for (int l = 2; l <= P; ++l) {
-            const double2 bl = s_coef[wid][l];
-            rres += l * (bl.x * zr - bl.y * zi);
-            ires += l * (bl.x * zi + bl.y * zr);
-            const double rn = zr * zr1 - zi * zi1;
-            const double in = zr * zi1 + zi * zr1;
-            zr = rn;
-            zi = in;
+                            const float4 q0 = tile[j], q1 = tile[j + 1], q2 = tile[j + 2], q3 = tile[j + 3];
+                            const float dx0 = tx - q0.x, dy0 = ty - q0.y;
+                            const float dx1 = tx - q1.x, dy1 = ty - q1.y;
+                            const float dx2 = tx - q2.x, dy2 = ty - q2.y;
+                            const float dx3 = tx - q3.x, dy3 = ty - q3.y;
+                            const float f0 = q0.z * fast_rcp(dx0 * dx0 + (dy0 * dy0 + EPS_REG32));
+                            const float f1 = q1.z * fast_rcp(dx1 * dx1 + (dy1 * dy1 + EPS_REG32));
+                            const float f2 = q2.z * fast_rcp(dx2 * dx2 + (dy2 * dy2 + EPS_REG32));
+                            const float f3 = q3.z * fast_rcp(dx3 * dx3 + (dy3 * dy3 + EPS_REG32));
+                            ax0 += dx0 * f0;
+                            ay0 += dy0 * f0;
+                            ax1 += dx1 * f1;
+                            ay1 += dy1 * f1;
+                            ax0 += dx2 * f2;
+                            ay0 += dy2 * f2;
+                            ax1 += dx3 * f3;
+                            ay1 += dy3 * f3;
+                        }
+                        for (; j < cnt; ++j) {
+#pragma clang fp contract(fast)
+                            const float4 q = tile[j];
+                            const float dx = tx - q.x, dy = ty - q.y;
+                            const float f = q.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+                            ax0 += dx * f;
+                            ay0 += dy * f;
+                        }
+                        fx += (double)(ax0 + ax1);
+                        fy += (double)(ay0 + ay1);
+                        tc_wave_sync();
+                    }
+                } else {
+                    for (int b0 = ns; b0 < ne; b0 += 64) {
+                        const int cnt = min(64, ne - b0);
+                        if (lane < cnt)
+                            s_src[wid][lane] = src[b0 + lane];
+                        tc_wave_sync();
+                        double gx = 0, gy = 0;
+                        int j = 0;
+                        for (; j + 2 <= cnt; j += 2) {
+#pragma clang fp contract(fast)
+                            const double4 q0 = s_src[wid][j], q1 = s_src[wid][j + 1];
+                            const double dx0 = xt - q0.x, dy0 = yt - q0.y;
+                            const double dx1 = xt - q1.x, dy1 = yt - q1.y;
+                            const double f0 = q0.z * rcp64(dx0 * dx0 + (dy0 * dy0 + EPS_REG));
+                            const double f1 = q1.z * rcp64(dx1 * dx1 + (dy1 * dy1 + EPS_REG));
+                            fx += dx0 * f0;
+                            fy += dy0 * f0;
+                            gx += dx1 * f1;
+                            gy += dy1 * f1;
+                        }
+                        if (j < cnt) {
+#pragma clang fp contract(fast)
+                            const double4 q = s_src[wid][j];
+                            const double dx = xt - q.x, dy = yt - q.y;
+                            const double f = q.z * rcp64(dx * dx + (dy * dy + EPS_REG));
+                            fx += dx * f;
+                            fy += dy * f;
+                        }
+                        fx += gx;
+                        fy += gy;
+                        tc_wave_sync();
+                    }
+                }
+                continue;
+            }
+            // open: children pushed in order 0..3, popped 3..0
+            force_push(nodes, tc_uniform(rec.child), 4, bk, &s_stk[wid][sp], &s_node[wid][sp]);
+            sp += 4;
         }
-        fx += rres;
-        fy -= ires;
+
+        if (nq) {
+            tc_wave_sync();
+            e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_coef[wid], lre, lim);
+        }
+        // fold the two half-wave partial local expansions
+        lre += __shfl_xor(lre, 32, 64);
+        lim += __shfl_xor(lim, 32, 64);
+        // downward_l2p_8x8 (force-kernels.m4:157-191) about the brick centre
+        if (lane < P)
+            s_coef[wid][lane + 1] = make_double2(lre, lim);
+        tc_wave_sync();
+        {
+#pragma clang fp contract(fast)
+            const double zl = h * (0 - 0.5 * (TILE - 1));
+            const double zr1 = zl + ix * h, zi1 = zl + iy * h;
+            const double2 b1 = s_coef[wid][1];
+            double rres = b1.x, ires = b1.y;
+            double zr = zr1, zi = zi1;
+            for (int l = 2; l <= P; ++l) {
+                const double2 bl = s_coef[wid][l];
+                rres += l * (bl.x * zr - bl.y * zi);
+                ires += l * (bl.x * zi + bl.y * zr);
+                const double rn = zr * zr1 - zi * zi1;
+                const double in = zr * zi1 + zi * zr1;
+                zr = rn;
+                zi = in;
+            }
+            fx += rres;
+            fy -= ires;
+        }
+        const size_t o = (size_t)b * 1024 + (size_t)(bx + ix) + 32 * (size_t)(by + iy);
+        xres[o] = fx;
+        yres[o] = fy;
+        nl2p += 64;
+        tc_wave_sync();
     }
-    const size_t o = (size_t)b * 1024 + (size_t)(bx + ix) + 32 * (size_t)(by + iy);
-    xres[o] = fx;
-    yres[o] = fy;
     if (counters && lane == 0) {
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
         atomicAdd(&counters[TC_CNT_E2L], ne2l);
-        atomicAdd(&counters[TC_CNT_L2P], 64ull);
+        atomicAdd(&counters[TC_CNT_L2P], nl2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
         atomicAdd(&counters[TC_CNT_E2P_SLOTS], ne2p);
     }
@@ -795,20 +876,48 @@ void launch_force_targets(const TcTree &t, double th2, const double *xt, const d
     TC_HIP(hipGetLastError());
 }
 
+// Workgroups of a persistent kernel that fit the device at once (occupancy x
+// CUs), cached per kernel and device.
+template <auto Kernel> int resident_blocks(int device)
+{
+    static std::atomic<int> cache[64];
+    if (device < 0 || device >= 64)
+        throw TcError(-3, "device id out of range");
+    int v = cache[device].load(std::memory_order_relaxed);
+    if (!v) {
+        int per_cu = 0, cus = 0;
+        TC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, 256, 0));
+        TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        if (per_cu < 1 || cus < 1)
+            throw TcError(-3, "kernel does not fit on the device");
+        v = per_cu * cus;
+        cache[device].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
 template <int P>
 void launch_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
-                  int nblocks, double *xr, double *yr, int precision, unsigned long long *counters, hipStream_t st)
+                  int nblocks, double *xr, double *yr, int precision, unsigned long long *counters, uint32_t *work,
+                  hipStream_t st)
 {
+    if (nblocks > (1 << 26))
+        throw TcError(-2, "tc_eval_force: too many blocks");
     const int nbricks = nblocks * 16;
-    const int grid = tiles_of(nbricks, 4);
-    if (precision == 1)
+    int device = 0;
+    TC_HIP(hipGetDevice(&device));
+    tc_fill_u32(work, 0u, 1, st);
+    if (precision == 1) {
+        const int grid = std::min(tiles_of(nbricks, 4), resident_blocks<k_force<P, true>>(device));
         k_force<P, true><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), t.exps32.as<float2>(),
                                                t.src.as<double4>(), t.src32.as<float4>(), x0s, y0s, hs, nbricks,
-                                               theta, xr, yr, counters);
-    else
+                                               theta, xr, yr, counters, work);
+    } else {
+        const int grid = std::min(tiles_of(nbricks, 4), resident_blocks<k_force<P, false>>(device));
         k_force<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
                                                 t.src.as<double4>(), nullptr, x0s, y0s, hs, nbricks, theta, xr, yr,
-                                                counters);
+                                                counters, work);
+    }
     TC_HIP(hipGetLastError());
 }
 
@@ -877,7 +986,7 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
 
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
-                   hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+                   uint32_t *work, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (nblocks <= 0)
         return;
@@ -886,12 +995,12 @@ void tc_eval_force(const TcTree &t, double theta, const double *x0s, const doubl
     if (ev0)
         TC_HIP(hipEventRecord(ev0, st));
     switch (t.order) {
-    case 8: launch_force<8>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
-    case 12: launch_force<12>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
-    case 16: launch_force<16>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
-    case 20: launch_force<20>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
-    case 24: launch_force<24>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
-    case 32: launch_force<32>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, st); break;
+    case 8: launch_force<8>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+    case 12: launch_force<12>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+    case 16: launch_force<16>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+    case 20: launch_force<20>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+    case 24: launch_force<24>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+    case 32: launch_force<32>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
     default: throw TcError(-3, "tc_eval_force: unsupported order");
     }
     if (ev1)

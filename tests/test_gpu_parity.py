@@ -53,6 +53,21 @@ def test_morton_sort_duplicates_stable(pkg, oracle, wl, ctx64):
     assert np.array_equal(keys, oks)
 
 
+def test_morton_sort_two_keys_stable(pkg, oracle, wl, ctx64):
+    # two distinct keys over many tiles: each 1024-key run of the sort holds
+    # hundreds of equal digits, so ranks within a run reach their maximum
+    st = wl.Stream(6)
+    n = 3 * 4096 * 16 + 77
+    pick = st.uniform(n) < 0.5
+    x = np.where(pick, 0.1, 0.9)
+    y = np.where(pick, 0.2, 0.8)
+    box, keys, perm = _device_sort(pkg, ctx64, x, y)
+    okeys = oracle.keys(x, y, oracle.bbox(x, y))
+    operm, oks = oracle.sort(okeys)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(keys, oks)
+
+
 def test_morton_sort_cell_edges(pkg, oracle, ctx64):
     from golden_util import load_fixture
     fx = load_fixture("morton_edges")
