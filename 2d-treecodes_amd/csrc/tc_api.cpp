@@ -391,7 +391,7 @@ TC_API int tc_set_option(tc_context *c, int key, int value)
         set_error("treecode: null context");
         return -1;
     }
-    auto valid_order = [](int p) { return p == 8 || p == 12 || p == 16 || p == 20 || p == 24 || p == 32; };
+    auto valid_order = [](int p) { return tc_order_supported(p); };
     switch (key) {
     case TC_OPT_PRECISION:
         if (value != 0 && value != 1) break;

@@ -979,7 +979,7 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
         throw TcError(-3, "tc_build_tree: leaf capacity must be >= 1");
     if (n >= (1 << 30))
         throw TcError(-3, "tc_build_tree: at most 2^30 - 1 sources per tree");
-    if (order != 8 && order != 12 && order != 16 && order != 20 && order != 24 && order != 32)
+    if (!tc_order_supported(order))
         throw TcError(-3, "tc_build_tree: unsupported expansion order " + std::to_string(order));
     t.n = n;
     t.order = order;
@@ -1054,12 +1054,10 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
 
     const int *sid = t.selfid.as<int>();
     switch (order) {
-    case 8: upward_launch<8>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
-    case 12: upward_launch<12>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
-    case 16: upward_launch<16>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
-    case 20: upward_launch<20>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
-    case 24: upward_launch<24>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
-    case 32: upward_launch<32>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+#define TC_CASE(P) \
+    case P: upward_launch<P>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+        TC_ORDERS(TC_CASE)
+#undef TC_CASE
     }
 }
 

@@ -23,6 +23,15 @@
 #define TC_MAXORDER 32
 #define TC_STACK 48  // DFS stack: 3 * LMAX + 1 entries suffice
 
+// Expansion orders with compiled kernels: X(p) for each.
+#define TC_ORDERS(X) X(6) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(32)
+inline bool tc_order_supported(int p)
+{
+#define TC_ORDER_EQ(P) || p == P
+    return false TC_ORDERS(TC_ORDER_EQ);
+#undef TC_ORDER_EQ
+}
+
 struct alignas(16) TcNode {
     double xcom, ycom, r, mass;
     int s, e, child, level;  // child: first child id, 0 for a leaf

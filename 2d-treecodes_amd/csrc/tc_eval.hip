@@ -317,7 +317,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 // C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
 template <int P>
 TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double4 *qrec, int nq, double x0b,
-                     double y0b, double (*s_binom)[64], double2 *s_c, double &lre, double &lim)
+                     double y0b, double (*s_binom)[64], const double *s_ninv, double2 *s_c, double &lre,
+                     double &lim)
 {
 #pragma clang fp contract(fast)
     const int lane = tc_lane();
@@ -329,7 +330,7 @@ TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double
         const double4 nd = qrec[h];  // {xcom, ycom, mass, -}
         mass = nd.z;
         const double zx = nd.x - x0b, zy = nd.y - y0b;
-        const double inv = 1.0 / (zx * zx + zy * zy);
+        const double inv = rcp64(zx * zx + zy * zy);
         ur = zx * inv;
         ui = -zy * inv;
         double br = ur, bi = ui;
@@ -356,7 +357,7 @@ TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double
     tc_wave_sync();
     if (on) {
         const int L = l + 1;
-        double rt = (L == 1) ? -mass : mass * (-(1.0 / (double)L));
+        double rt = mass * s_ninv[L];  // -Q / L
         double it = 0;
         const double2 *c = s_c + h * 33;
         for (int kk = 1; kk <= P; ++kk) {
@@ -447,9 +448,12 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
     __shared__ int s_e2lq[4][2];
     __shared__ double4 s_e2lrec[4][2];
     __shared__ double s_binom[TC_MAXORDER][64];
+    __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
     for (int i = threadIdx.x; i < P * 64; i += 256)
         s_binom[i >> 6][i & 63] = c_binomT[i];
+    if (threadIdx.x <= TC_MAXORDER)
+        s_ninv[threadIdx.x] = threadIdx.x ? -(1.0 / (double)threadIdx.x) : 0.0;
     __syncthreads();
 
     const int wid = threadIdx.x >> 6, lane = tc_lane();
@@ -508,8 +512,8 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
                 ++nq;
                 if (nq == 2) {
                     tc_wave_sync();
-                    e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], 2, bk.x0b, bk.y0b, s_binom, s_coef[wid], lre,
-                                lim);
+                    e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], 2, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid],
+                                lre, lim);
                     nq = 0;
                 }
                 continue;
@@ -620,7 +624,8 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
 
         if (nq) {
             tc_wave_sync();
-            e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_coef[wid], lre, lim);
+            e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid], lre,
+                        lim);
         }
         // fold the two half-wave partial local expansions
         lre += __shfl_xor(lre, 32, 64);
@@ -972,12 +977,10 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
     if (ev0)
         TC_HIP(hipEventRecord(ev0, st));
     switch (t.order) {
-    case 8: launch_potential<8>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
-    case 12: launch_potential<12>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
-    case 16: launch_potential<16>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
-    case 20: launch_potential<20>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
-    case 24: launch_potential<24>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
-    case 32: launch_potential<32>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+#define TC_CASE(P) \
+    case P: launch_potential<P>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+        TC_ORDERS(TC_CASE)
+#undef TC_CASE
     default: throw TcError(-3, "tc_eval_potential: unsupported order");
     }
     if (ev1)
@@ -995,12 +998,10 @@ void tc_eval_force(const TcTree &t, double theta, const double *x0s, const doubl
     if (ev0)
         TC_HIP(hipEventRecord(ev0, st));
     switch (t.order) {
-    case 8: launch_force<8>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
-    case 12: launch_force<12>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
-    case 16: launch_force<16>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
-    case 20: launch_force<20>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
-    case 24: launch_force<24>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
-    case 32: launch_force<32>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+#define TC_CASE(P) \
+    case P: launch_force<P>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+        TC_ORDERS(TC_CASE)
+#undef TC_CASE
     default: throw TcError(-3, "tc_eval_force: unsupported order");
     }
     if (ev1)
@@ -1021,12 +1022,10 @@ void tc_eval_force_targets(const TcTree &t, double theta, const double *xt, cons
     if (ev0)
         TC_HIP(hipEventRecord(ev0, st));
     switch (t.order) {
-    case 8: launch_force_targets<8>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
-    case 12: launch_force_targets<12>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
-    case 16: launch_force_targets<16>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
-    case 20: launch_force_targets<20>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
-    case 24: launch_force_targets<24>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
-    case 32: launch_force_targets<32>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+#define TC_CASE(P) \
+    case P: launch_force_targets<P>(t, th2, xt, yt, tperm, m, xr, yr, precision, counters, st); break;
+        TC_ORDERS(TC_CASE)
+#undef TC_CASE
     default: throw TcError(-3, "tc_eval_force_targets: unsupported order");
     }
     if (ev1)

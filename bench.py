@@ -279,6 +279,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--tune", action="store_true",
                     help="re-run the autotuner for order and leaf capacity per precision (rel err <= 5e-7)")
+    ap.add_argument("--tune-cache", default="", metavar="PATH",
+                    help="file the autotuner records its choices in (default: the package's tuned.json)")
     ap.add_argument("--reference-params", action="store_true",
                     help="use the reference's order/leaf (12/512 potential, 24/192 force) instead of tuned ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -319,10 +321,11 @@ def main():
         if args.tune:
             from treecodes2d_amd import autotune
             log("autotuning", prec)
-            grid = dict(orders=(8, 12, 16), leaves=(64, 128, 256, 512)) if kind == "potential" else \
-                dict(orders=(16, 20, 24), leaves=(64, 128, 192, 384))
-            res = autotune.Tuner(pkg).tune(kind, data, theta=args.theta, precision=1 if prec == "f32" else 0,
-                                           err_target=5e-7, reps=2, **grid)
+            grid = dict(orders=(8, 10, 12, 14), leaves=(64, 128, 256, 512)) if kind == "potential" else \
+                dict(orders=(12, 14, 16, 20), leaves=(64, 128, 192, 384))
+            tuner = autotune.Tuner(pkg, cache_path=args.tune_cache or autotune.CACHE)
+            res = tuner.tune(kind, data, theta=args.theta, precision=1 if prec == "f32" else 0,
+                             err_target=5e-7, reps=2, **grid)
             order, leaf = res["order"], res["leaf"]
             tuned[prec] = {"order": order, "leaf": leaf, "err_sampled": res["err"],
                            "table": [(t["order"], t["leaf"], round(t["ms"], 3), t["err"]) for t in res["table"]]}

@@ -84,7 +84,7 @@ typedef struct tc_context tc_context;
 /* Options (tc_set_option keys) */
 enum {
     TC_OPT_PRECISION = 0,      /* 0 = plain FP64 (default), 1 = FP32 bulk P2P/E2P */
-    TC_OPT_POT_ORDER = 1,      /* potential expansion order (8,12,16,20,24,32), default 12 */
+    TC_OPT_POT_ORDER = 1,      /* potential expansion order (6,8,10,12,14,16,20,24,32), default 12 */
     TC_OPT_POT_LEAF = 2,       /* potential leaf capacity, default 512 */
     TC_OPT_FORCE_ORDER = 3,    /* force expansion order, default 24 */
     TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
