@@ -825,17 +825,24 @@ __global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int
 // the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta (force trees: j b_j,
 // the derivative series' coefficients).  Since
 // |a_j| <= sum|v| r^j / j, dividing the coefficient itself j times keeps every
-// intermediate bounded (no overflow of s^-j for tiny radii).
+// intermediate bounded (no overflow of s^-j for tiny radii).  Entry p of a
+// node's p + 1 records holds the E2P constants (s, Q ln2/2) for potential
+// trees (the FP32 E2P works with log2) and (s, Q) for force trees.
 template <int P>
 __global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const double *exps, int nnodes, int deriv,
                                                 float2 *ex32)
 {
     const int id = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = tc_lane();
-    if (id >= nnodes || lane >= P)
+    if (id >= nnodes || lane > P)
         return;
     const double r = nodes[id].r;
     const double sc = r > 1e-30 ? r : 1e-30;
+    if (lane == P) {
+        const double q = nodes[id].mass;
+        ex32[(size_t)(P + 1) * id + P] = make_float2((float)sc, (float)(deriv ? q : q * 0.34657359027997264));
+        return;
+    }
     const double *ex = exps + (size_t)2 * P * id;
     // force trees store k b_k, the coefficients of the derivative series
     const double k = deriv ? (double)(lane + 1) : 1.0;
@@ -844,7 +851,7 @@ __global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const doubl
         a /= sc;
         b /= sc;
     }
-    ex32[(size_t)P * id + lane] = make_float2((float)a, (float)b);
+    ex32[(size_t)(P + 1) * id + lane] = make_float2((float)a, (float)b);
 }
 
 int tiles_of(int n, int t) { return (n + t - 1) / t; }
@@ -1033,7 +1040,7 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     t.nodew.reserve(sizeof(double4) * (size_t)nn);
     t.selfid.reserve(sizeof(int) * (size_t)(T + 1));
     if (fp32)
-        t.exps32.reserve(sizeof(float2) * (size_t)order * nn);
+        t.exps32.reserve(sizeof(float2) * (size_t)(order + 1) * nn);
     ws.istart.reserve(sizeof(int) * (size_t)(T + 1));
     ws.ilevel.reserve(sizeof(int) * (size_t)(T + 1));
     // per-node topology scratch: child | se | level | cell

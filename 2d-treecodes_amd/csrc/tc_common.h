@@ -110,6 +110,16 @@ TC_DEV double tc_wave_tree_sum(double v)
     return v;
 }
 
+// Sum over the wavefront, butterfly order (every lane holds a sum of all 64
+// values; lanes may differ in the last bit).
+TC_DEV float tc_wave_sum_f32(float v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+        v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
 TC_DEV double tc_wave_max(double v)
 {
 #pragma unroll

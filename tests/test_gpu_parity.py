@@ -332,11 +332,16 @@ def test_force_targets(pkg, oracle, wl, ctx64, ctx32, precision):
 
 
 def test_target_order_invariance(pkg, wl, ctx64, ctx32):
-    # each lane accumulates in its own target's DFS order, so results do not depend on
+    # FP64: each lane accumulates in its own target's DFS order, so results do not depend on
     # how targets are grouped into waves: a permutation of the inputs permutes the outputs
+    # bit for bit.  FP32: a leaf that few lanes of a wave need is summed source-parallel
+    # (another rounding order), and which leaves those are depends on the grouping, so a
+    # permutation (which regroups targets whose sort keys tie) may move the last bits.
     xs, ys, vs, xt, yt = wl.mixed_plummer_potential(50000, seed=66)
     perm = np.random.default_rng(1).permutation(len(xt))
-    for ctx in (ctx64, ctx32):
-        a = ctx.potential(0.5, xs, ys, vs, xt, yt)
-        b = ctx.potential(0.5, xs, ys, vs, xt[perm], yt[perm])
-        assert np.array_equal(a[perm], b)
+    a = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    b = ctx64.potential(0.5, xs, ys, vs, xt[perm], yt[perm])
+    assert np.array_equal(a[perm], b)
+    a = ctx32.potential(0.5, xs, ys, vs, xt, yt)
+    b = ctx32.potential(0.5, xs, ys, vs, xt[perm], yt[perm])
+    assert rel_l2(b, a[perm]) < 1e-9
