@@ -585,6 +585,51 @@ __global__ __launch_bounds__(256) void k_int_nodes(const uint32_t *keys, int n, 
     }
 }
 
+// Reduce-scatter form of the pairwise lane tree: at offsets 32, 16, ..., 1 a
+// lane keeps half of its values and adds the partner's copies of that half,
+// so after log2(NV) halvings (then plain butterfly steps) lane l holds the
+// full sum of value (l >> (6 - log2 NV)).  Every sum is formed from the same
+// pairs in the same order as tc_wave_tree_sum (a + b == b + a), so results are
+// bit-identical to reducing each value separately, with NV - 1 + (6 - log2 NV)
+// shuffles instead of 6 NV.
+template <int C> TC_DEV void rs_halve(double *v, int off, bool up)
+{
+#pragma unroll
+    for (int j = 0; j < C / 2; ++j) {
+        const double mine = up ? v[C / 2 + j] : v[j];
+        const double send = up ? v[j] : v[C / 2 + j];
+        v[j] = mine + __shfl_xor(send, off, 64);
+    }
+}
+
+template <int NV> TC_DEV double wave_reduce_scatter(double *v)
+{
+    static_assert(NV == 4 || NV == 32 || NV == 64, "reduce-scatter width");
+    const int lane = tc_lane();
+    if (NV == 64) {
+        rs_halve<64>(v, 32, lane & 32);
+        rs_halve<32>(v, 16, lane & 16);
+        rs_halve<16>(v, 8, lane & 8);
+        rs_halve<8>(v, 4, lane & 4);
+        rs_halve<4>(v, 2, lane & 2);
+        rs_halve<2>(v, 1, lane & 1);
+    } else if (NV == 32) {
+        rs_halve<32>(v, 32, lane & 32);
+        rs_halve<16>(v, 16, lane & 16);
+        rs_halve<8>(v, 8, lane & 8);
+        rs_halve<4>(v, 4, lane & 4);
+        rs_halve<2>(v, 2, lane & 2);
+        v[0] = v[0] + __shfl_xor(v[0], 1, 64);
+    } else {
+        rs_halve<4>(v, 32, lane & 32);
+        rs_halve<2>(v, 16, lane & 16);
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1)
+            v[0] = v[0] + __shfl_xor(v[0], off, 64);
+    }
+    return v[0];
+}
+
 // ------------------------------------------------------------------ upward
 // One wavefront per leaf: build_leaf (upward.cpp:61-93) = upward_setup
 // (upward-common.ispc:13-57) + upward_p2e (upward-kernels.m4:52-92), in the
@@ -615,8 +660,11 @@ __global__ __launch_bounds__(256) void k_leaf_upward(const uint32_t *child, cons
         px += q.x * a;
         py += q.y * a;
     }
-    const double m = tc_wave_tree_sum(pm), w = tc_wave_tree_sum(pw);
-    const double wx = tc_wave_tree_sum(px), wy = tc_wave_tree_sum(py);
+    // lane 0: mass, 16: w, 32: wx, 48: wy (bit-identical to four tree sums)
+    double g4[4] = {pm, pw, px, py};
+    const double gsum = wave_reduce_scatter<4>(g4);
+    const double m = __shfl(gsum, 0, 64), w = __shfl(gsum, 16, 64);
+    const double wx = __shfl(gsum, 32, 64), wy = __shfl(gsum, 48, 64);
     const double xc = w != 0 ? wx / w : (x0 + 0.5 * h);
     const double yc = w != 0 ? wy / w : (y0 + 0.5 * h);
 
@@ -657,14 +705,18 @@ __global__ __launch_bounds__(256) void k_leaf_upward(const uint32_t *child, cons
             src32[i] = make_float4((float)r0, (float)i0, (float)v, 0.f);
     }
     double *ex = exps + (size_t)2 * P * id;
+    {
+        // [Re a_1..a_p][Im a_1..a_p] reduced together; lane l ends with entry
+        // l (64 entries) or l >> 1 (32 entries)
+        constexpr int NV = 2 * P <= 32 ? 32 : 64;
+        double v[NV];
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-        const double a = tc_wave_tree_sum(re[k]);
-        const double b = tc_wave_tree_sum(im[k]);
-        if (lane == k) {
-            ex[k] = a;
-            ex[P + k] = b;
-        }
+        for (int k = 0; k < NV; ++k)
+            v[k] = k < P ? re[k] : (k < 2 * P ? im[k - P] : 0.0);
+        const double sum = wave_reduce_scatter<NV>(v);
+        const int idx = NV == 64 ? lane : lane >> 1;
+        if (idx < 2 * P && (NV == 64 || (lane & 1) == 0))
+            ex[idx] = sum;
     }
     if (lane == 0) {
         TcNode nd;
@@ -824,34 +876,39 @@ __global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int
 // FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, 1e-30), so that
 // the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta (force trees: j b_j,
 // the derivative series' coefficients).  Since
-// |a_j| <= sum|v| r^j / j, dividing the coefficient itself j times keeps every
-// intermediate bounded (no overflow of s^-j for tiny radii).  Entry p of a
+// |a_j| <= sum|v| r^j / j and s >= r, scaling the coefficient itself by 1/s
+// j times keeps every intermediate bounded (no overflow of s^-j for tiny
+// radii).  Entry p of a
 // node's p + 1 records holds the E2P constants (s, Q ln2/2) for potential
 // trees (the FP32 E2P works with log2) and (s, Q) for force trees.
 template <int P>
 __global__ __launch_bounds__(256) void k_exps32(const TcNode *nodes, const double *exps, int nnodes, int deriv,
                                                 float2 *ex32)
 {
-    const int id = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // G lanes per node (p + 1 records each), 64 / G nodes per wave
+    constexpr int G = P + 1 <= 16 ? 16 : (P + 1 <= 32 ? 32 : 64);
     const int lane = tc_lane();
-    if (id >= nnodes || lane > P)
+    const int id = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / G) + lane / G;
+    const int k = lane % G;
+    if (id >= nnodes || k > P)
         return;
     const double r = nodes[id].r;
     const double sc = r > 1e-30 ? r : 1e-30;
-    if (lane == P) {
+    if (k == P) {
         const double q = nodes[id].mass;
         ex32[(size_t)(P + 1) * id + P] = make_float2((float)sc, (float)(deriv ? q : q * 0.34657359027997264));
         return;
     }
     const double *ex = exps + (size_t)2 * P * id;
-    // force trees store k b_k, the coefficients of the derivative series
-    const double k = deriv ? (double)(lane + 1) : 1.0;
-    double a = k * ex[lane], b = k * ex[P + lane];
-    for (int j = 0; j <= lane; ++j) {
-        a /= sc;
-        b /= sc;
+    // force trees store j b_j, the coefficients of the derivative series
+    const double kj = deriv ? (double)(k + 1) : 1.0;
+    const double inv = 1.0 / sc;
+    double a = kj * ex[k], b = kj * ex[P + k];
+    for (int j = 0; j <= k; ++j) {
+        a *= inv;
+        b *= inv;
     }
-    ex32[(size_t)(P + 1) * id + lane] = make_float2((float)a, (float)b);
+    ex32[(size_t)(P + 1) * id + k] = make_float2((float)a, (float)b);
 }
 
 int tiles_of(int n, int t) { return (n + t - 1) / t; }
@@ -971,8 +1028,9 @@ static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, cons
         }
     }
     if (fp32) {
-        k_exps32<P><<<tiles_of(t.nnodes, 4), 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), t.nnodes,
-                                                            deriv ? 1 : 0, t.exps32.as<float2>());
+        constexpr int per_wave = P + 1 <= 16 ? 4 : (P + 1 <= 32 ? 2 : 1);
+        k_exps32<P><<<tiles_of(t.nnodes, 4 * per_wave), 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(),
+                                                                        t.nnodes, deriv ? 1 : 0, t.exps32.as<float2>());
         TC_HIP(hipGetLastError());
     }
 }
