@@ -363,19 +363,24 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 }
 
 // ------------------------------------------------------------------ force kernels
-// Multipole-to-local (downward_e2l, force-kernels.m4:102-155) for up to two
-// nodes at once: lanes 0-31 take node q[0], lanes 32-63 node q[1]; lane l of a
-// half owns local coefficient l+1.  z0 = com - brick centre, u = 1/z0; each
-// lane forms u^(l+1) by binary powering, publishes (-1)^k a_k u^k, then sums
+// Multipole-to-local (downward_e2l, force-kernels.m4:102-155) for up to
+// E2L_NQ nodes at once: the wave splits into groups of E2L_G lanes (16 for
+// p <= 16, else 32), group g takes node q[g] and lane l of a group owns local
+// coefficient l+1.  z0 = com - brick centre, u = 1/z0; each lane forms
+// u^(l+1) by binary powering, publishes (-1)^k a_k u^k, then sums
 // C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
+template <int P> constexpr int E2L_G = P <= 16 ? 16 : 32;
+template <int P> constexpr int E2L_NQ = 64 / E2L_G<P>;
+
 template <int P>
-TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double4 *qrec, int nq, double x0b,
-                     double y0b, double (*s_binom)[64], const double *s_ninv, double2 *s_c, double &lre,
-                     double &lim)
+TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const double4 *qrec, int nq, double x0b,
+                      double y0b, double (*s_binom)[64], const double *s_ninv, double2 *s_c, double &lre,
+                      double &lim)
 {
 #pragma clang fp contract(fast)
+    constexpr int G = E2L_G<P>;
     const int lane = tc_lane();
-    const int h = lane >> 5, l = lane & 31;
+    const int h = lane / G, l = lane % G;
     const bool on = h < nq && l < P;
     double ur = 0, ui = 0, pr = 1, pi = 0, mass = 0;
     if (h < nq) {
@@ -387,9 +392,9 @@ TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double
         ur = zx * inv;
         ui = -zy * inv;
         double br = ur, bi = ui;
-        int e = l + 1;
+        int e = l + 1;  // <= G: 5 bits for G = 16, 6 for G = 32
 #pragma unroll
-        for (int bit = 0; bit < 6; ++bit) {
+        for (int bit = 0; bit < (G == 16 ? 5 : 6); ++bit) {
             if (e & 1) {
                 const double tr = pr * br - pi * bi;
                 pi = pr * bi + pi * br;
@@ -404,7 +409,7 @@ TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double
             const double *ex = exps + (size_t)2 * P * id;
             const double ar = ex[l], ai = ex[P + l];
             const double sgn = (l & 1) ? 1.0 : -1.0;  // (-1)^k, k = l + 1
-            s_c[h * 33 + l + 1] = make_double2(sgn * (ar * pr - ai * pi), sgn * (ar * pi + ai * pr));
+            s_c[h * (G + 1) + l + 1] = make_double2(sgn * (ar * pr - ai * pi), sgn * (ar * pi + ai * pr));
         }
     }
     tc_wave_sync();
@@ -412,7 +417,7 @@ TC_DEV void e2l_pair(const double *__restrict__ exps, const int *q, const double
         const int L = l + 1;
         double rt = mass * s_ninv[L];  // -Q / L
         double it = 0;
-        const double2 *c = s_c + h * 33;
+        const double2 *c = s_c + h * (G + 1);
         for (int kk = 1; kk <= P; ++kk) {
             const double bb = s_binom[kk - 1][L + kk - 1];
             const double2 ck = c[kk];
@@ -497,9 +502,9 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
     __shared__ int s_node[4][TC_STACK];
     __shared__ TcStackRec s_stk[4][TC_STACK];
     __shared__ double4 s_src[4][64];
-    __shared__ double2 s_coef[4][2 * 33];
-    __shared__ int s_e2lq[4][2];
-    __shared__ double4 s_e2lrec[4][2];
+    __shared__ double2 s_coef[4][68];  // E2L_NQ groups of E2L_G + 1 entries; L2P uses [1..P]
+    __shared__ int s_e2lq[4][4];
+    __shared__ double4 s_e2lrec[4][4];
     __shared__ double s_binom[TC_MAXORDER][64];
     __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
@@ -555,7 +560,7 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
                 continue;
             const int id = tc_uniform(s_node[wid][sp]);
             if (action == ACT_E2L) {
-                // queued; downward_e2l runs on pairs of nodes (e2l_pair)
+                // queued; downward_e2l runs on E2L_NQ nodes at a time (e2l_group)
                 ++ne2l;
                 if (lane == 0) {
                     const TcStackRec &q = s_stk[wid][sp];
@@ -563,10 +568,10 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
                     s_e2lrec[wid][nq] = make_double4(q.xcom, q.ycom, q.mass, 0.0);
                 }
                 ++nq;
-                if (nq == 2) {
+                if (nq == E2L_NQ<P>) {
                     tc_wave_sync();
-                    e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], 2, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid],
-                                lre, lim);
+                    e2l_group<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid],
+                                 lre, lim);
                     nq = 0;
                 }
                 continue;
@@ -676,12 +681,15 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
 
         if (nq) {
             tc_wave_sync();
-            e2l_pair<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid], lre,
-                        lim);
+            e2l_group<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid], lre,
+                         lim);
         }
-        // fold the two half-wave partial local expansions
-        lre += __shfl_xor(lre, 32, 64);
-        lim += __shfl_xor(lim, 32, 64);
+        // fold the groups' partial local expansions into lanes 0..E2L_G-1
+#pragma unroll
+        for (int off = 32; off >= E2L_G<P>; off >>= 1) {
+            lre += __shfl_xor(lre, off, 64);
+            lim += __shfl_xor(lim, off, 64);
+        }
         // downward_l2p_8x8 (force-kernels.m4:157-191) about the brick centre
         if (lane < P)
             s_coef[wid][lane + 1] = make_double2(lre, lim);

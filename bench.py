@@ -117,6 +117,20 @@ class GpuCase:
         avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
         return dt, avg
 
+    def host_timed(self, data, reps=3):
+        """Best of `reps` solves through the host-pointer path (inputs and outputs in pageable host
+        memory, so each solve includes the H2D and D2H copies)."""
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            if self.kind == "potential":
+                self.ctx.potential(self.theta, data["xs"], data["ys"], data["vs"], data["xt"], data["yt"])
+            else:
+                self.ctx.force(self.theta, data["xs"], data["ys"], data["vs"], data["x0s"], data["y0s"], data["hs"])
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
     def counters(self):
         self.ctx.set(pkg.OPT_COUNTERS, 1)
         self.step()
@@ -271,6 +285,7 @@ def main():
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-array (PCIe-inclusive) timing")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64"], default=None, help="time one precision only (profiling)")
     ap.add_argument("--serve-requests", type=int, default=2000,
@@ -359,6 +374,13 @@ def main():
     if not args.no_error and "f32" in cases and "f64" in cases:
         log("direct-sum error check")
         errs = sampled_error(cases["f32"], cases["f64"], kind, data)
+    host = None
+    if not args.no_host:
+        log("host-array C ABI timing")
+        hdt = cases[head].host_timed(data)
+        host = {"ms": hdt * 1e3, "mtargets_per_s": m / hdt / 1e6, "dtype": head,
+                "path": "context solve with host arrays (pageable): H2D copies, build, solve, D2H copy"}
+        log("  %.3f ms -> %.1f Mtargets/s" % (hdt * 1e3, host["mtargets_per_s"]))
     serving = None
     if args.serve_requests > 0:
         log("served stream: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
@@ -395,6 +417,7 @@ def main():
                      "note": "VALU-bound: peak is the vector %s rate (no MFMA in this path)" % head.upper()},
         "cpu_baseline": cpu,
         "serving": serving,
+        "host_abi": host,
     }
     if "f32" in timings and "f64" in timings:
         res["fp64_value"] = m / timings["f64"][0] / 1e6
