@@ -148,9 +148,6 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
         const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
         double *dout = out;
         if (!device_ptrs) {
-            for (int i = 0; i < m; ++i)
-                if (!std::isfinite(xt[i]) || !std::isfinite(yt[i]))
-                    throw TcError(-4, "treecode_potential: non-finite target coordinate");
             const size_t ns = rebuild ? (size_t)n : 0;
             c->io.reserve(sizeof(double) * (3 * ns + 3 * (size_t)m));
             double *base = c->io.as<double>();
@@ -175,8 +172,14 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
         c->ev_valid = c->profile;
         c->eval_has_tsort = true;
         if (!device_ptrs) {
+            // the target sort's bounding-box pass flags non-finite coordinates
+            uint32_t bad = 0;
             TC_HIP(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->ws, m), sizeof(bad), hipMemcpyDeviceToHost,
+                                  c->stream));
             TC_HIP(hipStreamSynchronize(c->stream));
+            if (bad)
+                throw TcError(-4, "treecode_potential: non-finite target coordinate");
         }
     });
 }
@@ -267,9 +270,6 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
         const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
         double *dxr = xr, *dyr = yr;
         if (!device_ptrs) {
-            for (int i = 0; i < m; ++i)
-                if (!std::isfinite(xt[i]) || !std::isfinite(yt[i]))
-                    throw TcError(-4, "treecode_force: non-finite target coordinate");
             c->io.reserve(sizeof(double) * (3 * (size_t)n + 4 * (size_t)m));
             double *base = c->io.as<double>();
             double *bxs = base, *bys = base + n, *bvs = base + 2 * (size_t)n;
@@ -291,9 +291,14 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
                               c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
         if (!device_ptrs) {
+            uint32_t bad = 0;
             TC_HIP(hipMemcpyAsync(xr, dxr, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
             TC_HIP(hipMemcpyAsync(yr, dyr, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+            TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->ws, m), sizeof(bad), hipMemcpyDeviceToHost,
+                                  c->stream));
             TC_HIP(hipStreamSynchronize(c->stream));
+            if (bad)
+                throw TcError(-4, "treecode_force: non-finite target coordinate");
         }
     });
 }

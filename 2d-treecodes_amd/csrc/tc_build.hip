@@ -1126,6 +1126,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     }
 }
 
+const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m) { return tc_bbox_bad_flag(ws, m); }
+
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st)
 {
     ws.tkeys.reserve(sizeof(uint32_t) * (size_t)m + 64);

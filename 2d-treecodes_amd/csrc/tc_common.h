@@ -224,6 +224,9 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
                    bool fp32, bool deriv32, TcScratch &ws, hipStream_t st);
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);
+// Device flag (uint32, nonzero = a non-finite coordinate) set by the last
+// tc_sort_targets call on m targets; valid until the next bounding-box pass.
+const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m);
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
                        hipEvent_t ev0, hipEvent_t ev1);

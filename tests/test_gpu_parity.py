@@ -220,6 +220,15 @@ def test_empty_and_error_paths(pkg, wl, ctx64):
     # the context stays usable after an error
     xs[10] = 0.5
     ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    # non-finite targets are reported too (flagged on the device by the target sort)
+    bad = yt.copy()
+    bad[777] = np.inf
+    with pytest.raises(pkg.TreecodeError):
+        ctx64.potential(0.5, xs, ys, vs, xt, bad)
+    with pytest.raises(pkg.TreecodeError):
+        ctx64.force_targets(0.5, xs, ys, vs, xt, bad)
+    good = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    assert np.all(np.isfinite(good))
 
 
 def test_coincident_sources(pkg, oracle, ctx64, ctx32):
