@@ -16,6 +16,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 void tc_upload_binomials_eval(hipStream_t st);
@@ -35,6 +36,11 @@ struct tc_context {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_valid = false, eval_has_tsort = false;
     unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0, 0, 0};
+    // pinned staging for host arrays: two buffers, so the CPU copy of one
+    // chunk overlaps the DMA of the other (allocated on first large copy)
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_busy[2] = {false, false};
 };
 
 namespace {
@@ -78,10 +84,107 @@ void ensure_binomials(int device, hipStream_t st)
     }
 }
 
-void copy_in(const double *h, double *d, int n, hipStream_t st)
+constexpr size_t STAGE_BYTES = size_t(16) << 20;  // per staging buffer
+constexpr size_t STAGE_MIN = size_t(4) << 20;     // smaller copies go direct
+
+// memcpy split over up to 8 threads (pageable <-> pinned host memory).
+void par_memcpy(void *dst, const void *src, size_t bytes)
 {
-    if (n > 0)
-        TC_HIP(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st));
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nt = bytes < (size_t(2) << 20) ? 1 : std::min<size_t>(8, hw ? hw : 1);
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = ((bytes + nt - 1) / nt + 63) & ~size_t(63);
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) {
+        const size_t o = t * part;
+        if (o >= bytes)
+            break;
+        th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, std::min(part, bytes - o)); });
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (auto &t : th)
+        t.join();
+}
+
+void stage_ready(tc_context *c)
+{
+    if (c->stage[0])
+        return;
+    for (int b = 0; b < 2; ++b) {
+        TC_HIP(hipHostMalloc(&c->stage[b], STAGE_BYTES, hipHostMallocDefault));
+        TC_HIP(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
+    }
+}
+
+// Waits until staging buffer b is no longer read or written by the device.
+void stage_wait(tc_context *c, int b)
+{
+    if (c->stage_busy[b]) {
+        TC_HIP(hipEventSynchronize(c->stage_ev[b]));
+        c->stage_busy[b] = false;
+    }
+}
+
+// Host array -> device, stream-ordered.  Large copies go through the pinned
+// staging buffers in 16 MiB chunks: the CPU fills one buffer while the DMA
+// engine drains the other.
+void copy_in(tc_context *c, const double *h, double *d, long long n)
+{
+    const size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 0);
+    if (!bytes)
+        return;
+    if (bytes < STAGE_MIN) {
+        TC_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+        return;
+    }
+    stage_ready(c);
+    int b = 0;
+    for (size_t o = 0; o < bytes; o += STAGE_BYTES, b ^= 1) {
+        const size_t len = std::min(STAGE_BYTES, bytes - o);
+        stage_wait(c, b);
+        par_memcpy(c->stage[b], (const char *)h + o, len);
+        TC_HIP(hipMemcpyAsync((char *)d + o, c->stage[b], len, hipMemcpyHostToDevice, c->stream));
+        TC_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+        c->stage_busy[b] = true;
+    }
+}
+
+// Device -> host array; returns with the data in h (synchronises the stream).
+void copy_out(tc_context *c, const double *d, double *h, long long n)
+{
+    const size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 0);
+    if (bytes < STAGE_MIN) {
+        if (bytes)
+            TC_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+        TC_HIP(hipStreamSynchronize(c->stream));
+        return;
+    }
+    stage_ready(c);
+    size_t pend_o[2] = {0, 0}, pend_len[2] = {0, 0};
+    int b = 0;
+    auto drain = [&](int k) {
+        if (pend_len[k]) {
+            stage_wait(c, k);
+            par_memcpy((char *)h + pend_o[k], c->stage[k], pend_len[k]);
+            pend_len[k] = 0;
+        }
+    };
+    for (size_t o = 0; o < bytes; o += STAGE_BYTES, b ^= 1) {
+        const size_t len = std::min(STAGE_BYTES, bytes - o);
+        drain(b);
+        stage_wait(c, b);
+        TC_HIP(hipMemcpyAsync(c->stage[b], (const char *)d + o, len, hipMemcpyDeviceToHost, c->stream));
+        TC_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+        c->stage_busy[b] = true;
+        pend_o[b] = o;
+        pend_len[b] = len;
+    }
+    drain(b);
+    drain(b ^ 1);
+    TC_HIP(hipStreamSynchronize(c->stream));
 }
 
 unsigned long long *prepare_counters(tc_context *c)
@@ -155,12 +258,12 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             double *bxt = base + 3 * ns, *byt = bxt + m;
             dout = byt + m;
             if (rebuild) {
-                copy_in(xs, bxs, n, c->stream);
-                copy_in(ys, bys, n, c->stream);
-                copy_in(vs, bvs, n, c->stream);
+                copy_in(c, xs, bxs, n);
+                copy_in(c, ys, bys, n);
+                copy_in(c, vs, bvs, n);
             }
-            copy_in(xt, bxt, m, c->stream);
-            copy_in(yt, byt, m, c->stream);
+            copy_in(c, xt, bxt, m);
+            copy_in(c, yt, byt, m);
             dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
         }
         unsigned long long *cnt = prepare_counters(c);
@@ -174,10 +277,9 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
         if (!device_ptrs) {
             // the target sort's bounding-box pass flags non-finite coordinates
             uint32_t bad = 0;
-            TC_HIP(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
             TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->ws, m), sizeof(bad), hipMemcpyDeviceToHost,
                                   c->stream));
-            TC_HIP(hipStreamSynchronize(c->stream));
+            copy_out(c, dout, out, m);
             if (bad)
                 throw TcError(-4, "treecode_potential: non-finite target coordinate");
         }
@@ -221,13 +323,13 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
             dxr = bh + nb;
             dyr = dxr + nt;
             if (rebuild) {
-                copy_in(xs, bxs, n, c->stream);
-                copy_in(ys, bys, n, c->stream);
-                copy_in(vs, bvs, n, c->stream);
+                copy_in(c, xs, bxs, n);
+                copy_in(c, ys, bys, n);
+                copy_in(c, vs, bvs, n);
             }
-            copy_in(x0s, bx0, nb, c->stream);
-            copy_in(y0s, by0, nb, c->stream);
-            copy_in(hs, bh, nb, c->stream);
+            copy_in(c, x0s, bx0, nb);
+            copy_in(c, y0s, by0, nb);
+            copy_in(c, hs, bh, nb);
             dxs = bxs; dys = bys; dvs = bvs; dx0 = bx0; dy0 = by0; dh = bh;
         }
         unsigned long long *cnt = prepare_counters(c);
@@ -241,9 +343,8 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
         c->ev_valid = c->profile;
         c->eval_has_tsort = false;
         if (!device_ptrs) {
-            TC_HIP(hipMemcpyAsync(xr, dxr, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream));
-            TC_HIP(hipMemcpyAsync(yr, dyr, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream));
-            TC_HIP(hipStreamSynchronize(c->stream));
+            copy_out(c, dxr, xr, (long long)nt);
+            copy_out(c, dyr, yr, (long long)nt);
         }
     });
 }
@@ -276,11 +377,11 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
             double *bxt = base + 3 * (size_t)n, *byt = bxt + m;
             dxr = byt + m;
             dyr = dxr + m;
-            copy_in(xs, bxs, n, c->stream);
-            copy_in(ys, bys, n, c->stream);
-            copy_in(vs, bvs, n, c->stream);
-            copy_in(xt, bxt, m, c->stream);
-            copy_in(yt, byt, m, c->stream);
+            copy_in(c, xs, bxs, n);
+            copy_in(c, ys, bys, n);
+            copy_in(c, vs, bvs, n);
+            copy_in(c, xt, bxt, m);
+            copy_in(c, yt, byt, m);
             dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
         }
         unsigned long long *cnt = prepare_counters(c);
@@ -292,11 +393,10 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
         c->ev_valid = c->profile;
         if (!device_ptrs) {
             uint32_t bad = 0;
-            TC_HIP(hipMemcpyAsync(xr, dxr, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
-            TC_HIP(hipMemcpyAsync(yr, dyr, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
             TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->ws, m), sizeof(bad), hipMemcpyDeviceToHost,
                                   c->stream));
-            TC_HIP(hipStreamSynchronize(c->stream));
+            copy_out(c, dxr, xr, m);
+            copy_out(c, dyr, yr, m);
             if (bad)
                 throw TcError(-4, "treecode_force: non-finite target coordinate");
         }
@@ -383,6 +483,12 @@ TC_API void tc_context_destroy(tc_context *c)
     for (auto &e : c->ev)
         if (e)
             (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; ++b) {
+        if (c->stage[b])
+            (void)hipHostFree(c->stage[b]);
+        if (c->stage_ev[b])
+            (void)hipEventDestroy(c->stage_ev[b]);
+    }
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

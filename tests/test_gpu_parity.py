@@ -196,6 +196,26 @@ def test_dropin_abi(pkg, oracle, wl):
     assert rel_l2(fx, rx) <= FP64_TOL and rel_l2(fy, ry) <= FP64_TOL
 
 
+def test_host_arrays_staged_copies(pkg, wl, ctx32):
+    # host arrays of 4 MiB and more travel through the pinned staging buffers in
+    # 16 MiB chunks (2.1M doubles = 2 chunks): results equal the device-array path
+    xs, ys, vs, xt, yt = wl.uniform_potential(2_100_000, seed=71)
+    host = ctx32.potential(0.5, xs, ys, vs, xt, yt)
+    d = {k: ctx32.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt).items()}
+    out = ctx32.empty(len(xt))
+    ctx32.potential(0.5, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
+    ctx32.synchronize()
+    assert np.array_equal(host, out.numpy())
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(600_000, nbx=50, h=1.0 / 2048, seed=72)
+    x0s, y0s, hs = x0s[:2100].copy(), y0s[:2100].copy(), hs[:2100].copy()  # 2.15M targets
+    fx, fy = ctx32.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    d = {k: ctx32.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs).items()}
+    xr, yr = ctx32.empty(len(x0s) * 1024), ctx32.empty(len(x0s) * 1024)
+    ctx32.force(0.5, d["xs"], d["ys"], d["vs"], d["x0s"], d["y0s"], d["hs"], xr=xr, yr=yr, device=True)
+    ctx32.synchronize()
+    assert np.array_equal(fx, xr.numpy()) and np.array_equal(fy, yr.numpy())
+
+
 def test_deterministic_and_resident(pkg, wl, ctx64):
     xs, ys, vs, xt, yt = wl.uniform_potential(65536, seed=14)
     a = ctx64.potential(0.5, xs, ys, vs, xt, yt)
