@@ -123,8 +123,16 @@ def gaussian16_force(n, nblocks=16384, seed=1455):
     centres = 0.1 + 0.8 * st.uniform(32).reshape(16, 2)
     sig = np.exp(np.log(0.005) + (np.log(0.05) - np.log(0.005)) * st.uniform(16))
     xs, ys, vs = _gaussian_mixture(st, n, centres, sig)
-    # blocks: quarter at h=1/4096, half at 1/2048, quarter at 1/1024 on the densest cells
-    counts = [nblocks // 4, nblocks // 2, nblocks - nblocks // 4 - nblocks // 2]
+    # blocks on the densest cells of each spacing: a quarter at h=1/1024 and half at
+    # 1/2048 (at most all 32^2 and 64^2 cells of [0,1]^2), the rest at h=1/4096 (at
+    # most 128^2), so that nblocks <= 21504 blocks exist and finer blocks sit where
+    # the particles are densest
+    coarse = min(nblocks // 4, 32 * 32)
+    mid = min(nblocks // 2, 64 * 64)
+    fine = nblocks - coarse - mid
+    if fine > 128 * 128:
+        raise ValueError("gaussian16_force: at most 21504 blocks")
+    counts = [fine, mid, coarse]
     x0l, y0l, hl = [], [], []
     for h, cnt in zip([1.0 / 4096, 1.0 / 2048, 1.0 / 1024], counts):
         side = 32 * h
