@@ -130,6 +130,33 @@ def test_potential_fp64_vs_oracle(pkg, oracle, wl, ctx64, theta):
     assert (st["p2p"], st["e2p"]) == (cnt[0], cnt[1])
 
 
+@pytest.mark.parametrize("order,leaf", [(6, 256), (10, 256), (14, 128), (20, 64), (32, 512)])
+def test_potential_orders_vs_oracle(pkg, oracle, wl, order, leaf):
+    # every compiled expansion order (tuned configurations use 10 / 256 and 10 / 64)
+    ctx = pkg.Context(counters=True, pot_order=order, pot_leaf=leaf)
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(30000, seed=40 + order)
+    out = ctx.potential(0.5, xs, ys, vs, xt, yt)
+    ref, cnt = oracle.potential(0.5, xs, ys, vs, xt, yt, order=order, leafcap=leaf)
+    assert rel_l2(out, ref) <= FP64_TOL
+    st = ctx.stats()["counters"]
+    assert (st["p2p"], st["e2p"]) == (cnt[0], cnt[1])
+    ctx.close()
+
+
+@pytest.mark.parametrize("order,leaf", [(8, 64), (12, 128), (12, 192), (16, 192), (20, 128)])
+def test_force_orders_vs_oracle(pkg, oracle, wl, order, leaf):
+    # E2L runs four nodes per wave for orders <= 16 and two above
+    ctx = pkg.Context(counters=True, force_order=order, force_leaf=leaf)
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian16_force(40000, nblocks=64, seed=60 + order)
+    fx, fy = ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, cnt = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs, order=order, leafcap=leaf)
+    assert rel_l2(fx, rx) <= FP64_TOL
+    assert rel_l2(fy, ry) <= FP64_TOL
+    st = ctx.stats()["counters"]
+    assert (st["p2p"], st["e2p"], st["e2l"], st["l2p"]) == tuple(int(c) for c in cnt)
+    ctx.close()
+
+
 def test_potential_targets_elsewhere(pkg, oracle, wl, ctx64):
     # targets outside the source box, in random order, fewer than sources
     st = wl.Stream(77)
