@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
             // target.  Chosen when k (C 9.5 + 16.5) < n 9.8, the measured
             // nanoseconds per SIMD of the two forms (FP32 op 1.18 ns, log 3.6 ns;
             // C chunks of 64 of the n sources, k active lanes).
-            if (F32 && nact * (nchunk * 19 + 33) < nsrc * 20) {
+            if (F32 && (long long)nact * (nchunk * 19 + 33) < (long long)nsrc * 20) {
                 nslots += 64ull * (unsigned long long)nchunk * (unsigned long long)nact;
                 const float txl = (float)(x - nd.xcom), tyl = (float)(y - nd.ycom);
                 unsigned long long rest = omask;
