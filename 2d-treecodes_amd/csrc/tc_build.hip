@@ -736,19 +736,28 @@ __global__ __launch_bounds__(256) void k_leaf_upward(const uint32_t *child, cons
     }
 }
 
-// One 64-thread block per internal node of one level: the parent update of
-// upward.cpp:95-218 and upward_e2e (upward-kernels.m4:95-149), children
-// summed in order c = 0..3 (lane l owns coefficient l+1).
+// Parent update of upward.cpp:95-218 and upward_e2e (upward-kernels.m4:95-149)
+// for internal node `rank`, by one wavefront.  The children's translations run
+// in parallel: lane group g (E2E_G lanes: 16 for p <= 16, else 32) takes child
+// g of a round, one lane of it forms the powers z0^-j by the reference's
+// recurrence, lane l the coefficient l+1.  Each child's contribution lands in
+// LDS and lanes add them in order c = 0..3, so every coefficient is formed by
+// the same operations as a child-by-child loop.
+template <int P> constexpr int E2E_G = P <= 16 ? 16 : 32;
+
+template <int P> struct E2eLds {
+    double rinv[64 / E2E_G<P>][P + 1], iinv[64 / E2E_G<P>][P + 1];
+    double rc[64 / E2E_G<P>][P + 1], ic[64 / E2E_G<P>][P + 1];
+    double cre[4][P], cim[4][P];
+};
+
 template <int P>
-__global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int r1, const int2 *se,
-                                                   const int *level, const int2 *cell, const double *box,
-                                                   TcNode *nodes, double4 *nodew, double *exps)
+TC_DEV void e2e_node(int rank, const int *selfid, const int2 *se, const int *level, const int2 *cell,
+                     const double *box, TcNode *nodes, double4 *nodew, double *exps, E2eLds<P> &L)
 {
-    __shared__ double s_rinv[P + 1], s_iinv[P + 1], s_rc[P + 1], s_ic[P + 1];
-    const int rank = r0 + blockIdx.x;
-    if (rank >= r1)
-        return;
-    const int lane = threadIdx.x;
+    constexpr int G = E2E_G<P>, NCH = 64 / G;
+    const int lane = tc_lane();
+    const int grp = lane / G, gl = lane % G;
     const int id = selfid[rank];
     const int cb = 1 + 4 * rank;
     const int l = level[id];
@@ -794,61 +803,75 @@ __global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int
             dim = exps[(size_t)2 * P * (cb + nz) + P + lane];
         }
     } else if (zcount < 3) {
-        for (int c = 0; c < 4; ++c) {
-            if (cw[c].x == 0)
-                continue;
-            const double x0 = ch[c].xcom - xcp, y0 = ch[c].ycom - ycp;
-            const double mass = ch[c].mass;
+        for (int c0 = 0; c0 < 4; c0 += NCH) {
+            // this lane's child (the index differs between lane groups, so it
+            // is loaded rather than picked out of ch[], which would go to scratch)
+            const int c = c0 + grp;
+            const TcNode chc = nodes[cb + c];
+            const bool act = nodew[cb + c].x != 0;
+            const double x0 = chc.xcom - xcp, y0 = chc.ycom - ycp;
+            const bool ident = x0 == 0 && y0 == 0;
             const double *ca = exps + (size_t)2 * P * (cb + c);
-            if (x0 == 0 && y0 == 0) {
-                if (lane < P) {
-                    dre += ca[lane];
-                    dim += ca[P + lane];
-                }
-                continue;
-            }
-            if (lane == 0) {
+            if (act && !ident && gl == 0) {
                 const double r2z0 = x0 * x0 + y0 * y0;
                 const double ri1 = x0 / r2z0, ii1 = -y0 / r2z0;
                 double ri = ri1, ii = ii1;
-                s_rinv[1] = ri;
-                s_iinv[1] = ii;
+                L.rinv[grp][1] = ri;
+                L.iinv[grp][1] = ii;
                 for (int j = 2; j <= P; ++j) {
                     const double rn = ri * ri1 - ii * ii1;
                     const double in = ri * ii1 + ii * ri1;
                     ri = rn;
                     ii = in;
-                    s_rinv[j] = ri;
-                    s_iinv[j] = ii;
+                    L.rinv[grp][j] = ri;
+                    L.iinv[grp][j] = ii;
                 }
             }
-            __syncthreads();
-            if (lane < P) {
-                const int j = lane + 1;
-                s_rc[j] = ca[j - 1] * s_rinv[j] - ca[P + j - 1] * s_iinv[j];
-                s_ic[j] = ca[j - 1] * s_iinv[j] + ca[P + j - 1] * s_rinv[j];
+            tc_wave_sync();
+            if (act && !ident && gl < P) {
+                const int jj = gl + 1;
+                L.rc[grp][jj] = ca[jj - 1] * L.rinv[grp][jj] - ca[P + jj - 1] * L.iinv[grp][jj];
+                L.ic[grp][jj] = ca[jj - 1] * L.iinv[grp][jj] + ca[P + jj - 1] * L.rinv[grp][jj];
             }
-            __syncthreads();
-            if (lane < P) {
-                const int ll = lane + 1;
-                double rt = (ll == 1) ? -mass : mass * (-(1.0 / (double)ll));
-                double it = s_ic[1];
-                rt = rt + s_rc[1];
-                for (int k = 2; k <= ll; ++k) {
-                    const double b = c_binom[(ll - 1) * 64 + (k - 1)];
-                    rt = rt + s_rc[k] * b;
-                    it = it + s_ic[k] * b;
+            tc_wave_sync();
+            if (gl < P) {
+                double cre = 0, cim = 0;
+                if (act && ident) {
+                    cre = ca[gl];
+                    cim = ca[P + gl];
+                } else if (act) {
+                    const int ll = gl + 1;
+                    const double mass = chc.mass;
+                    double rt = (ll == 1) ? -mass : mass * (-(1.0 / (double)ll));
+                    double it = L.ic[grp][1];
+                    rt = rt + L.rc[grp][1];
+                    for (int k = 2; k <= ll; ++k) {
+                        const double bk = c_binom[(ll - 1) * 64 + (k - 1)];
+                        rt = rt + L.rc[grp][k] * bk;
+                        it = it + L.ic[grp][k] * bk;
+                    }
+                    const double ri = L.rinv[grp][ll], ii = L.iinv[grp][ll];
+                    const double invz2 = ri * ri + ii * ii;
+                    const double invinv = invz2 != 0 ? 1 / invz2 : 0;
+                    const double rz = ri * invinv;
+                    const double iz = -ii * invinv;
+                    cre = rt * rz - it * iz;
+                    cim = rt * iz + it * rz;
                 }
-                const double ri = s_rinv[ll], ii = s_iinv[ll];
-                const double invz2 = ri * ri + ii * ii;
-                const double invinv = invz2 != 0 ? 1 / invz2 : 0;
-                const double rz = ri * invinv;
-                const double iz = -ii * invinv;
-                dre += rt * rz - it * iz;
-                dim += rt * iz + it * rz;
+                L.cre[c][gl] = cre;
+                L.cim[c][gl] = cim;
             }
-            __syncthreads();
+            tc_wave_sync();
         }
+        if (lane < P) {
+            for (int c = 0; c < 4; ++c) {
+                if (cw[c].x == 0)
+                    continue;
+                dre += L.cre[c][lane];
+                dim += L.cim[c][lane];
+            }
+        }
+        tc_wave_sync();
     }
     if (lane < P) {
         exps[(size_t)2 * P * id + lane] = dre;
@@ -870,6 +893,43 @@ __global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int
         nd.pad0 = nd.pad1 = 0;
         nodes[id] = nd;
         nodew[id] = make_double4(wsum, wxsum, wysum, 0.0);
+    }
+}
+
+// One wavefront per internal node of one level (ranks r0..r1-1).
+template <int P>
+__global__ __launch_bounds__(64) void k_e2e_level(const int *selfid, int r0, int r1, const int2 *se,
+                                                   const int *level, const int2 *cell, const double *box,
+                                                   TcNode *nodes, double4 *nodew, double *exps)
+{
+    __shared__ E2eLds<P> L;
+    const int rank = r0 + blockIdx.x;
+    if (rank >= r1)
+        return;
+    e2e_node<P>(rank, selfid, se, level, cell, box, nodes, nodew, exps, L);
+}
+
+// The small top levels in one launch: levels lt-1 .. 0 (ranks starts[L] ..
+// starts[L+1]-1, at most 4 nodes per wave each), E2E_TOP_WAVES waves striding
+// over a level's nodes, a workgroup barrier between levels (children before
+// parents).
+constexpr int E2E_TOP_WAVES = 16;
+template <int P>
+__global__ __launch_bounds__(64 * E2E_TOP_WAVES) void k_e2e_top(const int *selfid, const int *starts, int lt,
+                                                                 const int2 *se, const int *level,
+                                                                 const int2 *cell, const double *box,
+                                                                 TcNode *nodes, double4 *nodew, double *exps)
+{
+    __shared__ E2eLds<P> L[E2E_TOP_WAVES];
+    const int wid = threadIdx.x >> 6;
+    for (int lv = lt - 1; lv >= 0; --lv) {
+        const int r0 = starts[lv], r1 = starts[lv + 1];
+        for (int rank = r0 + wid; rank < r1; rank += E2E_TOP_WAVES)
+            e2e_node<P>(rank, selfid, se, level, cell, box, nodes, nodew, exps, L[wid]);
+        // the next level reads what this one wrote (other waves of the block)
+        __threadfence();
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
 }
 
@@ -1011,7 +1071,7 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
 
 template <int P>
 static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, const int *lev, const int2 *cell,
-                          const int *selfid, bool fp32, bool deriv, hipStream_t st)
+                          const int *selfid, const int *starts_d, bool fp32, bool deriv, hipStream_t st)
 {
     const double *box = t.box_d.as<double>();
     k_leaf_upward<P><<<tiles_of(t.nnodes, 4), 256, 0, st>>>(child, se, lev, cell, t.src.as<double4>(), box,
@@ -1019,13 +1079,23 @@ static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, cons
                                                              t.nodew.as<double4>(),
                                                              fp32 ? t.src32.as<float4>() : nullptr);
     TC_HIP(hipGetLastError());
-    for (int L = TC_LMAX - 1; L >= 0; --L) {
+    // levels whose node count is small run together in one launch
+    int lt = 0;
+    while (lt < TC_LMAX && t.level_start[lt + 1] - t.level_start[lt] <= 4 * E2E_TOP_WAVES)
+        ++lt;
+    for (int L = TC_LMAX - 1; L >= lt; --L) {
         const int r0 = t.level_start[L], r1 = t.level_start[L + 1];
         if (r1 > r0) {
             k_e2e_level<P><<<r1 - r0, 64, 0, st>>>(selfid, r0, r1, se, lev, cell, box, t.nodes.as<TcNode>(),
                                                     t.nodew.as<double4>(), t.exps.as<double>());
             TC_HIP(hipGetLastError());
         }
+    }
+    if (lt > 0 && t.level_start[lt] > 0) {
+        k_e2e_top<P><<<1, 64 * E2E_TOP_WAVES, 0, st>>>(selfid, starts_d, lt, se, lev, cell, box,
+                                                       t.nodes.as<TcNode>(), t.nodew.as<double4>(),
+                                                       t.exps.as<double>());
+        TC_HIP(hipGetLastError());
     }
     if (fp32) {
         constexpr int per_wave = P + 1 <= 16 ? 4 : (P + 1 <= 32 ? 2 : 1);
@@ -1120,7 +1190,7 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     const int *sid = t.selfid.as<int>();
     switch (order) {
 #define TC_CASE(P) \
-    case P: upward_launch<P>(t, child, se, lev, cell, sid, fp32, deriv32, st); break;
+    case P: upward_launch<P>(t, child, se, lev, cell, sid, info, fp32, deriv32, st); break;
         TC_ORDERS(TC_CASE)
 #undef TC_CASE
     }

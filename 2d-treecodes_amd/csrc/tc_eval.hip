@@ -194,6 +194,10 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     if (!live)
         return;
     double acc = 0;
+    // FP32 E2P results are summed in FP32 over 8 of the wave's evaluations, then
+    // flushed into the FP64 accumulator (each term is already an FP32 value)
+    float acc_e2p = 0.f;
+    int n_e2p = 0;
     unsigned long long np2p = 0, ne2p = 0, nslots = 0, neslots = 0;
     int sp = 0;
     s_node[wid][0] = 0;
@@ -213,11 +217,15 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         const bool acc_ok = in && (nd.r * nd.r < th2 * r2);
         const unsigned long long amask = tc_ballot(acc_ok);
         if (acc_ok) {
-            if (F32) {
-                acc += (double)e2p_pot32<P>((float)dx, (float)dy, exps32 + (size_t)(P + 1) * id);
-            } else {
+            if (F32)
+                acc_e2p += e2p_pot32<P>((float)dx, (float)dy, exps32 + (size_t)(P + 1) * id);
+            else
                 acc += e2p_pot64<P>(dx, dy, nd.mass, exps + (size_t)2 * P * id);
-            }
+        }
+        if (F32 && amask && ++n_e2p == 8) {
+            acc += (double)acc_e2p;
+            acc_e2p = 0.f;
+            n_e2p = 0;
         }
         ne2p += __popcll(amask);
         neslots += amask ? 64ull : 0ull;
@@ -353,7 +361,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         }
     }
     if (valid)
-        out[ti] = acc;
+        out[ti] = acc + (double)acc_e2p;
     if (counters && lane == 0) {
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
