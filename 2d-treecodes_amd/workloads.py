@@ -147,6 +147,23 @@ def gaussian16_force(n, nblocks=16384, seed=1455):
     return xs, ys, vs, np.concatenate(x0l), np.concatenate(y0l), np.concatenate(hl)
 
 
+def frame_targets(m, inner=0.25, outer=1.0, seed=1456):
+    """m points uniform in the square frame [-outer, 1+outer]^2 minus [-inner, 1+inner]^2."""
+    st = Stream(seed)
+    x = np.empty(m)
+    y = np.empty(m)
+    todo = np.arange(m)
+    while len(todo):
+        k = len(todo)
+        px = -outer + (1.0 + 2.0 * outer) * st.uniform(k)
+        py = -outer + (1.0 + 2.0 * outer) * st.uniform(k)
+        ok = ~((px > -inner) & (px < 1.0 + inner) & (py > -inner) & (py < 1.0 + inner))
+        x[todo[ok]] = px[ok]
+        y[todo[ok]] = py[ok]
+        todo = todo[~ok]
+    return x, y
+
+
 def request_stream(nreq=10000, rate_hz=2000.0, seed=1454, pot_frac=0.7, reuse_frac=0.25,
                    pot_n=(1e3, 1e6), force_n=(1e4, 2e6), force_nb=(16, 2048), thetas=(0.3, 0.5, 0.7)):
     """Request descriptors of the served stream (inputs are generated lazily from the seeds)."""

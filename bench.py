@@ -73,6 +73,21 @@ def make_inputs(cfg, n):
         xs, ys, vs, x0s, y0s, hs = workloads.gaussian16_force(n, nblocks=16384)
         return "force", dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs), \
             "force_mrag 16-Gaussian N=%d, 16384 blocks of 32x32, h in {1/1024,1/2048,1/4096}" % n
+    if cfg == "near":
+        # near-field probe: theta = 0 never accepts a node, so the traversal is P2P only
+        n = n or (1 << 17)
+        xs, ys, vs, xt, yt = workloads.mixed_plummer_potential(n)
+        return "potential", dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt), \
+            "near-field probe: potential half-uniform half-Plummer(a=0.02) N=M=%d, theta 0 (P2P only)" % n
+    if cfg == "far":
+        # far-field probe: targets in a frame around the source box, at least 1/20 of
+        # its side away, are served by multipoles only (no leaf is ever opened)
+        n = n or (1 << 24)
+        xs, ys, vs, _, _ = workloads.mixed_plummer_potential(n)
+        xt, yt = workloads.frame_targets(n, inner=0.05, outer=0.5, seed=1456)
+        return "potential", dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt), \
+            "far-field probe: potential N=M=%d, Plummer-mix sources in [0,1]^2, targets in the frame " \
+            "0.05-0.5 around it (E2P only)" % n
     raise SystemExit("unknown config " + cfg)
 
 
@@ -278,9 +293,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c5", "near", "far"])
     ap.add_argument("--n", type=int, default=0, help="override the number of sources")
-    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--theta", type=float, default=None, help="default 0.5 (0 for the near-field probe)")
     ap.add_argument("--order", type=int, default=0)
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
@@ -301,6 +316,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.theta is None:
+        args.theta = 0.0 if args.config == "near" else 0.5
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not supported (--gpus %d)" % args.gpus,
               file=sys.stderr)
