@@ -286,43 +286,58 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 }
                 acc += part * (double)LN2_HALF;
             } else if (F32) {
+                // Dense: sources staged 64 at a time in LDS.  Lanes l and l^32
+                // split each tile (even / odd sources) and each computes both
+                // targets, its own and its partner's: one broadcast LDS read
+                // feeds two pair evaluations, which halves the LDS return
+                // traffic (768 bytes per read and wave) that otherwise limits
+                // this loop.  Partner sums are exchanged at the end of the leaf.
                 nslots += 64ull * (unsigned long long)nsrc;
                 const float tx = (float)(x - nd.xcom), ty = (float)(y - nd.ycom);
+                const float ux = __shfl_xor(tx, 32, 64), uy = __shfl_xor(ty, 32, 64);
+                const int half = lane >> 5;
+                const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
                 float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
-                double leafsum = 0;
+                double own = 0, other = 0;
                 for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
                     const int cnt = min(64, nd.e - b0);
                     if (lane < cnt)
                         tile[lane] = src32[b0 + lane];
                     tc_wave_sync();
-                    if (act) {
+                    float s0 = 0, s1 = 0, o0 = 0, o1 = 0;
+                    int j = 0;
+                    for (; 2 * j + 3 < cnt; j += 2) {
 #pragma clang fp contract(fast)
-                        float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-                        int j = 0;
-                        for (; j + 4 <= cnt; j += 4) {
-                            const float4 q0 = tile[j], q1 = tile[j + 1], q2 = tile[j + 2], q3 = tile[j + 3];
-                            float ax = tx - q0.x, ay = ty - q0.y;
-                            s0 += q0.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
-                            ax = tx - q1.x;
-                            ay = ty - q1.y;
-                            s1 += q1.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
-                            ax = tx - q2.x;
-                            ay = ty - q2.y;
-                            s2 += q2.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
-                            ax = tx - q3.x;
-                            ay = ty - q3.y;
-                            s3 += q3.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
-                        }
-                        for (; j < cnt; ++j) {
-                            const float4 q = tile[j];
-                            const float ax = tx - q.x, ay = ty - q.y;
-                            s0 += q.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
-                        }
-                        leafsum += (double)((s0 + s1) + (s2 + s3));
+                        const float4 qa = tile[2 * j + half], qb = tile[2 * j + 2 + half];
+                        float ax = tx - qa.x, ay = ty - qa.y;
+                        s0 += qa.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                        ax = ux - qa.x;
+                        ay = uy - qa.y;
+                        o0 += qa.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                        ax = tx - qb.x;
+                        ay = ty - qb.y;
+                        s1 += qb.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                        ax = ux - qb.x;
+                        ay = uy - qb.y;
+                        o1 += qb.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
                     }
+                    for (; 2 * j < cnt; ++j) {
+#pragma clang fp contract(fast)
+                        const int idx = 2 * j + half;
+                        const float4 qa = idx < cnt ? tile[idx] : zero4;
+                        float ax = tx - qa.x, ay = ty - qa.y;
+                        s0 += qa.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                        ax = ux - qa.x;
+                        ay = uy - qa.y;
+                        o0 += qa.z * fast_log2(ax * ax + (ay * ay + EPS_REG32));
+                    }
+                    own += (double)(s0 + s1);
+                    other += (double)(o0 + o1);
                     tc_wave_sync();
                 }
-                acc += leafsum * (double)LN2_HALF;
+                const double leafsum = own + __shfl_xor(other, 32, 64);
+                if (act)
+                    acc += leafsum * (double)LN2_HALF;
             } else {
                 nslots += 64ull * (unsigned long long)nsrc;
                 double leafsum = 0, leafsum2 = 0;
