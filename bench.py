@@ -74,11 +74,14 @@ def make_inputs(cfg, n):
         return "force", dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs), \
             "force_mrag 16-Gaussian N=%d, 16384 blocks of 32x32, h in {1/1024,1/2048,1/4096}" % n
     if cfg == "near":
-        # near-field probe: theta = 0 never accepts a node, so the traversal is P2P only
-        n = n or (1 << 17)
-        xs, ys, vs, xt, yt = workloads.mixed_plummer_potential(n)
+        # near-field probe: theta = 0 never accepts a node, so the traversal is P2P
+        # only; 2^20 targets give every SIMD its full complement of waves
+        n = n or (1 << 16)
+        xs, ys, vs, _, _ = workloads.mixed_plummer_potential(n)
+        _, _, _, xt, yt = workloads.mixed_plummer_potential(1 << 20, seed=1457)
         return "potential", dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt), \
-            "near-field probe: potential half-uniform half-Plummer(a=0.02) N=M=%d, theta 0 (P2P only)" % n
+            "near-field probe: potential half-uniform half-Plummer(a=0.02) N=%d, M=%d, theta 0 (P2P only)" % (
+                n, len(xt))
     if cfg == "far":
         # far-field probe: targets in a frame around the source box, at least 1/20 of
         # its side away, are served by multipoles only (no leaf is ever opened)
