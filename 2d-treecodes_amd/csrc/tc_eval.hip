@@ -394,10 +394,12 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 // C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
 template <int P> constexpr int E2L_G = P <= 16 ? 16 : 32;
 template <int P> constexpr int E2L_NQ = 64 / E2L_G<P>;
+// row stride of the LDS binomial table C(n, k), k < P, n < P + E2L_G
+template <int P> constexpr int E2L_BW = P + E2L_G<P>;
 
 template <int P>
 TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const double4 *qrec, int nq, double x0b,
-                      double y0b, double (*s_binom)[64], const double *s_ninv, double2 *s_c, double &lre,
+                      double y0b, const double *s_binom, const double *s_ninv, double2 *s_c, double &lre,
                       double &lim)
 {
 #pragma clang fp contract(fast)
@@ -442,7 +444,7 @@ TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const doubl
         double it = 0;
         const double2 *c = s_c + h * (G + 1);
         for (int kk = 1; kk <= P; ++kk) {
-            const double bb = s_binom[kk - 1][L + kk - 1];
+            const double bb = s_binom[(kk - 1) * E2L_BW<P> + L + kk - 1];
             const double2 ck = c[kk];
             rt += bb * ck.x;
             it += bb * ck.y;
@@ -514,7 +516,7 @@ TC_DEV void force_push(const TcNode *__restrict__ nodes, int first, int count, c
 }
 
 template <int P, bool F32>
-__global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+__global__ __launch_bounds__(256, 6) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
                                                const double *__restrict__ y0s, const double *__restrict__ hs,
@@ -528,11 +530,11 @@ __global__ __launch_bounds__(256) void k_force(const TcNode *__restrict__ nodes,
     __shared__ double2 s_coef[4][68];  // E2L_NQ groups of E2L_G + 1 entries; L2P uses [1..P]
     __shared__ int s_e2lq[4][4];
     __shared__ double4 s_e2lrec[4][4];
-    __shared__ double s_binom[TC_MAXORDER][64];
+    __shared__ double s_binom[P * E2L_BW<P>];
     __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
-    for (int i = threadIdx.x; i < P * 64; i += 256)
-        s_binom[i >> 6][i & 63] = c_binomT[i];
+    for (int i = threadIdx.x; i < P * E2L_BW<P>; i += 256)
+        s_binom[i] = c_binomT[(i / E2L_BW<P>) * 64 + i % E2L_BW<P>];
     if (threadIdx.x <= TC_MAXORDER)
         s_ninv[threadIdx.x] = threadIdx.x ? -(1.0 / (double)threadIdx.x) : 0.0;
     __syncthreads();
