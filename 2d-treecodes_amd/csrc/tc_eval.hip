@@ -289,9 +289,10 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 // Dense: sources staged 64 at a time in LDS.  Lanes l and l^32
                 // split each tile (even / odd sources) and each computes both
                 // targets, its own and its partner's: one broadcast LDS read
-                // feeds two pair evaluations, which halves the LDS return
-                // traffic (768 bytes per read and wave) that otherwise limits
-                // this loop.  Partner sums are exchanged at the end of the leaf.
+                // (768 bytes of LDS return per wave) feeds two pair evaluations,
+                // which keeps the LDS well below the VALU's rate even when few
+                // waves are resident.  Partner sums are exchanged at the end of
+                // the leaf.
                 nslots += 64ull * (unsigned long long)nsrc;
                 const float tx = (float)(x - nd.xcom), ty = (float)(y - nd.ycom);
                 const float ux = __shfl_xor(tx, 32, 64), uy = __shfl_xor(ty, 32, 64);
