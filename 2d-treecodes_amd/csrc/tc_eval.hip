@@ -527,7 +527,7 @@ __global__ __launch_bounds__(256, 6) void k_force(const TcNode *__restrict__ nod
 {
     __shared__ int s_node[4][TC_STACK];
     __shared__ TcStackRec s_stk[4][TC_STACK];
-    __shared__ double4 s_src[4][64];
+    __shared__ double4 s_src[4][F32 ? 32 : 64];  // FP32 tiles are 64 float4
     __shared__ double2 s_coef[4][68];  // E2L_NQ groups of E2L_G + 1 entries; L2P uses [1..P]
     __shared__ int s_e2lq[4][4];
     __shared__ double4 s_e2lrec[4][4];
