@@ -1,16 +1,19 @@
 // tc_eval.hip -- theta-criterion traversals on gfx950.
 //
 // Potential (drivers/treecode-potential.cpp:25-86): one wavefront per 64
-// Morton-sorted targets.  The wave walks the tree once with a stack of
+// Hilbert-sorted targets.  The wave walks the tree once with a stack of
 // (node, lane mask) entries; every lane takes exactly the accept/open
 // decisions of its own per-target DFS, so each target sees the reference's
 // interaction list.  Node data and expansion coefficients are wave-uniform
-// (scalar loads); leaf sources are staged through LDS 64 at a time and
-// broadcast to the lanes.
+// (scalar loads).  At a leaf, sources are either staged through LDS 64 at a
+// time and broadcast to the lanes (dense), or, when few lanes need the leaf,
+// spread over the lanes with one reduction per active target (sparse, FP32).
 //
 // Force (drivers/treecode-force.cpp:94-199): one wavefront per 8x8 brick of a
-// 32x32 block, lane = target; decisions are brick-wide, so the DFS is
-// uniform.  E2L is wave-cooperative (lane l owns local coefficient l+1).
+// 32x32 block, lane = target, bricks dealt from a device counter to a
+// persistent grid.  Decisions are brick-wide, taken for four children at once
+// when a node is opened; E2L is wave-cooperative (lane l of a lane group owns
+// local coefficient l+1).
 //
 // precision 0 = plain FP64; 1 = FP32 bulk: P2P and E2P run in FP32 on
 // leaf-relative offsets with per-tile flushes into FP64 accumulators, while
@@ -173,6 +176,9 @@ TC_DEV void e2p_force32(float zr, float zi, const float2 *__restrict__ ex, float
     gy = -fmaf(ur, vi, ui * vr);
 }
 
+// EvaluatePotential::evaluate (treecode-potential.cpp:25-86) for targets
+// tperm[64 w .. 64 w + 63] on wave w: masked DFS with the accept test
+// r^2 < theta^2 d^2 per lane, E2P for accepting lanes, P2P at leaves.
 template <int P, bool F32>
 __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                    const float2 *__restrict__ exps32,
@@ -389,7 +395,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 // ------------------------------------------------------------------ force kernels
 // Multipole-to-local (downward_e2l, force-kernels.m4:102-155) for up to
 // E2L_NQ nodes at once: the wave splits into groups of E2L_G lanes (16 for
-// p <= 16, else 32), group g takes node q[g] and lane l of a group owns local
+// p <= 16, else 32), group g takes queued node g and lane l of a group owns local
 // coefficient l+1.  z0 = com - brick centre, u = 1/z0; each lane forms
 // u^(l+1) by binary powering, publishes (-1)^k a_k u^k, then sums
 // C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
