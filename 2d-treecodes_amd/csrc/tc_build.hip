@@ -933,12 +933,11 @@ __global__ __launch_bounds__(64 * E2E_TOP_WAVES) void k_e2e_top(const int *selfi
     }
 }
 
-// FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, 1e-30), so that
-// the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta (force trees: j b_j,
-// the derivative series' coefficients).  Since
-// |a_j| <= sum|v| r^j / j and s >= r, scaling the coefficient itself by 1/s
-// j times keeps every intermediate bounded (no overflow of s^-j for tiny
-// radii).  Entry p of a
+// FP32 mirror of the expansions: b_j = a_j / s^j with s = max(r, 1e-30), so
+// that the FP32 E2P evaluates sum b_j (s/z)^j with |s/z| < theta (force trees:
+// j b_j, the derivative series' coefficients).  Since |a_j| <= sum|v| r^j / j
+// and s >= r, scaling the coefficient itself by 1/s j times keeps every
+// intermediate bounded (no overflow of s^-j for tiny radii).  Entry p of a
 // node's p + 1 records holds the E2P constants (s, Q ln2/2) for potential
 // trees (the FP32 E2P works with log2) and (s, Q) for force trees.
 template <int P>
