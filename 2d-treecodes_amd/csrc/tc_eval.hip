@@ -394,20 +394,21 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 
 // ------------------------------------------------------------------ force kernels
 // Multipole-to-local (downward_e2l, force-kernels.m4:102-155) for up to
-// E2L_NQ nodes at once: the wave splits into groups of E2L_G lanes (16 for
-// p <= 16, else 32), group g takes queued node g and lane l of a group owns local
-// coefficient l+1.  z0 = com - brick centre, u = 1/z0; each lane forms
-// u^(l+1) by binary powering, publishes (-1)^k a_k u^k, then sums
-// C(l+k, k-1) (-1)^k a_k u^k over k and multiplies by u^(l+1).
+// E2L_NQ nodes at once, from coefficients and node records staged in LDS: the
+// wave splits into groups of E2L_G lanes (16 for p <= 16, else 32), group g
+// takes node g and lane l of a group owns local coefficient l+1.
+// z0 = com - brick centre, u = 1/z0; each lane forms u^(l+1) by binary
+// powering, publishes (-1)^k a_k u^k, then sums C(l+k, k-1) (-1)^k a_k u^k
+// over k and multiplies by u^(l+1).
 template <int P> constexpr int E2L_G = P <= 16 ? 16 : 32;
 template <int P> constexpr int E2L_NQ = 64 / E2L_G<P>;
 // row stride of the LDS binomial table C(n, k), k < P, n < P + E2L_G
 template <int P> constexpr int E2L_BW = P + E2L_G<P>;
 
+// rec[g] = {xcom, ycom, r, mass}; co[g * 2P + j] = Re a_(j+1), co[g * 2P + P + j] = Im a_(j+1)
 template <int P>
-TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const double4 *qrec, int nq, double x0b,
-                      double y0b, const double *s_binom, const double *s_ninv, double2 *s_c, double &lre,
-                      double &lim)
+TC_DEV void e2l_group(const double4 *rec, const double *co, int nq, double x0b, double y0b, const double *s_binom,
+                      const double *s_ninv, double2 *s_c, double &lre, double &lim)
 {
 #pragma clang fp contract(fast)
     constexpr int G = E2L_G<P>;
@@ -416,9 +417,8 @@ TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const doubl
     const bool on = h < nq && l < P;
     double ur = 0, ui = 0, pr = 1, pi = 0, mass = 0;
     if (h < nq) {
-        const int id = q[h];
-        const double4 nd = qrec[h];  // {xcom, ycom, mass, -}
-        mass = nd.z;
+        const double4 nd = rec[h];
+        mass = nd.w;
         const double zx = nd.x - x0b, zy = nd.y - y0b;
         const double inv = rcp64(zx * zx + zy * zy);
         ur = zx * inv;
@@ -438,8 +438,7 @@ TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const doubl
             br = sr;
         }
         if (on) {
-            const double *ex = exps + (size_t)2 * P * id;
-            const double ar = ex[l], ai = ex[P + l];
+            const double ar = co[h * 2 * P + l], ai = co[h * 2 * P + P + l];
             const double sgn = (l & 1) ? 1.0 : -1.0;  // (-1)^k, k = l + 1
             s_c[h * (G + 1) + l + 1] = make_double2(sgn * (ar * pr - ai * pi), sgn * (ar * pi + ai * pr));
         }
@@ -450,6 +449,7 @@ TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const doubl
         double rt = mass * s_ninv[L];  // -Q / L
         double it = 0;
         const double2 *c = s_c + h * (G + 1);
+#pragma unroll 2
         for (int kk = 1; kk <= P; ++kk) {
             const double bb = s_binom[(kk - 1) * E2L_BW<P> + L + kk - 1];
             const double2 ck = c[kk];
@@ -462,27 +462,41 @@ TC_DEV void e2l_group(const double *__restrict__ exps, const int *q, const doubl
     tc_wave_sync();
 }
 
-// Force stack records: the first 48 bytes of a node (centre, radius, mass,
-// source range, first child) plus the brick's decision for it, which lanes
-// 0-3 take for the four children at once when a node is opened (one node per
-// lane, brick-uniform inputs), so a pop only reads LDS and branches.
-enum TcAction { ACT_SKIP = 0, ACT_E2L = 1, ACT_E2P = 2, ACT_P2P = 3, ACT_OPEN = 4 };
-
-struct alignas(16) TcStackRec {
+// Node record of the force traversal: the first 48 bytes of a TcNode (centre,
+// radius, mass, source range, first child).
+struct alignas(16) TcForceRec {
     double xcom, ycom, r, mass;
-    int s, e, child, action;
+    int s, e, child, level;
 };
-static_assert(sizeof(TcStackRec) == 48 && offsetof(TcNode, level) == 44, "stack record is TcNode's first 48 B");
+static_assert(sizeof(TcForceRec) == 48 && offsetof(TcNode, level) == 44, "force record is TcNode's first 48 B");
+
+enum TcAction { ACT_SKIP = 0, ACT_E2L = 1, ACT_E2P = 2, ACT_P2P = 3, ACT_OPEN = 4 };
 
 struct TcBrick {
     double x0b, y0b, rbrick, xlo, xhi, ylo, yhi, invtheta, th2;
 };
 
+TC_DEV TcForceRec force_load(const TcNode *__restrict__ nodes, int id)
+{
+    const double2 *g = reinterpret_cast<const double2 *>(nodes + id);
+    const double2 a = g[0], b = g[1], c = g[2];
+    TcForceRec nd;
+    nd.xcom = a.x;
+    nd.ycom = a.y;
+    nd.r = b.x;
+    nd.mass = b.y;
+    nd.s = __double2loint(c.x);
+    nd.e = __double2hiint(c.x);
+    nd.child = __double2loint(c.y);
+    nd.level = __double2hiint(c.y);
+    return nd;
+}
+
 // The per-brick decisions of treecode-force.cpp:101-199 for one node: empty
 // nodes are skipped; E2L when (|c - brick centre| / r - 1) > 1/theta and the
 // brick radius <= r; E2P when r^2 < theta^2 d^2, d the distance from the node
 // centre to the brick's closest point; else P2P at a leaf, else open.
-TC_DEV int force_action(const TcStackRec &nd, const TcBrick &bk)
+TC_DEV int force_action(const TcForceRec &nd, const TcBrick &bk)
 {
     if (nd.s == nd.e)
         return ACT_SKIP;
@@ -499,31 +513,298 @@ TC_DEV int force_action(const TcStackRec &nd, const TcBrick &bk)
     return nd.child == 0 ? ACT_P2P : ACT_OPEN;
 }
 
-// Lanes 0..count-1 load node first+lane, classify it and write its stack
-// record and id at stk[lane], ids[lane].
-TC_DEV void force_push(const TcNode *__restrict__ nodes, int first, int count, const TcBrick &bk, TcStackRec *stk,
-                       int *ids)
+// Per-wave LDS of the force traversal.  The stack holds node ids; each step
+// pops up to 64 of them, one per lane, so one round trip to memory classifies
+// a whole frontier.  The E2L / E2P / P2P lists collect node ids and are
+// flushed in batches whose records, coefficients or sources are copied into
+// `buf` by global_load_lds (memory to LDS without passing through VGPRs).
+constexpr int FS_CAP = 256;                  // stack entries
+constexpr int FS_RESERVE = 3 * TC_LMAX + 4;  // growth bound of a one-node-at-a-time DFS
+constexpr int FQ_CAP = 128;                  // entries per work list
+constexpr int FBUF_BYTES = 2048;             // staging buffer
+
+template <int P, bool F32> struct ForceCfg {
+    // E2P batch: node records (32 B) + coefficients per node
+    static constexpr int E2P_BYTES = 32 + (F32 ? 8 * (P + 1) : 16 * P);
+    static constexpr int E2P_QB = FBUF_BYTES / E2P_BYTES;
+    // E2L batch: node records (32 B) + 2p coefficients per node, whole groups
+    static constexpr int E2L_BYTES = 32 + 16 * P;
+    static constexpr int E2L_QB = FBUF_BYTES / E2L_BYTES / E2L_NQ<P> * E2L_NQ<P>;
+    // P2P tiles: two 1 KiB halves of buf, 64 FP32 sources (float4) or 32
+    // FP64 sources (x,y and v,- halves of the double4 in separate 512 B rows)
+    static constexpr int TILE_N = F32 ? 64 : 32;
+    static_assert(E2P_QB >= 1 && E2L_QB >= E2L_NQ<P>, "staging buffer too small for this order");
+};
+
+typedef __attribute__((address_space(3))) void tc_lds_void;
+typedef __attribute__((address_space(1))) const void tc_glb_void;
+
+// global_load_lds: lane i copies BYTES (4 or 16) from gsrc to lds + i * BYTES.
+// Lanes that are switched off copy nothing.  The copy is counted by vmcnt.
+template <int BYTES> TC_DEV void lds_dma(const void *gsrc, void *lds)
+{
+    static_assert(BYTES == 4 || BYTES == 16, "global_load_lds copies 4 or 16 bytes per lane here");
+    if constexpr (BYTES == 16)
+        __builtin_amdgcn_global_load_lds((tc_glb_void *)gsrc, (tc_lds_void *)lds, 16, 0, 0);
+    else
+        __builtin_amdgcn_global_load_lds((tc_glb_void *)gsrc, (tc_lds_void *)lds, 4, 0, 0);
+}
+
+// s_waitcnt vmcnt(0): every vector memory operation of this wave, the LDS
+// copies included, has completed.
+TC_DEV void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Node records {xcom, ycom, r, mass} of list[b0 .. b0 + cnt) into rec[0 .. cnt).
+TC_DEV void stage_records(const int *list, int b0, int cnt, const TcNode *__restrict__ nodes, double4 *rec)
 {
     const int lane = tc_lane();
-    if (lane < count) {
-        const double2 *g = reinterpret_cast<const double2 *>(nodes + first + lane);
-        const double2 a = g[0], b = g[1], c = g[2];
-        TcStackRec nd;
-        nd.xcom = a.x;
-        nd.ycom = a.y;
-        nd.r = b.x;
-        nd.mass = b.y;
-        nd.s = __double2loint(c.x);
-        nd.e = __double2hiint(c.x);
-        nd.child = __double2loint(c.y);
-        nd.action = force_action(nd, bk);
-        stk[lane] = nd;
-        ids[lane] = first + lane;
+    if (lane < 2 * cnt)
+        lds_dma<16>(reinterpret_cast<const char *>(nodes + list[b0 + (lane >> 1)]) + 16 * (lane & 1), rec);
+    if (64 + lane < 2 * cnt)
+        lds_dma<16>(reinterpret_cast<const char *>(nodes + list[b0 + 32 + (lane >> 1)]) + 16 * (lane & 1),
+                    rec + 32);
+}
+
+// Rows of ROW bytes (a multiple of CH, CH = 4 or 16) at base + ROW * list[b0 + q]
+// for q < cnt, packed into dst.
+template <int ROW, int CH, int QB>
+TC_DEV void stage_rows(const int *list, int b0, int cnt, const void *base, void *dst)
+{
+    constexpr int NCH = ROW / CH;
+    constexpr int IT = (QB * NCH + 63) / 64;
+    static_assert(ROW % CH == 0, "row is not a whole number of chunks");
+    const int lane = tc_lane();
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+        const int i = lane + 64 * u, q = i / NCH, c = i - q * NCH;
+        if (i < cnt * NCH)
+            lds_dma<CH>(static_cast<const char *>(base) + (size_t)ROW * list[b0 + q] + CH * c,
+                        static_cast<char *>(dst) + 64 * CH * u);
     }
 }
 
+// E2P of the listed nodes at the brick's 64 targets (force_e2p_8x8,
+// force-kernels.m4:57-98), E2P_QB nodes per staged batch.
 template <int P, bool F32>
-__global__ __launch_bounds__(256, 6) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+TC_DEV void force_flush_e2p(const int *list, int n, const TcNode *__restrict__ nodes,
+                            const double *__restrict__ exps, const float2 *__restrict__ exps32, double4 *buf,
+                            double xb, double yb, int ix, int iy, double h, double &fx, double &fy)
+{
+    constexpr int QB = ForceCfg<P, F32>::E2P_QB;
+    double4 *rec = buf;
+    void *co = rec + QB;
+    for (int b0 = 0; b0 < n; b0 += QB) {
+        const int cnt = min(QB, n - b0);
+        stage_records(list, b0, cnt, nodes, rec);
+        if (F32)
+            stage_rows<8 * (P + 1), 4, QB>(list, b0, cnt, exps32, co);
+        else
+            stage_rows<16 * P, 16, QB>(list, b0, cnt, exps, co);
+        vm_wait_all();
+        tc_wave_sync();
+        for (int t = 0; t < cnt; ++t) {
+            const double4 r = rec[t];
+            const double rz = (xb - r.x) + ix * h, iz = (yb - r.y) + iy * h;
+            if (F32) {
+                float gx, gy;
+                e2p_force32<P>((float)rz, (float)iz, static_cast<const float2 *>(co) + t * (P + 1), gx, gy);
+                fx += (double)gx;
+                fy += (double)gy;
+            } else {
+                e2p_force64<P>(rz, iz, r.w, static_cast<const double *>(co) + t * 2 * P, fx, fy);
+            }
+        }
+        tc_wave_sync();
+    }
+}
+
+// E2L of the listed nodes about the brick centre (downward_e2l), E2L_QB
+// nodes per staged batch, E2L_NQ per e2l_group call.
+template <int P, bool F32>
+TC_DEV void force_flush_e2l(const int *list, int n, const TcNode *__restrict__ nodes,
+                            const double *__restrict__ exps, double4 *buf, double x0b, double y0b,
+                            const double *s_binom, const double *s_ninv, double2 *s_c, double &lre, double &lim)
+{
+    constexpr int QB = ForceCfg<P, F32>::E2L_QB;
+    double4 *rec = buf;
+    double *co = reinterpret_cast<double *>(rec + QB);
+    for (int b0 = 0; b0 < n; b0 += QB) {
+        const int cnt = min(QB, n - b0);
+        stage_records(list, b0, cnt, nodes, rec);
+        stage_rows<16 * P, 16, QB>(list, b0, cnt, exps, co);
+        vm_wait_all();
+        tc_wave_sync();
+        for (int g0 = 0; g0 < cnt; g0 += E2L_NQ<P>)
+            e2l_group<P>(rec + g0, co + g0 * 2 * P, min(E2L_NQ<P>, cnt - g0), x0b, y0b, s_binom, s_ninv, s_c, lre,
+                         lim);
+    }
+}
+
+TC_DEV double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Copy the P2P tile [b0, min(b0 + TILE_N, ne)) into an LDS half (see ForceCfg).
+template <bool F32>
+TC_DEV void p2p_tile_load(const double4 *__restrict__ src, const float4 *__restrict__ src32, int b0, int ne,
+                          void *dst)
+{
+    const int lane = tc_lane();
+    if (F32) {
+        if (b0 + lane < ne)
+            lds_dma<16>(src32 + b0 + lane, dst);
+    } else {
+        if (b0 + (lane & 31) < ne)
+            lds_dma<16>(reinterpret_cast<const char *>(src + b0 + (lane & 31)) + 16 * (lane >> 5), dst);
+    }
+}
+
+// One P2P tile of cnt sources from `cur` while the next tile (if `more`) is
+// copied into `nxt`.  The two halves never alias, which the restrict
+// qualifiers tell the compiler, so it does not wait for the copy before
+// reading `cur`.
+template <bool F32>
+TC_DEV void p2p_tile_sum(const void *__restrict__ cur, void *__restrict__ nxt, bool more, int nb0, int nne,
+                         const double4 *__restrict__ src, const float4 *__restrict__ src32, int cnt, double xt,
+                         double yt, double xcom, double ycom, double &fx, double &fy)
+{
+    if (more)
+        p2p_tile_load<F32>(src, src32, nb0, nne, nxt);
+    if (F32) {
+        const float tx = (float)(xt - xcom), ty = (float)(yt - ycom);
+        const float4 *tile = static_cast<const float4 *>(cur);
+        float ax0 = 0.f, ay0 = 0.f, ax1 = 0.f, ay1 = 0.f;
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {
+#pragma clang fp contract(fast)
+            const float4 q0 = tile[j], q1 = tile[j + 1], q2 = tile[j + 2], q3 = tile[j + 3];
+            const float dx0 = tx - q0.x, dy0 = ty - q0.y;
+            const float dx1 = tx - q1.x, dy1 = ty - q1.y;
+            const float dx2 = tx - q2.x, dy2 = ty - q2.y;
+            const float dx3 = tx - q3.x, dy3 = ty - q3.y;
+            const float f0 = q0.z * fast_rcp(dx0 * dx0 + (dy0 * dy0 + EPS_REG32));
+            const float f1 = q1.z * fast_rcp(dx1 * dx1 + (dy1 * dy1 + EPS_REG32));
+            const float f2 = q2.z * fast_rcp(dx2 * dx2 + (dy2 * dy2 + EPS_REG32));
+            const float f3 = q3.z * fast_rcp(dx3 * dx3 + (dy3 * dy3 + EPS_REG32));
+            ax0 += dx0 * f0;
+            ay0 += dy0 * f0;
+            ax1 += dx1 * f1;
+            ay1 += dy1 * f1;
+            ax0 += dx2 * f2;
+            ay0 += dy2 * f2;
+            ax1 += dx3 * f3;
+            ay1 += dy3 * f3;
+        }
+        for (; j < cnt; ++j) {
+#pragma clang fp contract(fast)
+            const float4 q = tile[j];
+            const float dx = tx - q.x, dy = ty - q.y;
+            const float f = q.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            ax0 += dx * f;
+            ay0 += dy * f;
+        }
+        fx += (double)(ax0 + ax1);
+        fy += (double)(ay0 + ay1);
+    } else {
+        const double2 *xy = static_cast<const double2 *>(cur);
+        const double2 *vv = xy + 32;
+        double gx = 0, gy = 0;
+        int j = 0;
+        for (; j + 2 <= cnt; j += 2) {
+#pragma clang fp contract(fast)
+            const double2 p0 = xy[j], p1 = xy[j + 1];
+            const double v0 = vv[j].x, v1 = vv[j + 1].x;
+            const double dx0 = xt - p0.x, dy0 = yt - p0.y;
+            const double dx1 = xt - p1.x, dy1 = yt - p1.y;
+            const double f0 = v0 * rcp64(dx0 * dx0 + (dy0 * dy0 + EPS_REG));
+            const double f1 = v1 * rcp64(dx1 * dx1 + (dy1 * dy1 + EPS_REG));
+            fx += dx0 * f0;
+            fy += dy0 * f0;
+            gx += dx1 * f1;
+            gy += dy1 * f1;
+        }
+        if (j < cnt) {
+#pragma clang fp contract(fast)
+            const double2 p = xy[j];
+            const double v = vv[j].x;
+            const double dx = xt - p.x, dy = yt - p.y;
+            const double f = v * rcp64(dx * dx + (dy * dy + EPS_REG));
+            fx += dx * f;
+            fy += dy * f;
+        }
+        fx += gx;
+        fy += gy;
+    }
+}
+
+// P2P of the listed leaves at the brick's 64 targets (force_p2p_8x8,
+// force-kernels.m4:15-55).  Sources pass through the two LDS halves of buf in
+// tiles of TILE_N: the copy of the next tile (of this leaf or the next one)
+// runs while the current one is summed.
+template <int P, bool F32>
+TC_DEV void force_flush_p2p(const int *list, int n, const TcNode *__restrict__ nodes,
+                            const double4 *__restrict__ src, const float4 *__restrict__ src32, double4 *buf,
+                            double xt, double yt, double &fx, double &fy, unsigned long long &np2p)
+{
+    constexpr int T = ForceCfg<P, F32>::TILE_N;
+    const int lane = tc_lane();
+    char *const base = reinterpret_cast<char *>(buf);
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int cn = min(64, n - c0);
+        double lx = 0, ly = 0;
+        int ls = 0, le = 0;
+        if (lane < cn) {
+            const TcForceRec nd = force_load(nodes, list[c0 + lane]);
+            lx = nd.xcom;
+            ly = nd.ycom;
+            ls = nd.s;
+            le = nd.e;
+        }
+        int t = 0;
+        int ns = __builtin_amdgcn_readlane(ls, 0), ne = __builtin_amdgcn_readlane(le, 0);
+        double xc = readlane_f64(lx, 0), yc = readlane_f64(ly, 0);
+        int b0 = ns, ph = 0;
+        np2p += 64ull * (unsigned long long)(ne - ns);
+        p2p_tile_load<F32>(src, src32, b0, ne, base);
+        while (t < cn) {
+            const int cnt = min(T, ne - b0);
+            const double xcc = xc, ycc = yc;
+            b0 += T;
+            if (b0 >= ne) {
+                ++t;
+                if (t < cn) {
+                    ns = __builtin_amdgcn_readlane(ls, t);
+                    ne = __builtin_amdgcn_readlane(le, t);
+                    xc = readlane_f64(lx, t);
+                    yc = readlane_f64(ly, t);
+                    b0 = ns;
+                    np2p += 64ull * (unsigned long long)(ne - ns);
+                }
+            }
+            vm_wait_all();  // the current tile has landed
+            tc_wave_sync();
+            p2p_tile_sum<F32>(base + ph * (FBUF_BYTES / 2), base + (ph ^ 1) * (FBUF_BYTES / 2), t < cn, b0, ne, src,
+                              src32, cnt, xt, yt, xcc, ycc, fx, fy);
+            tc_wave_sync();
+            ph ^= 1;
+        }
+    }
+}
+
+// EvaluateForce::evaluate (treecode-force.cpp:94-199) with one wavefront per
+// 8x8 brick (lane = target).  Bricks are dealt out one at a time from a device
+// counter and each wave runs until the queue is empty: brick costs differ by
+// orders of magnitude across block sizes, and a workgroup's LDS is only
+// released when all four of its waves are done, so fixed brick-per-wave
+// launches left SIMDs idle.  The brick's interaction lists are the
+// reference's (same decisions per node); they are evaluated in list order
+// rather than depth-first order.
+template <int P, bool F32>
+__global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
                                                const double *__restrict__ y0s, const double *__restrict__ hs,
@@ -531,12 +812,10 @@ __global__ __launch_bounds__(256, 6) void k_force(const TcNode *__restrict__ nod
                                                double *__restrict__ yres, unsigned long long *counters,
                                                uint32_t *__restrict__ work)
 {
-    __shared__ int s_node[4][TC_STACK];
-    __shared__ TcStackRec s_stk[4][TC_STACK];
-    __shared__ double4 s_src[4][F32 ? 32 : 64];  // FP32 tiles are 64 float4
+    __shared__ int s_stack[4][FS_CAP];
+    __shared__ int s_list[4][3][FQ_CAP];  // E2L, E2P, P2P node ids
+    __shared__ double4 s_buf[4][FBUF_BYTES / 32];
     __shared__ double2 s_coef[4][68];  // E2L_NQ groups of E2L_G + 1 entries; L2P uses [1..P]
-    __shared__ int s_e2lq[4][4];
-    __shared__ double4 s_e2lrec[4][4];
     __shared__ double s_binom[P * E2L_BW<P>];
     __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
@@ -547,11 +826,9 @@ __global__ __launch_bounds__(256, 6) void k_force(const TcNode *__restrict__ nod
     __syncthreads();
 
     const int wid = threadIdx.x >> 6, lane = tc_lane();
+    int *stack = s_stack[wid];
+    int *lst_l = s_list[wid][0], *lst_e = s_list[wid][1], *lst_p = s_list[wid][2];
     unsigned long long np2p = 0, ne2p = 0, ne2l = 0, nl2p = 0;
-    // Bricks are dealt out one at a time from a device counter and each wave
-    // runs until the queue is empty: brick costs differ by orders of magnitude
-    // across block sizes, and a workgroup's LDS is only released when all four
-    // of its waves are done, so fixed brick-per-wave launches left SIMDs idle.
     for (;;) {
         uint32_t grab = 0;
         if (lane == 0)
@@ -563,158 +840,92 @@ __global__ __launch_bounds__(256, 6) void k_force(const TcNode *__restrict__ nod
         const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
         const double x0 = x0s[b], y0 = y0s[b], h = hs[b];
         const int ix = lane & 7, iy = lane >> 3;
-        // targets as in force_p2p_8x8 (force-kernels.m4:28-32)
-        const double xb = x0 + (bx + 0) * h, yb = y0 + (by + 0) * h;
+        // targets as in force_p2p_8x8 (force-kernels.m4:28-32); the brick-wide
+        // values are moved to scalar registers
+        const double xb = tc_uniform(x0 + (bx + 0) * h), yb = tc_uniform(y0 + (by + 0) * h);
         const double xt = xb + ix * h, yt = yb + iy * h;
         // brick geometry (treecode-force.cpp:108,113-114,146-147)
         TcBrick bk;
-        bk.x0b = x0 + h * (bx + 0.5 * (TILE - 1));
-        bk.y0b = y0 + h * (by + 0.5 * (TILE - 1));
-        bk.rbrick = 1.4142135623730951 * h * (TILE - 1) * 0.5;
-        bk.xlo = x0 + bx * h;
-        bk.xhi = x0 + (bx + TILE - 1) * h;
-        bk.ylo = y0 + by * h;
-        bk.yhi = y0 + (by + TILE - 1) * h;
-        bk.invtheta = 1 / theta;
-        bk.th2 = theta * theta;
+        bk.x0b = tc_uniform(x0 + h * (bx + 0.5 * (TILE - 1)));
+        bk.y0b = tc_uniform(y0 + h * (by + 0.5 * (TILE - 1)));
+        bk.rbrick = tc_uniform(1.4142135623730951 * h * (TILE - 1) * 0.5);
+        bk.xlo = tc_uniform(x0 + bx * h);
+        bk.xhi = tc_uniform(x0 + (bx + TILE - 1) * h);
+        bk.ylo = tc_uniform(y0 + by * h);
+        bk.yhi = tc_uniform(y0 + (by + TILE - 1) * h);
+        bk.invtheta = tc_uniform(1 / theta);
+        bk.th2 = tc_uniform(theta * theta);
 
         double fx = 0, fy = 0;
         double lre = 0, lim = 0;
-        int nq = 0;
-
-        int sp = 1;
-        force_push(nodes, 0, 1, bk, &s_stk[wid][0], &s_node[wid][0]);
+        int sp = 1, nl = 0, ne = 0, np = 0;
+        if (lane == 0)
+            stack[0] = 0;
+        tc_wave_sync();
         while (sp > 0) {
-            --sp;
+            // Pop k ids.  Opening all of them pushes at most 3k more, so k is
+            // capped to keep FS_RESERVE entries free; with one pop per step the
+            // stack grows by at most 3 per level below the popped node, which
+            // the reserve covers, so the stack never overflows.
+            int k = (FS_CAP - FS_RESERVE - sp) / 3;
+            k = k < 1 ? 1 : (k > 64 ? 64 : k);
+            k = k < sp ? k : sp;
+            sp -= k;
+            int act = ACT_SKIP, id = 0, child = 0;
+            if (lane < k) {
+                id = stack[sp + lane];
+                const TcForceRec nd = force_load(nodes, id);
+                act = force_action(nd, bk);
+                child = nd.child;
+            }
+            const unsigned long long lt = tc_lanemask_lt();
+            const unsigned long long mo = tc_ballot(act == ACT_OPEN), ml = tc_ballot(act == ACT_E2L);
+            const unsigned long long me = tc_ballot(act == ACT_E2P), mp = tc_ballot(act == ACT_P2P);
             tc_wave_sync();
-            const int action = tc_uniform(s_stk[wid][sp].action);
-            if (action == ACT_SKIP)
-                continue;
-            const int id = tc_uniform(s_node[wid][sp]);
-            if (action == ACT_E2L) {
-                // queued; downward_e2l runs on E2L_NQ nodes at a time (e2l_group)
-                ++ne2l;
-                if (lane == 0) {
-                    const TcStackRec &q = s_stk[wid][sp];
-                    s_e2lq[wid][nq] = id;
-                    s_e2lrec[wid][nq] = make_double4(q.xcom, q.ycom, q.mass, 0.0);
-                }
-                ++nq;
-                if (nq == E2L_NQ<P>) {
-                    tc_wave_sync();
-                    e2l_group<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid],
-                                 lre, lim);
-                    nq = 0;
-                }
-                continue;
+            if (act == ACT_OPEN) {
+                const int pos = sp + 4 * __popcll(mo & lt);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    stack[pos + c] = child + c;
+            } else if (act == ACT_E2L) {
+                lst_l[nl + __popcll(ml & lt)] = id;
+            } else if (act == ACT_E2P) {
+                lst_e[ne + __popcll(me & lt)] = id;
+            } else if (act == ACT_P2P) {
+                lst_p[np + __popcll(mp & lt)] = id;
             }
-            const TcStackRec &rec = s_stk[wid][sp];
-            const double xcom = rec.xcom, ycom = rec.ycom;
-            if (action == ACT_E2P) {
-                // force_e2p_8x8 (force-kernels.m4:57-98)
-                ne2p += 64;
-                const double mass = rec.mass;
-                const double rz = (xb - xcom) + ix * h, iz = (yb - ycom) + iy * h;
-                if (F32) {
-                    float gx, gy;
-                    e2p_force32<P>((float)rz, (float)iz, exps32 + (size_t)(P + 1) * id, gx, gy);
-                    fx += (double)gx;
-                    fy += (double)gy;
-                } else {
-                    e2p_force64<P>(rz, iz, mass, exps + (size_t)2 * P * id, fx, fy);
-                }
-                continue;
+            sp += 4 * __popcll(mo);
+            nl += __popcll(ml);
+            ne += __popcll(me);
+            np += __popcll(mp);
+            tc_wave_sync();
+            // a list that might not take another 64 ids is flushed now
+            if (nl > FQ_CAP - 64) {
+                ne2l += nl;
+                force_flush_e2l<P, F32>(lst_l, nl, nodes, exps, s_buf[wid], bk.x0b, bk.y0b, s_binom, s_ninv,
+                                        s_coef[wid], lre, lim);
+                nl = 0;
             }
-            if (action == ACT_P2P) {
-                // force_p2p_8x8 (force-kernels.m4:15-55)
-                const int ns = tc_uniform(rec.s), ne = tc_uniform(rec.e);
-                np2p += 64ull * (unsigned long long)(ne - ns);
-                if (F32) {
-                    const float tx = (float)(xt - xcom), ty = (float)(yt - ycom);
-                    float4 *tile = reinterpret_cast<float4 *>(&s_src[wid][0]);
-                    for (int b0 = ns; b0 < ne; b0 += 64) {
-                        const int cnt = min(64, ne - b0);
-                        if (lane < cnt)
-                            tile[lane] = src32[b0 + lane];
-                        tc_wave_sync();
-                        float ax0 = 0.f, ay0 = 0.f, ax1 = 0.f, ay1 = 0.f;
-                        int j = 0;
-                        for (; j + 4 <= cnt; j += 4) {
-#pragma clang fp contract(fast)
-                            const float4 q0 = tile[j], q1 = tile[j + 1], q2 = tile[j + 2], q3 = tile[j + 3];
-                            const float dx0 = tx - q0.x, dy0 = ty - q0.y;
-                            const float dx1 = tx - q1.x, dy1 = ty - q1.y;
-                            const float dx2 = tx - q2.x, dy2 = ty - q2.y;
-                            const float dx3 = tx - q3.x, dy3 = ty - q3.y;
-                            const float f0 = q0.z * fast_rcp(dx0 * dx0 + (dy0 * dy0 + EPS_REG32));
-                            const float f1 = q1.z * fast_rcp(dx1 * dx1 + (dy1 * dy1 + EPS_REG32));
-                            const float f2 = q2.z * fast_rcp(dx2 * dx2 + (dy2 * dy2 + EPS_REG32));
-                            const float f3 = q3.z * fast_rcp(dx3 * dx3 + (dy3 * dy3 + EPS_REG32));
-                            ax0 += dx0 * f0;
-                            ay0 += dy0 * f0;
-                            ax1 += dx1 * f1;
-                            ay1 += dy1 * f1;
-                            ax0 += dx2 * f2;
-                            ay0 += dy2 * f2;
-                            ax1 += dx3 * f3;
-                            ay1 += dy3 * f3;
-                        }
-                        for (; j < cnt; ++j) {
-#pragma clang fp contract(fast)
-                            const float4 q = tile[j];
-                            const float dx = tx - q.x, dy = ty - q.y;
-                            const float f = q.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
-                            ax0 += dx * f;
-                            ay0 += dy * f;
-                        }
-                        fx += (double)(ax0 + ax1);
-                        fy += (double)(ay0 + ay1);
-                        tc_wave_sync();
-                    }
-                } else {
-                    for (int b0 = ns; b0 < ne; b0 += 64) {
-                        const int cnt = min(64, ne - b0);
-                        if (lane < cnt)
-                            s_src[wid][lane] = src[b0 + lane];
-                        tc_wave_sync();
-                        double gx = 0, gy = 0;
-                        int j = 0;
-                        for (; j + 2 <= cnt; j += 2) {
-#pragma clang fp contract(fast)
-                            const double4 q0 = s_src[wid][j], q1 = s_src[wid][j + 1];
-                            const double dx0 = xt - q0.x, dy0 = yt - q0.y;
-                            const double dx1 = xt - q1.x, dy1 = yt - q1.y;
-                            const double f0 = q0.z * rcp64(dx0 * dx0 + (dy0 * dy0 + EPS_REG));
-                            const double f1 = q1.z * rcp64(dx1 * dx1 + (dy1 * dy1 + EPS_REG));
-                            fx += dx0 * f0;
-                            fy += dy0 * f0;
-                            gx += dx1 * f1;
-                            gy += dy1 * f1;
-                        }
-                        if (j < cnt) {
-#pragma clang fp contract(fast)
-                            const double4 q = s_src[wid][j];
-                            const double dx = xt - q.x, dy = yt - q.y;
-                            const double f = q.z * rcp64(dx * dx + (dy * dy + EPS_REG));
-                            fx += dx * f;
-                            fy += dy * f;
-                        }
-                        fx += gx;
-                        fy += gy;
-                        tc_wave_sync();
-                    }
-                }
-                continue;
+            if (ne > FQ_CAP - 64) {
+                ne2p += 64ull * ne;
+                force_flush_e2p<P, F32>(lst_e, ne, nodes, exps, exps32, s_buf[wid], xb, yb, ix, iy, h, fx, fy);
+                ne = 0;
             }
-            // open: children pushed in order 0..3, popped 3..0
-            force_push(nodes, tc_uniform(rec.child), 4, bk, &s_stk[wid][sp], &s_node[wid][sp]);
-            sp += 4;
+            if (np > FQ_CAP - 64) {
+                force_flush_p2p<P, F32>(lst_p, np, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+                np = 0;
+            }
         }
-
-        if (nq) {
-            tc_wave_sync();
-            e2l_group<P>(exps, s_e2lq[wid], s_e2lrec[wid], nq, bk.x0b, bk.y0b, s_binom, s_ninv, s_coef[wid], lre,
-                         lim);
+        if (np)
+            force_flush_p2p<P, F32>(lst_p, np, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+        if (ne) {
+            ne2p += 64ull * ne;
+            force_flush_e2p<P, F32>(lst_e, ne, nodes, exps, exps32, s_buf[wid], xb, yb, ix, iy, h, fx, fy);
+        }
+        if (nl) {
+            ne2l += nl;
+            force_flush_e2l<P, F32>(lst_l, nl, nodes, exps, s_buf[wid], bk.x0b, bk.y0b, s_binom, s_ninv,
+                                    s_coef[wid], lre, lim);
         }
         // fold the groups' partial local expansions into lanes 0..E2L_G-1
 #pragma unroll
