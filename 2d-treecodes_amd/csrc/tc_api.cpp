@@ -26,7 +26,7 @@ enum TcKind { TC_KIND_NONE = 0, TC_KIND_POTENTIAL = 1, TC_KIND_FORCE = 2 };
 struct tc_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    int precision = 0, pot_order = 12, pot_leaf = 32 * 16, force_order = 24, force_leaf = 192;
+    int precision = 0, pot_order = 12, pot_leaf = 32 * 16, force_order = 24, force_leaf = 192, force_chunks = 0;
     bool profile = false, count = false;
     int tree_kind = TC_KIND_NONE;
     int tree_precision = 0;
@@ -337,9 +337,9 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
         if (rebuild)
             build(c, dxs, dys, dvs, n, TC_KIND_FORCE);
         record(c, 1);
-        c->ws.work.reserve(256);
-        tc_eval_force(c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws.work.as<uint32_t>(),
-                      c->stream, c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+        tc_eval_force(c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws, c->force_chunks,
+                      c->stream,
+                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
         c->eval_has_tsort = false;
         if (!device_ptrs) {
@@ -526,6 +526,10 @@ TC_API int tc_set_option(tc_context *c, int key, int value)
         return 0;
     case TC_OPT_PROFILE:
         c->profile = value != 0;
+        return 0;
+    case TC_OPT_FORCE_CHUNKS:
+        if (value < 0) break;
+        c->force_chunks = value;
         return 0;
     case TC_OPT_COUNTERS:
         c->count = value != 0;

@@ -205,11 +205,12 @@ struct TcTree {
 
 // Scratch reused across calls of one context.
 struct TcScratch {
-    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints, packed, work;
+    DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints, packed, work,
+        flist;
     void release()
     {
         for (DevBuf *b : {&k2, &v2, &hist, &part, &scan_tmp, &istart, &ilevel, &info, &tkeys, &tperm, &tsorted,
-                          &bbox_part, &ints, &packed, &work})
+                          &bbox_part, &ints, &packed, &work, &flist})
             b->release();
     }
 };
@@ -232,7 +233,7 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
                        hipEvent_t ev0, hipEvent_t ev1);
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
-                   uint32_t *work, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+                   TcScratch &ws, int max_chunks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 void tc_eval_force_targets(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *xr,
                            double *yr, int precision, unsigned long long *counters, TcScratch &ws,
                            hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);

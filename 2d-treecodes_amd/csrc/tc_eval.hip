@@ -530,10 +530,33 @@ template <int P, bool F32> struct ForceCfg {
     // E2L batch: node records (32 B) + 2p coefficients per node, whole groups
     static constexpr int E2L_BYTES = 32 + 16 * P;
     static constexpr int E2L_QB = FBUF_BYTES / E2L_BYTES / E2L_NQ<P> * E2L_NQ<P>;
-    // P2P tiles: two 1 KiB halves of buf, 64 FP32 sources (float4) or 32
-    // FP64 sources (x,y and v,- halves of the double4 in separate 512 B rows)
-    static constexpr int TILE_N = F32 ? 64 : 32;
     static_assert(E2P_QB >= 1 && E2L_QB >= E2L_NQ<P>, "staging buffer too small for this order");
+};
+
+// P2P tiles: two 1 KiB halves of the staging buffer, 64 FP32 sources (float4)
+// or 32 FP64 sources (x,y and v,- halves of the double4 in separate 512 B rows)
+template <bool F32> constexpr int P2P_TILE_N = F32 ? 64 : 32;
+// Near field of the force traversal for one range of bricks (brick indices
+// relative to the range).  The traversal sums a brick's first FP2P_INLINE
+// P2P leaves itself; further leaves go to chunks of up to 64 leaves from a
+// shared pool, linked per brick in traversal order, each summed by its own
+// wave (k_force_chunks) and added to the brick in chain order
+// (k_force_combine), so that a brick with thousands of near leaves does not
+// leave one wave running long after the rest.  A brick whose chunks do not fit
+// in the pool is marked in pmark and listed in ovf; the NEAR instantiation of
+// k_force then repeats its traversal and sums all its P2P leaves.
+constexpr int FP2P_INLINE = 64;
+enum { FN_OVF = 0, FN_CHUNKS = 1, FN_OBRICKS = 2, FN_NCTR = 4 };
+struct TcNearLists {
+    uint32_t *pmark;    // [nb] 1: the brick is listed in ovf
+    int *ohead;         // [nb] first chunk of a brick listed in obricks
+    uint32_t *obricks;  // [nb] bricks with chunks
+    uint32_t *ovf;      // [nb] bricks marked FP2P_OVERFLOW (absolute brick ids)
+    int4 *cdesc;        // [maxc] chunk: brick, leaf count, next chunk of the brick (-1 ends)
+    uint32_t *cpool;    // [maxc * 64] leaf ids
+    double2 *cpart;     // [maxc * 64] chunk sums per target
+    uint32_t *ctr;      // [FN_NCTR] counters
+    int maxc;
 };
 
 typedef __attribute__((address_space(3))) void tc_lds_void;
@@ -675,40 +698,57 @@ TC_DEV void p2p_tile_sum(const void *__restrict__ cur, void *__restrict__ nxt, b
     if (more)
         p2p_tile_load<F32>(src, src32, nb0, nne, nxt);
     if (F32) {
+        // Lanes l and l^32 split the tile (even / odd sources) and each
+        // evaluates both targets, its own and its partner's: one broadcast LDS
+        // read feeds two pair evaluations.  Partner sums are exchanged at the
+        // end of the tile.
         const float tx = (float)(xt - xcom), ty = (float)(yt - ycom);
+        const float ux = __shfl_xor(tx, 32, 64), uy = __shfl_xor(ty, 32, 64);
+        const int half = tc_lane() >> 5;
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 *tile = static_cast<const float4 *>(cur);
-        float ax0 = 0.f, ay0 = 0.f, ax1 = 0.f, ay1 = 0.f;
+        float sx0 = 0.f, sy0 = 0.f, sx1 = 0.f, sy1 = 0.f;  // own target
+        float ox0 = 0.f, oy0 = 0.f, ox1 = 0.f, oy1 = 0.f;  // partner's target
         int j = 0;
-        for (; j + 4 <= cnt; j += 4) {
+        for (; 2 * j + 3 < cnt; j += 2) {
 #pragma clang fp contract(fast)
-            const float4 q0 = tile[j], q1 = tile[j + 1], q2 = tile[j + 2], q3 = tile[j + 3];
-            const float dx0 = tx - q0.x, dy0 = ty - q0.y;
-            const float dx1 = tx - q1.x, dy1 = ty - q1.y;
-            const float dx2 = tx - q2.x, dy2 = ty - q2.y;
-            const float dx3 = tx - q3.x, dy3 = ty - q3.y;
-            const float f0 = q0.z * fast_rcp(dx0 * dx0 + (dy0 * dy0 + EPS_REG32));
-            const float f1 = q1.z * fast_rcp(dx1 * dx1 + (dy1 * dy1 + EPS_REG32));
-            const float f2 = q2.z * fast_rcp(dx2 * dx2 + (dy2 * dy2 + EPS_REG32));
-            const float f3 = q3.z * fast_rcp(dx3 * dx3 + (dy3 * dy3 + EPS_REG32));
-            ax0 += dx0 * f0;
-            ay0 += dy0 * f0;
-            ax1 += dx1 * f1;
-            ay1 += dy1 * f1;
-            ax0 += dx2 * f2;
-            ay0 += dy2 * f2;
-            ax1 += dx3 * f3;
-            ay1 += dy3 * f3;
+            const float4 qa = tile[2 * j + half], qb = tile[2 * j + 2 + half];
+            float dx = tx - qa.x, dy = ty - qa.y;
+            float f = qa.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            sx0 += dx * f;
+            sy0 += dy * f;
+            dx = ux - qa.x;
+            dy = uy - qa.y;
+            f = qa.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            ox0 += dx * f;
+            oy0 += dy * f;
+            dx = tx - qb.x;
+            dy = ty - qb.y;
+            f = qb.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            sx1 += dx * f;
+            sy1 += dy * f;
+            dx = ux - qb.x;
+            dy = uy - qb.y;
+            f = qb.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            ox1 += dx * f;
+            oy1 += dy * f;
         }
-        for (; j < cnt; ++j) {
+        for (; 2 * j < cnt; ++j) {
 #pragma clang fp contract(fast)
-            const float4 q = tile[j];
-            const float dx = tx - q.x, dy = ty - q.y;
-            const float f = q.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
-            ax0 += dx * f;
-            ay0 += dy * f;
+            const int idx = 2 * j + half;
+            const float4 qa = idx < cnt ? tile[idx] : zero4;
+            float dx = tx - qa.x, dy = ty - qa.y;
+            float f = qa.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            sx0 += dx * f;
+            sy0 += dy * f;
+            dx = ux - qa.x;
+            dy = uy - qa.y;
+            f = qa.z * fast_rcp(dx * dx + (dy * dy + EPS_REG32));
+            ox0 += dx * f;
+            oy0 += dy * f;
         }
-        fx += (double)(ax0 + ax1);
-        fy += (double)(ay0 + ay1);
+        fx += (double)(sx0 + sx1) + (double)__shfl_xor(ox0 + ox1, 32, 64);
+        fy += (double)(sy0 + sy1) + (double)__shfl_xor(oy0 + oy1, 32, 64);
     } else {
         const double2 *xy = static_cast<const double2 *>(cur);
         const double2 *vv = xy + 32;
@@ -745,12 +785,12 @@ TC_DEV void p2p_tile_sum(const void *__restrict__ cur, void *__restrict__ nxt, b
 // force-kernels.m4:15-55).  Sources pass through the two LDS halves of buf in
 // tiles of TILE_N: the copy of the next tile (of this leaf or the next one)
 // runs while the current one is summed.
-template <int P, bool F32>
+template <bool F32>
 TC_DEV void force_flush_p2p(const int *list, int n, const TcNode *__restrict__ nodes,
                             const double4 *__restrict__ src, const float4 *__restrict__ src32, double4 *buf,
                             double xt, double yt, double &fx, double &fy, unsigned long long &np2p)
 {
-    constexpr int T = ForceCfg<P, F32>::TILE_N;
+    constexpr int T = P2P_TILE_N<F32>;
     const int lane = tc_lane();
     char *const base = reinterpret_cast<char *>(buf);
     for (int c0 = 0; c0 < n; c0 += 64) {
@@ -795,6 +835,59 @@ TC_DEV void force_flush_p2p(const int *list, int n, const TcNode *__restrict__ n
     }
 }
 
+// P2P of the listed leaves of brick `wave`: the brick's first FP2P_INLINE
+// leaves (ninl counts them) are summed here, further ones go to chunks (see
+// TcNearLists); prev is the brick's last chunk.
+template <bool F32>
+TC_DEV void force_near_p2p(const int *list, int n, const TcNearLists &nl, int wave, int &ninl, int &prev,
+                           bool &ovfl, const TcNode *__restrict__ nodes, const double4 *__restrict__ src,
+                           const float4 *__restrict__ src32, double4 *buf, double xt, double yt, double &fx,
+                           double &fy, unsigned long long &np2p)
+{
+    if (ovfl)
+        return;
+    const int lane = tc_lane();
+    const int take = min(n, FP2P_INLINE - ninl);
+    if (take > 0)
+        force_flush_p2p<F32>(list, take, nodes, src, src32, buf, xt, yt, fx, fy, np2p);
+    ninl += take;
+    for (int off = take; off < n; off += 64) {
+        const int cnt = min(64, n - off);
+        int c = 0;
+        if (lane == 0)
+            c = (int)atomicAdd(&nl.ctr[FN_CHUNKS], 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= nl.maxc) {
+            ovfl = true;
+            return;
+        }
+        if (lane < cnt)
+            nl.cpool[(size_t)c * 64 + lane] = (uint32_t)list[off + lane];
+        if (lane == 0) {
+            nl.cdesc[c] = make_int4(wave, cnt, -1, 0);
+            if (prev >= 0) {
+                nl.cdesc[prev].z = c;
+            } else {
+                nl.ohead[wave] = c;
+                nl.obricks[atomicAdd(&nl.ctr[FN_OBRICKS], 1u)] = (uint32_t)wave;
+            }
+        }
+        prev = c;
+    }
+}
+
+// P2P of the listed leaves after the first `skip` (which count down).
+template <bool F32>
+TC_DEV void force_skip_p2p(const int *list, int n, int &skip, const TcNode *__restrict__ nodes,
+                           const double4 *__restrict__ src, const float4 *__restrict__ src32, double4 *buf,
+                           double xt, double yt, double &fx, double &fy, unsigned long long &np2p)
+{
+    const int s0 = min(n, skip);
+    skip -= s0;
+    if (n > s0)
+        force_flush_p2p<F32>(list + s0, n - s0, nodes, src, src32, buf, xt, yt, fx, fy, np2p);
+}
+
 // EvaluateForce::evaluate (treecode-force.cpp:94-199) with one wavefront per
 // 8x8 brick (lane = target).  Bricks are dealt out one at a time from a device
 // counter and each wave runs until the queue is empty: brick costs differ by
@@ -802,15 +895,20 @@ TC_DEV void force_flush_p2p(const int *list, int n, const TcNode *__restrict__ n
 // released when all four of its waves are done, so fixed brick-per-wave
 // launches left SIMDs idle.  The brick's interaction lists are the
 // reference's (same decisions per node); they are evaluated in list order
-// rather than depth-first order.
-template <int P, bool F32>
+// rather than depth-first order.  With NEAR = false the pass computes the
+// far field (E2L, E2P, L2P) and the first FP2P_INLINE P2P leaves, writes the
+// outputs and hands further P2P leaves to k_force_chunks (see TcNearLists).
+// With NEAR = true the waves take the bricks whose leaves did not fit there,
+// repeat the same traversal and add all their P2P sums.
+template <int P, bool F32, bool NEAR>
 __global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
                                                const double *__restrict__ y0s, const double *__restrict__ hs,
-                                               int nbricks, double theta, double *__restrict__ xres,
-                                               double *__restrict__ yres, unsigned long long *counters,
-                                               uint32_t *__restrict__ work)
+                                               int first, int nbricks, double theta,
+                                               double *__restrict__ xres, double *__restrict__ yres,
+                                               unsigned long long *counters, uint32_t *__restrict__ work,
+                                               TcNearLists nlist)
 {
     __shared__ int s_stack[4][FS_CAP];
     __shared__ int s_list[4][3][FQ_CAP];  // E2L, E2P, P2P node ids
@@ -819,27 +917,39 @@ __global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nod
     __shared__ double s_binom[P * E2L_BW<P>];
     __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
-    for (int i = threadIdx.x; i < P * E2L_BW<P>; i += 256)
-        s_binom[i] = c_binomT[(i / E2L_BW<P>) * 64 + i % E2L_BW<P>];
-    if (threadIdx.x <= TC_MAXORDER)
-        s_ninv[threadIdx.x] = threadIdx.x ? -(1.0 / (double)threadIdx.x) : 0.0;
-    __syncthreads();
+    if constexpr (!NEAR) {
+        for (int i = threadIdx.x; i < P * E2L_BW<P>; i += 256)
+            s_binom[i] = c_binomT[(i / E2L_BW<P>) * 64 + i % E2L_BW<P>];
+        if (threadIdx.x <= TC_MAXORDER)
+            s_ninv[threadIdx.x] = threadIdx.x ? -(1.0 / (double)threadIdx.x) : 0.0;
+        __syncthreads();
+    }
+    // NEAR: the bricks listed in nlist.ovf, dealt out by position
+    const uint32_t nwork = NEAR ? nlist.ctr[FN_OVF] : (uint32_t)nbricks;
 
     const int wid = threadIdx.x >> 6, lane = tc_lane();
     int *stack = s_stack[wid];
     int *lst_l = s_list[wid][0], *lst_e = s_list[wid][1], *lst_p = s_list[wid][2];
     unsigned long long np2p = 0, ne2p = 0, ne2l = 0, nl2p = 0;
-    for (;;) {
+    for (uint32_t iter = 0;; ++iter) {
         uint32_t grab = 0;
-        if (lane == 0)
+        if (NEAR)
+            grab = (blockIdx.x + iter * gridDim.x) * 4 + wid;
+        else if (lane == 0)
             grab = atomicAdd(work, 1u);
-        const uint32_t wave = (uint32_t)__builtin_amdgcn_readlane((int)grab, 0);
-        if (wave >= (uint32_t)nbricks)
+        const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)grab);
+        if (wave >= nwork)
             break;
-        const int b = wave >> 4, brick = wave & 15;
+        const int gbrick = NEAR ? (int)nlist.ovf[wave] : first + (int)wave;
+        const int b = gbrick >> 4, brick = gbrick & 15;
         const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
         const double x0 = x0s[b], y0 = y0s[b], h = hs[b];
         const int ix = lane & 7, iy = lane >> 3;
+        int ninl = 0, prev = -1;
+        bool ovfl = false;
+        // NEAR: the far-field pass summed the brick's first FP2P_INLINE P2P
+        // leaves; the repeated traversal lists the leaves in the same order
+        int skip = FP2P_INLINE;
         // targets as in force_p2p_8x8 (force-kernels.m4:28-32); the brick-wide
         // values are moved to scalar registers
         const double xb = tc_uniform(x0 + (bx + 0) * h), yb = tc_uniform(y0 + (by + 0) * h);
@@ -887,16 +997,18 @@ __global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nod
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     stack[pos + c] = child + c;
-            } else if (act == ACT_E2L) {
+            } else if (act == ACT_E2L && !NEAR) {
                 lst_l[nl + __popcll(ml & lt)] = id;
-            } else if (act == ACT_E2P) {
+            } else if (act == ACT_E2P && !NEAR) {
                 lst_e[ne + __popcll(me & lt)] = id;
             } else if (act == ACT_P2P) {
                 lst_p[np + __popcll(mp & lt)] = id;
             }
             sp += 4 * __popcll(mo);
-            nl += __popcll(ml);
-            ne += __popcll(me);
+            if constexpr (!NEAR) {
+                nl += __popcll(ml);
+                ne += __popcll(me);
+            }
             np += __popcll(mp);
             tc_wave_sync();
             // a list that might not take another 64 ids is flushed now
@@ -912,12 +1024,31 @@ __global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nod
                 ne = 0;
             }
             if (np > FQ_CAP - 64) {
-                force_flush_p2p<P, F32>(lst_p, np, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+                if constexpr (NEAR)
+                    force_skip_p2p<F32>(lst_p, np, skip, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+                else
+                    force_near_p2p<F32>(lst_p, np, nlist, (int)wave, ninl, prev, ovfl, nodes, src, src32,
+                                        s_buf[wid], xt, yt, fx, fy, np2p);
                 np = 0;
             }
         }
+        if constexpr (NEAR) {
+            if (np)
+                force_skip_p2p<F32>(lst_p, np, skip, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+            const size_t o = (size_t)b * 1024 + (size_t)(bx + ix) + 32 * (size_t)(by + iy);
+            xres[o] += fx;
+            yres[o] += fy;
+            tc_wave_sync();
+            continue;
+        }
         if (np)
-            force_flush_p2p<P, F32>(lst_p, np, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+            force_near_p2p<F32>(lst_p, np, nlist, (int)wave, ninl, prev, ovfl, nodes, src, src32, s_buf[wid], xt,
+                                yt, fx, fy, np2p);
+        if (lane == 0) {
+            nlist.pmark[wave] = ovfl ? 1u : 0u;
+            if (ovfl)
+                nlist.ovf[atomicAdd(&nlist.ctr[FN_OVF], 1u)] = (uint32_t)gbrick;
+        }
         if (ne) {
             ne2p += 64ull * ne;
             force_flush_e2p<P, F32>(lst_e, ne, nodes, exps, exps32, s_buf[wid], xb, yb, ix, iy, h, fx, fy);
@@ -969,6 +1100,71 @@ __global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nod
         atomicAdd(&counters[TC_CNT_L2P], nl2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
         atomicAdd(&counters[TC_CNT_E2P_SLOTS], ne2p);
+    }
+}
+
+// P2P sums of the near-leaf chunks (see TcNearLists), one wave per chunk,
+// into cpart; k_force_combine adds them to the outputs.
+template <bool F32>
+__global__ __launch_bounds__(256) void k_force_chunks(const TcNode *__restrict__ nodes,
+                                                      const double4 *__restrict__ src,
+                                                      const float4 *__restrict__ src32,
+                                                      const double *__restrict__ x0s, const double *__restrict__ y0s,
+                                                      const double *__restrict__ hs, int first, TcNearLists nlist,
+                                                      unsigned long long *counters)
+{
+    __shared__ double4 s_buf[4][FBUF_BYTES / 32];
+    const int wid = threadIdx.x >> 6, lane = tc_lane();
+    const int nch = min((int)nlist.ctr[FN_CHUNKS], nlist.maxc);
+    unsigned long long np2p = 0;
+    for (int c = blockIdx.x * 4 + wid; c < nch; c += gridDim.x * 4) {
+        const int4 d = nlist.cdesc[c];
+        if (nlist.pmark[__builtin_amdgcn_readfirstlane(d.x)])
+            continue;  // summed by k_force<P, F32, true>
+        const int gbrick = first + __builtin_amdgcn_readfirstlane(d.x);
+        const int b = gbrick >> 4, brick = gbrick & 15;
+        const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
+        const double x0 = x0s[b], y0 = y0s[b], h = hs[b];
+        const int ix = lane & 7, iy = lane >> 3;
+        const double xb = tc_uniform(x0 + (bx + 0) * h), yb = tc_uniform(y0 + (by + 0) * h);
+        const double xt = xb + ix * h, yt = yb + iy * h;
+        double fx = 0, fy = 0;
+        force_flush_p2p<F32>(reinterpret_cast<const int *>(nlist.cpool + (size_t)c * 64),
+                             __builtin_amdgcn_readfirstlane(d.y), nodes, src, src32, s_buf[wid], xt, yt, fx, fy,
+                             np2p);
+        nlist.cpart[(size_t)c * 64 + lane] = make_double2(fx, fy);
+    }
+    if (counters && lane == 0) {
+        atomicAdd(&counters[TC_CNT_P2P], np2p);
+        atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
+    }
+}
+
+// Adds each brick's chunk sums to its outputs, in the order the traversal
+// produced the chunks.
+__global__ __launch_bounds__(256) void k_force_combine(const double *__restrict__ x0s, const double *__restrict__ y0s,
+                                                       const double *__restrict__ hs, int first, TcNearLists nlist,
+                                                       double *__restrict__ xres, double *__restrict__ yres)
+{
+    const int lane = tc_lane();
+    const int nob = (int)nlist.ctr[FN_OBRICKS];
+    for (int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < nob; i += gridDim.x * 4) {
+        const int wave = (int)nlist.obricks[i];
+        if (nlist.pmark[wave])
+            continue;
+        double sx = 0, sy = 0;
+        for (int c = nlist.ohead[wave]; c >= 0; c = nlist.cdesc[c].z) {
+            const double2 pc = nlist.cpart[(size_t)c * 64 + lane];
+            sx += pc.x;
+            sy += pc.y;
+        }
+        const int gbrick = first + wave;
+        const int b = gbrick >> 4, brick = gbrick & 15;
+        const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
+        const int ix = lane & 7, iy = lane >> 3;
+        const size_t o = (size_t)b * 1024 + (size_t)(bx + ix) + 32 * (size_t)(by + iy);
+        xres[o] += sx;
+        yres[o] += sy;
     }
 }
 
@@ -1203,29 +1399,72 @@ template <auto Kernel> int resident_blocks(int device)
     return v;
 }
 
+template <int P, bool F32>
+void launch_force_pass(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
+                       int first, int nb, double *xr, double *yr, unsigned long long *counters, uint32_t *work,
+                       const TcNearLists &nl, int device, hipStream_t st)
+{
+    const TcNode *nodes = t.nodes.as<TcNode>();
+    const double *exps = t.exps.as<double>();
+    const float2 *exps32 = F32 ? t.exps32.as<float2>() : nullptr;
+    const double4 *src = t.src.as<double4>();
+    const float4 *src32 = F32 ? t.src32.as<float4>() : nullptr;
+    tc_fill_u32(work, 0u, 1, st);
+    tc_fill_u32(nl.ctr, 0u, FN_NCTR, st);
+    const int grid = std::min(tiles_of(nb, 4), resident_blocks<k_force<P, F32, false>>(device));
+    k_force<P, F32, false><<<grid, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb, theta,
+                                                 xr, yr, counters, work, nl);
+    // the chunk and overflow counts are on the device: persistent grids
+    k_force_chunks<F32><<<512, 256, 0, st>>>(nodes, src, src32, x0s, y0s, hs, first, nl, counters);
+    k_force_combine<<<64, 256, 0, st>>>(x0s, y0s, hs, first, nl, xr, yr);
+    k_force<P, F32, true><<<256, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb,
+                                                    theta, xr, yr, counters, work, nl);
+    TC_HIP(hipGetLastError());
+}
+
 template <int P>
 void launch_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
-                  int nblocks, double *xr, double *yr, int precision, unsigned long long *counters, uint32_t *work,
-                  hipStream_t st)
+                  int nblocks, double *xr, double *yr, int precision, unsigned long long *counters, TcScratch &ws,
+                  int max_chunks, hipStream_t st)
 {
     if (nblocks > (1 << 26))
         throw TcError(-2, "tc_eval_force: too many blocks");
     const int nbricks = nblocks * 16;
     int device = 0;
     TC_HIP(hipGetDevice(&device));
-    tc_fill_u32(work, 0u, 1, st);
-    if (precision == 1) {
-        const int grid = std::min(tiles_of(nbricks, 4), resident_blocks<k_force<P, true>>(device));
-        k_force<P, true><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), t.exps32.as<float2>(),
-                                               t.src.as<double4>(), t.src32.as<float4>(), x0s, y0s, hs, nbricks,
-                                               theta, xr, yr, counters, work);
-    } else {
-        const int grid = std::min(tiles_of(nbricks, 4), resident_blocks<k_force<P, false>>(device));
-        k_force<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
-                                                t.src.as<double4>(), nullptr, x0s, y0s, hs, nbricks, theta, xr, yr,
-                                                counters, work);
+    // bricks per pass, which bounds the near-field hand-off buffers
+    constexpr int CHUNK = 1 << 20;
+    const int chunk = std::min(nbricks, CHUNK);
+    const int maxc = max_chunks > 0 ? max_chunks : std::max(4096, chunk / 4);
+    const size_t part_bytes = sizeof(double2) * 64 * (size_t)maxc;
+    const size_t u32s = (size_t)chunk * 4 + (size_t)maxc * (64 + 4) + FN_NCTR;
+    ws.work.reserve(256);
+    ws.flist.reserve(part_bytes + sizeof(uint32_t) * u32s);
+    TcNearLists nl;
+    nl.cpart = ws.flist.as<double2>();
+    uint32_t *u = reinterpret_cast<uint32_t *>(ws.flist.as<char>() + part_bytes);
+    nl.cdesc = reinterpret_cast<int4 *>(u);  // 16-byte aligned: part_bytes is a multiple of 1024
+    u += (size_t)maxc * 4;
+    nl.cpool = u;
+    u += (size_t)maxc * 64;
+    nl.pmark = u;
+    u += chunk;
+    nl.ohead = reinterpret_cast<int *>(u);
+    u += chunk;
+    nl.obricks = u;
+    u += chunk;
+    nl.ovf = u;
+    u += chunk;
+    nl.ctr = u;
+    nl.maxc = maxc;
+    uint32_t *work = ws.work.as<uint32_t>();
+    for (int first = 0; first < nbricks; first += chunk) {
+        const int nb = std::min(chunk, nbricks - first);
+        if (precision == 1)
+            launch_force_pass<P, true>(t, theta, x0s, y0s, hs, first, nb, xr, yr, counters, work, nl, device, st);
+        else
+            launch_force_pass<P, false>(t, theta, x0s, y0s, hs, first, nb, xr, yr, counters, work, nl, device, st);
     }
-    TC_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -1291,7 +1530,7 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
 
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
-                   uint32_t *work, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+                   TcScratch &ws, int max_chunks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (nblocks <= 0)
         return;
@@ -1300,8 +1539,10 @@ void tc_eval_force(const TcTree &t, double theta, const double *x0s, const doubl
     if (ev0)
         TC_HIP(hipEventRecord(ev0, st));
     switch (t.order) {
-#define TC_CASE(P) \
-    case P: launch_force<P>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, work, st); break;
+#define TC_CASE(P)                                                                                         \
+    case P:                                                                                                \
+        launch_force<P>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, ws, max_chunks, st); \
+        break;
         TC_ORDERS(TC_CASE)
 #undef TC_CASE
     default: throw TcError(-3, "tc_eval_force: unsupported order");

@@ -89,7 +89,11 @@ enum {
     TC_OPT_FORCE_ORDER = 3,    /* force expansion order, default 24 */
     TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
     TC_OPT_PROFILE = 5,        /* 1 = record phase times with HIP events */
-    TC_OPT_COUNTERS = 6        /* 1 = count interactions (see tc_get_stats) */
+    TC_OPT_COUNTERS = 6,       /* 1 = count interactions (see tc_get_stats) */
+    TC_OPT_FORCE_CHUNKS = 7    /* force near-field pool, in chunks of 64 leaves; 0 = automatic
+                                * (max(4096, bricks / 4)).  A brick with more than 64 P2P leaves
+                                * spreads the rest over chunks; bricks that find the pool full
+                                * repeat their traversal on a slower path (diagnostic setting) */
 };
 
 /* Phase times (tc_get_stats times_ms[]) */

@@ -157,6 +157,32 @@ def test_force_orders_vs_oracle(pkg, oracle, wl, order, leaf):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision,chunks", [(0, 0), (1, 0), (0, 2), (1, 2), (0, 1)])
+def test_force_many_near_leaves(pkg, oracle, wl, precision, chunks):
+    # leaves of at most 8 sources under coarse blocks (bricks a quarter of the
+    # box wide): every brick of the coarse block has hundreds of P2P leaves,
+    # more than the traversal sums itself, so the rest goes to 64-leaf chunks
+    # summed by other waves; the fine blocks need no chunks.  A pool of one or
+    # two chunks (TC_OPT_FORCE_CHUNKS) fills up, and the bricks left over take
+    # the repeated-traversal path, which must skip the leaves already summed.
+    st = wl.Stream(31)
+    n = 30000
+    xs, ys, vs = st.uniform(n), st.uniform(n), st.uniform(n, -1, 1)
+    x0s = np.array([0.0, 0.31, 0.62, 0.0])
+    y0s = np.array([0.0, 0.27, 0.55, 0.5])
+    hs = np.array([1.0 / 32, 1.0 / 1024, 1.0 / 2048, 1.0 / 64])
+    ctx = pkg.Context(precision=precision, counters=True, force_order=12, force_leaf=8)
+    ctx.set(pkg.OPT_FORCE_CHUNKS, chunks)
+    fx, fy = ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, cnt = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs, order=12, leafcap=8)
+    tol = FP64_TOL if precision == 0 else FP32_TOL
+    assert rel_l2(fx, rx) <= tol
+    assert rel_l2(fy, ry) <= tol
+    c = ctx.stats()["counters"]
+    assert (c["p2p"], c["e2p"], c["e2l"], c["l2p"]) == tuple(int(v) for v in cnt)
+    ctx.close()
+
+
 def test_potential_targets_elsewhere(pkg, oracle, wl, ctx64):
     # targets outside the source box, in random order, fewer than sources
     st = wl.Stream(77)
