@@ -402,8 +402,6 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
 // over k and multiplies by u^(l+1).
 template <int P> constexpr int E2L_G = P <= 16 ? 16 : 32;
 template <int P> constexpr int E2L_NQ = 64 / E2L_G<P>;
-// row stride of the LDS binomial table C(n, k), k < P, n < P + E2L_G
-template <int P> constexpr int E2L_BW = P + E2L_G<P>;
 
 // rec[g] = {xcom, ycom, r, mass}; co[g * 2P + j] = Re a_(j+1), co[g * 2P + P + j] = Im a_(j+1)
 template <int P>
@@ -451,7 +449,7 @@ TC_DEV void e2l_group(const double4 *rec, const double *co, int nq, double x0b, 
         const double2 *c = s_c + h * (G + 1);
 #pragma unroll 2
         for (int kk = 1; kk <= P; ++kk) {
-            const double bb = s_binom[(kk - 1) * E2L_BW<P> + L + kk - 1];
+            const double bb = s_binom[(kk - 1) * P + l];  // C(L + kk - 1, kk - 1)
             const double2 ck = c[kk];
             rt += bb * ck.x;
             it += bb * ck.y;
@@ -901,7 +899,7 @@ TC_DEV void force_skip_p2p(const int *list, int n, int &skip, const TcNode *__re
 // With NEAR = true the waves take the bricks whose leaves did not fit there,
 // repeat the same traversal and add all their P2P sums.
 template <int P, bool F32, bool NEAR>
-__global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+__global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
                                                const double *__restrict__ y0s, const double *__restrict__ hs,
@@ -914,12 +912,12 @@ __global__ __launch_bounds__(256, 5) void k_force(const TcNode *__restrict__ nod
     __shared__ int s_list[4][3][FQ_CAP];  // E2L, E2P, P2P node ids
     __shared__ double4 s_buf[4][FBUF_BYTES / 32];
     __shared__ double2 s_coef[4][68];  // E2L_NQ groups of E2L_G + 1 entries; L2P uses [1..P]
-    __shared__ double s_binom[P * E2L_BW<P>];
+    __shared__ double s_binom[P * P];  // [k - 1][L - 1] = C(L + k - 1, k - 1)
     __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
     if constexpr (!NEAR) {
-        for (int i = threadIdx.x; i < P * E2L_BW<P>; i += 256)
-            s_binom[i] = c_binomT[(i / E2L_BW<P>) * 64 + i % E2L_BW<P>];
+        for (int i = threadIdx.x; i < P * P; i += 256)
+            s_binom[i] = c_binomT[(i / P) * 64 + i % P + i / P + 1];
         if (threadIdx.x <= TC_MAXORDER)
             s_ninv[threadIdx.x] = threadIdx.x ? -(1.0 / (double)threadIdx.x) : 0.0;
         __syncthreads();
