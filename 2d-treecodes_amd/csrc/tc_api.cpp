@@ -718,8 +718,7 @@ TC_API int tc_morton_sort_device(tc_context *c, const double *x, const double *y
     return guarded(c, [&] {
         if (n <= 0)
             throw TcError(-3, "tc_morton_sort_device: n must be positive");
-        tc_bbox(x, y, n, box, c->ws, c->stream);
-        tc_morton_sort(x, y, nullptr, n, box, keys, perm, nullptr, c->ws, c->stream);
+        tc_sort_points(x, y, nullptr, n, box, keys, perm, nullptr, c->ws, c->stream, false);
     });
 }
 

@@ -380,6 +380,179 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     }
 }
 
+// ------------------------------------------------------------------ small sorts
+// Up to SS_N points in one workgroup: bounding box (k_bbox_partial /
+// k_bbox_final semantics), keys (Morton as k_keys or Hilbert as
+// k_keys_hilbert), the same stable 4-pass LSD radix sort as above with the
+// keys in registers and each pass's scatter in LDS, and for sources the gather
+// of the sorted records.  One launch instead of about sixteen, which is what a
+// small solve is made of.
+constexpr int SS_NT = 1024, SS_ITEMS = 16, SS_N = SS_NT * SS_ITEMS;
+
+template <bool HILBERT>
+__global__ __launch_bounds__(SS_NT) void k_sort_small(const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ v, int n, double *__restrict__ box_out,
+                                                      uint32_t *__restrict__ bad_out, uint32_t *__restrict__ keys_out,
+                                                      uint32_t *__restrict__ perm_out, double4 *__restrict__ src_out)
+{
+    constexpr int NW = SS_NT / 64, WITEMS = 64 * SS_ITEMS;
+    __shared__ uint32_t s_k[SS_N];
+    __shared__ uint32_t s_v[SS_N];
+    __shared__ uint32_t s_wrun[NW][RS_BINS];
+    __shared__ uint32_t s_w[NW + 1];
+    __shared__ double4 s_red[NW];
+    __shared__ double s_box[4];
+    __shared__ uint32_t s_bad[NW];
+    const int tid = threadIdx.x, lane = tc_lane(), wid = tid >> 6;
+
+    // this thread's points, item j of lane l of wave w being position
+    // w * WITEMS + 64 j + l (all loads in flight at once)
+    double xr[SS_ITEMS], yr[SS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SS_ITEMS; ++j) {
+        const int i = wid * WITEMS + j * 64 + lane;
+        xr[j] = i < n ? x[i] : 0.0;
+        yr[j] = i < n ? y[i] : 0.0;
+    }
+    // bounding box: max accumulators start at 0 as in the reference
+    double xmi = 1e13, ymi = 1e13, xma = 0, yma = 0;
+    bool nonfinite = false;
+#pragma unroll
+    for (int j = 0; j < SS_ITEMS; ++j) {
+        const int i = wid * WITEMS + j * 64 + lane;
+        if (i < n) {
+            const double a = xr[j], b = yr[j];
+            nonfinite |= !isfinite(a) || !isfinite(b);
+            xmi = a < xmi ? a : xmi;
+            xma = xma < a ? a : xma;
+            ymi = b < ymi ? b : ymi;
+            yma = yma < b ? b : yma;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o;
+        o = __shfl_xor(xmi, off, 64); xmi = o < xmi ? o : xmi;
+        o = __shfl_xor(ymi, off, 64); ymi = o < ymi ? o : ymi;
+        o = __shfl_xor(xma, off, 64); xma = xma < o ? o : xma;
+        o = __shfl_xor(yma, off, 64); yma = yma < o ? o : yma;
+    }
+    const bool wbad = tc_ballot(nonfinite) != 0;
+    if (lane == 0) {
+        s_red[wid] = make_double4(xmi, xma, ymi, yma);
+        s_bad[wid] = wbad ? 1u : 0u;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double4 r = s_red[0];
+        uint32_t bad = s_bad[0];
+        for (int w = 1; w < NW; ++w) {
+            r.x = s_red[w].x < r.x ? s_red[w].x : r.x;
+            r.y = r.y < s_red[w].y ? s_red[w].y : r.y;
+            r.z = s_red[w].z < r.z ? s_red[w].z : r.z;
+            r.w = r.w < s_red[w].w ? s_red[w].w : r.w;
+            bad |= s_bad[w];
+        }
+        const double ext0 = r.y - r.x, ext1 = r.w - r.z;
+        const double eps = 10000 * DBL_EPSILON;
+        double ext = (ext0 < ext1 ? ext1 : ext0) * (1 + 2 * eps);
+        if (!(ext > 0))
+            ext = 1.0;
+        s_box[0] = r.x - eps * ext;
+        s_box[1] = r.z - eps * ext;
+        s_box[2] = ext;
+        s_box[3] = (1.0 / ext) * (double)(1 << TC_LMAX);
+        for (int q = 0; q < 4; ++q)
+            box_out[q] = s_box[q];
+        *bad_out = bad;
+    }
+    __syncthreads();
+
+    const double xmin = s_box[0], ymin = s_box[1], scale = s_box[3];
+    uint32_t k[SS_ITEMS], vi[SS_ITEMS], dr[SS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SS_ITEMS; ++j) {
+        const int i = wid * WITEMS + j * 64 + lane;
+        const uint32_t qx = quantise(xr[j], xmin, scale), qy = quantise(yr[j], ymin, scale);
+        k[j] = i < n ? (HILBERT ? hilbert15(qx, qy) : (tc_spread15(qx) | (tc_spread15(qy) << 1))) : 0u;
+        vi[j] = (uint32_t)i;
+    }
+
+#pragma unroll 1
+    for (int shift = 0;; shift += 8) {
+        for (int d = lane; d < RS_BINS; d += 64)
+            s_wrun[wid][d] = 0;
+        tc_wave_sync();
+        // wave-local stable ranks, as in k_rs_scatter
+#pragma unroll
+        for (int j = 0; j < SS_ITEMS; ++j) {
+            const bool ok = wid * WITEMS + j * 64 + lane < n;
+            const uint32_t d = (k[j] >> shift) & (RS_BINS - 1);
+            unsigned long long peers = tc_ballot(ok);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const unsigned long long bb = tc_ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            const uint32_t lower = __popcll(peers & tc_lanemask_lt());
+            const uint32_t run = ok ? s_wrun[wid][d] : 0u;
+            tc_wave_sync();
+            if (ok && lower == 0)
+                s_wrun[wid][d] = run + (uint32_t)__popcll(peers);
+            tc_wave_sync();
+            dr[j] = ((run + lower) << 8) | d;
+        }
+        __syncthreads();
+        {
+            // digit tid: start = exclusive scan of the digit totals; wave w's
+            // keys of that digit follow waves 0..w-1
+            uint32_t c[NW], tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                c[w] = tid < RS_BINS ? s_wrun[w][tid] : 0u;
+                tot += c[w];
+            }
+            uint32_t off = block_scan_excl<SS_NT>(tot, s_w, nullptr);
+            if (tid < RS_BINS) {
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    s_wrun[w][tid] = off;
+                    off += c[w];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SS_ITEMS; ++j) {
+            if (wid * WITEMS + j * 64 + lane < n) {
+                const uint32_t pos = s_wrun[wid][dr[j] & 255u] + (dr[j] >> 8);
+                s_k[pos] = k[j];
+                s_v[pos] = vi[j];
+            }
+        }
+        __syncthreads();
+        if (shift == 24)
+            break;
+#pragma unroll
+        for (int j = 0; j < SS_ITEMS; ++j) {
+            const int p = wid * WITEMS + j * 64 + lane;
+            if (p < n) {
+                k[j] = s_k[p];
+                vi[j] = s_v[p];
+            }
+        }
+    }
+
+#pragma unroll 8
+    for (int p = tid; p < n; p += SS_NT) {
+        const uint32_t i = s_v[p];
+        keys_out[p] = s_k[p];
+        perm_out[p] = i;
+        if (src_out)
+            src_out[p] = make_double4(x[i], y[i], v[i], 0.0);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_gather_src(const double4 *packed, const uint32_t *perm, int n,
                                                      double4 *src)
 {
@@ -1037,6 +1210,32 @@ static const uint32_t *tc_bbox_bad_flag(TcScratch &ws, int n)
     return reinterpret_cast<const uint32_t *>(ws.bbox_part.as<double4>() + np);
 }
 
+void tc_sort_points(const double *x, const double *y, const double *v, int n, double *box_d, uint32_t *keys_out,
+                    uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert)
+{
+    if (n <= SS_N) {
+        ws.bbox_part.reserve(sizeof(double4) + 256);
+        uint32_t *bad = const_cast<uint32_t *>(tc_bbox_bad_flag(ws, n));
+        if (hilbert)
+            k_sort_small<true><<<1, SS_NT, 0, st>>>(x, y, v, n, box_d, bad, keys_out, perm_out, src_out);
+        else
+            k_sort_small<false><<<1, SS_NT, 0, st>>>(x, y, v, n, box_d, bad, keys_out, perm_out, src_out);
+        TC_HIP(hipGetLastError());
+        return;
+    }
+    tc_bbox(x, y, n, box_d, ws, st);
+    double4 *packed = nullptr;
+    if (src_out) {
+        ws.packed.reserve(sizeof(double4) * (size_t)n);
+        packed = ws.packed.as<double4>();
+    }
+    tc_morton_sort(x, y, v, n, box_d, keys_out, perm_out, packed, ws, st, hilbert);
+    if (src_out) {
+        k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(packed, perm_out, n, src_out);
+        TC_HIP(hipGetLastError());
+    }
+}
+
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
                     bool hilbert)
@@ -1128,12 +1327,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
         t.src32.reserve(sizeof(float4) * (size_t)n);
 
     double *box = t.box_d.as<double>();
-    tc_bbox(x, y, n, box, ws, st);
     uint32_t *keys = t.keys.as<uint32_t>(), *perm = t.perm.as<uint32_t>();
-    ws.packed.reserve(sizeof(double4) * (size_t)n);
-    tc_morton_sort(x, y, v, n, box, keys, perm, ws.packed.as<double4>(), ws, st, false);
-    k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(ws.packed.as<double4>(), perm, n, t.src.as<double4>());
-    TC_HIP(hipGetLastError());
+    tc_sort_points(x, y, v, n, box, keys, perm, t.src.as<double4>(), ws, st, false);
 
     // census of internal cells
     const int nblk = tiles_of(n, 4 * TB_CHUNK);
@@ -1206,7 +1401,5 @@ void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, h
     DevBuf &boxbuf = ws.info;
     ws.info.reserve(sizeof(int) * 32 + sizeof(double) * 4);
     double *box = reinterpret_cast<double *>(boxbuf.as<char>() + 128);
-    tc_bbox(xt, yt, m, box, ws, st);
-    tc_morton_sort(xt, yt, nullptr, m, box, ws.tkeys.as<uint32_t>(), ws.tperm.as<uint32_t>(), nullptr, ws, st,
-                   true);
+    tc_sort_points(xt, yt, nullptr, m, box, ws.tkeys.as<uint32_t>(), ws.tperm.as<uint32_t>(), nullptr, ws, st, true);
 }

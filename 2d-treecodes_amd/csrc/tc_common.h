@@ -217,6 +217,11 @@ struct TcScratch {
 
 // Internal entry points (tc_build.hip, tc_eval.hip)
 void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st);
+// Bounding box, keys (Morton, or Hilbert for targets) and stable sort of n
+// points; with src_out also the sorted records {x, y, v, 0}.  One workgroup
+// for n <= 16384, else the multi-kernel path.
+void tc_sort_points(const double *x, const double *y, const double *v, int n, double *box_d, uint32_t *keys_out,
+                    uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert);
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
                     bool hilbert = false);
