@@ -985,6 +985,13 @@ TC_DEV void e2e_node(int rank, const int *selfid, const int2 *se, const int *lev
             const double x0 = chc.xcom - xcp, y0 = chc.ycom - ycp;
             const bool ident = x0 == 0 && y0 == 0;
             const double *ca = exps + (size_t)2 * P * (cb + c);
+            // the child's coefficients, requested now so that the load overlaps
+            // the powers of 1/z0 below
+            double car = 0, cai = 0;
+            if (act && gl < P) {
+                car = ca[gl];
+                cai = ca[P + gl];
+            }
             if (act && !ident && gl == 0) {
                 const double r2z0 = x0 * x0 + y0 * y0;
                 const double ri1 = x0 / r2z0, ii1 = -y0 / r2z0;
@@ -1003,15 +1010,15 @@ TC_DEV void e2e_node(int rank, const int *selfid, const int2 *se, const int *lev
             tc_wave_sync();
             if (act && !ident && gl < P) {
                 const int jj = gl + 1;
-                L.rc[grp][jj] = ca[jj - 1] * L.rinv[grp][jj] - ca[P + jj - 1] * L.iinv[grp][jj];
-                L.ic[grp][jj] = ca[jj - 1] * L.iinv[grp][jj] + ca[P + jj - 1] * L.rinv[grp][jj];
+                L.rc[grp][jj] = car * L.rinv[grp][jj] - cai * L.iinv[grp][jj];
+                L.ic[grp][jj] = car * L.iinv[grp][jj] + cai * L.rinv[grp][jj];
             }
             tc_wave_sync();
             if (gl < P) {
                 double cre = 0, cim = 0;
                 if (act && ident) {
-                    cre = ca[gl];
-                    cim = ca[P + gl];
+                    cre = car;
+                    cim = cai;
                 } else if (act) {
                     const int ll = gl + 1;
                     const double mass = chc.mass;
@@ -1099,10 +1106,12 @@ __global__ __launch_bounds__(64 * E2E_TOP_WAVES) void k_e2e_top(const int *selfi
         const int r0 = starts[lv], r1 = starts[lv + 1];
         for (int rank = r0 + wid; rank < r1; rank += E2E_TOP_WAVES)
             e2e_node<P>(rank, selfid, se, level, cell, box, nodes, nodew, exps, L[wid]);
-        // the next level reads what this one wrote (other waves of the block)
-        __threadfence();
+        // the next level reads what this one wrote; the readers are waves of
+        // this workgroup, which share one CU and its vector L1, so the
+        // barrier's workgroup-scope release / acquire orders the global
+        // writes (device-scope fences would write back and invalidate caches
+        // at every level)
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
 }
 
