@@ -48,7 +48,7 @@ class TcRequest(ctypes.Structure):
                 ("xt", ctypes.c_void_p), ("yt", ctypes.c_void_p), ("m", ctypes.c_int),
                 ("x0s", ctypes.c_void_p), ("y0s", ctypes.c_void_p), ("hs", ctypes.c_void_p),
                 ("nblocks", ctypes.c_int), ("out0", ctypes.c_void_p), ("out1", ctypes.c_void_p),
-                ("source_id", ctypes.c_longlong)]
+                ("source_id", ctypes.c_longlong), ("order", ctypes.c_int), ("leaf", ctypes.c_int)]
 
 
 _lib = None
@@ -435,21 +435,22 @@ class Server:
             raise TreecodeError("tc_server_create failed: " + last_error())
         self._keep = {}
 
-    def submit_potential(self, theta, xs, ys, vs, xt, yt, precision=FP64, source_id=0):
+    def submit_potential(self, theta, xs, ys, vs, xt, yt, precision=FP64, source_id=0, order=0, leaf=0):
         xs, ys, vs, xt, yt = map(_f64, (xs, ys, vs, xt, yt))
         out = np.empty(len(xt))
         r = TcRequest(kind=0, precision=precision, theta=theta, xs=xs.ctypes.data, ys=ys.ctypes.data,
                       vs=vs.ctypes.data, n=len(xs), xt=xt.ctypes.data, yt=yt.ctypes.data, m=len(xt),
-                      out0=out.ctypes.data, source_id=source_id)
+                      out0=out.ctypes.data, source_id=source_id, order=order, leaf=leaf)
         return self._submit(r, (xs, ys, vs, xt, yt), (out,))
 
-    def submit_force(self, theta, xs, ys, vs, x0s, y0s, hs, precision=FP64, source_id=0):
+    def submit_force(self, theta, xs, ys, vs, x0s, y0s, hs, precision=FP64, source_id=0, order=0, leaf=0):
         xs, ys, vs, x0s, y0s, hs = map(_f64, (xs, ys, vs, x0s, y0s, hs))
         xr = np.empty(len(x0s) * 1024)
         yr = np.empty(len(x0s) * 1024)
         r = TcRequest(kind=1, precision=precision, theta=theta, xs=xs.ctypes.data, ys=ys.ctypes.data,
                       vs=vs.ctypes.data, n=len(xs), x0s=x0s.ctypes.data, y0s=y0s.ctypes.data, hs=hs.ctypes.data,
-                      nblocks=len(x0s), out0=xr.ctypes.data, out1=yr.ctypes.data, source_id=source_id)
+                      nblocks=len(x0s), out0=xr.ctypes.data, out1=yr.ctypes.data, source_id=source_id,
+                      order=order, leaf=leaf)
         return self._submit(r, (xs, ys, vs, x0s, y0s, hs), (xr, yr))
 
     def _submit(self, req, inputs, outputs):

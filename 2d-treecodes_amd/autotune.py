@@ -43,6 +43,22 @@ class Tuner:
         e = self.cache.get(bucket_key(kind, precision, theta, n, m))
         return (e["order"], e["leaf"]) if e else None
 
+    def nearest(self, kind, precision, theta, n, m, max_dist=3):
+        """(order, leaf) of the tuned bucket of the same kind, precision and theta
+        whose log2 N and log2 M are closest (sum of distances at most max_dist),
+        or None.  For a server: requests of every size, tuned on a few buckets."""
+        prefix = "%s/p%d/t%.2f/" % (kind, precision, theta)
+        ln, lm = math.log2(max(n, 1)), math.log2(max(m, 1))
+        best, best_d = None, None
+        for key, e in self.cache.items():
+            if not key.startswith(prefix):
+                continue
+            bn, bm = key[len(prefix):].split("/")
+            d = abs(ln - int(bn[1:])) + abs(lm - int(bm[1:]))
+            if d <= max_dist and (best_d is None or d < best_d):
+                best, best_d = (e["order"], e["leaf"]), d
+        return best
+
     def _save(self):
         if self.cache_path:
             tmp = self.cache_path + ".tmp"

@@ -45,12 +45,16 @@ struct Job {
     std::string err;
 };
 
+// The request's expansion order and leaf capacity (0 selects the reference's).
+int req_order(const tc_request &r) { return r.order > 0 ? r.order : (r.kind == TC_REQ_POTENTIAL ? 12 : 24); }
+int req_leaf(const tc_request &r) { return r.leaf > 0 ? r.leaf : (r.kind == TC_REQ_POTENTIAL ? 512 : 192); }
+
 // Upper estimate of the HBM a request occupies while it runs.
 size_t footprint(const tc_request &r)
 {
     const size_t n = (size_t)(r.n > 0 ? r.n : 0);
-    const int p = r.kind == TC_REQ_POTENTIAL ? 12 : 24;
-    const int leaf = r.kind == TC_REQ_POTENTIAL ? 512 : 192;
+    const int p = req_order(r);
+    const int leaf = req_leaf(r);
     const size_t nodes = 8 * n / (size_t)leaf + 64;
     size_t b = n * (24 + 32 + 8 + 8 + 8 + (r.precision ? 16 : 0));
     b += nodes * (64 + 32 + 24 + 16 * (size_t)p + (r.precision ? 8 * (size_t)p : 0));
@@ -68,7 +72,7 @@ struct tc_server {
     size_t budget = 0, in_use = 0, peak = 0;
     std::vector<tc_context *> ctx;
     std::vector<long long> res_src;
-    std::vector<int> res_kind, res_prec;
+    std::vector<int> res_kind, res_prec, res_order, res_leaf;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Job *> queue;
@@ -85,14 +89,19 @@ namespace {
 bool resident(const tc_server *s, int slot, const tc_request &r)
 {
     return r.source_id != 0 && s->res_src[slot] == r.source_id && s->res_kind[slot] == r.kind &&
-           s->res_prec[slot] == r.precision;
+           s->res_prec[slot] == r.precision && s->res_order[slot] == req_order(r) && s->res_leaf[slot] == req_leaf(r);
 }
 
 void run_job(tc_server *s, int slot, Job *j)
 {
     tc_context *c = s->ctx[slot];
     const tc_request &r = j->r;
+    const bool pot = r.kind == TC_REQ_POTENTIAL;
     int rc = tc_set_option(c, TC_OPT_PRECISION, r.precision);
+    if (rc == 0)
+        rc = tc_set_option(c, pot ? TC_OPT_POT_ORDER : TC_OPT_FORCE_ORDER, req_order(r));
+    if (rc == 0)
+        rc = tc_set_option(c, pot ? TC_OPT_POT_LEAF : TC_OPT_FORCE_LEAF, req_leaf(r));
     if (rc == 0) {
         if (r.kind == TC_REQ_POTENTIAL)
             rc = j->reused ? tc_potential_resident(c, r.theta, r.xt, r.yt, r.m, r.out0, 0)
@@ -146,6 +155,8 @@ void worker(tc_server *s, int slot)
             s->res_src[slot] = pick->r.source_id;
             s->res_kind[slot] = pick->r.kind;
             s->res_prec[slot] = pick->r.precision;
+            s->res_order[slot] = req_order(pick->r);
+            s->res_leaf[slot] = req_leaf(pick->r);
             s->completed++;
             s->reused += pick->reused;
         } else {
@@ -183,6 +194,8 @@ TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib
         s->ctx.push_back(c);
         s->res_src.push_back(0);
         s->res_kind.push_back(-1);
+        s->res_order.push_back(0);
+        s->res_leaf.push_back(0);
         s->res_prec.push_back(-1);
     }
     for (int i = 0; i < nslots; ++i)

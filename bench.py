@@ -241,18 +241,64 @@ def cpu_baseline(kind, data, theta, order, leaf, budget_s):
             "sample": sample + "; extrapolated to %d targets" % m}
 
 
-def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454):
+# Size buckets tuned for the served stream (FP32): potential N = M; force
+# (N, blocks).  With the nearest-bucket lookup (at most 3 in summed log2
+# distance) they cover the stream's request sizes.
+SERVE_POT_N = (1 << 11, 1 << 14, 1 << 17, 1 << 20)
+SERVE_FORCE = [(n, nb) for n in (1 << 14, 1 << 17, 1 << 20) for nb in (32, 256, 2048)]
+SERVE_THETAS = (0.3, 0.5, 0.7)
+SERVE_ERR = 2.5e-7  # a 4x margin under the metric's 1e-6 for sizes between buckets
+
+
+def tune_serving(cache_path):
+    """Autotune order and leaf capacity (FP32) for the served stream's size buckets."""
+    from treecodes2d_amd import autotune
+    tuner = autotune.Tuner(pkg, cache_path=cache_path or autotune.CACHE)
+    done = []
+    cases = [("potential", n, 0) for n in SERVE_POT_N] + [("force", n, nb) for n, nb in SERVE_FORCE]
+    for kind, n, nb in cases:
+        r = {"kind": kind, "n": n, "nblocks": nb, "gen": "gauss4", "src_seed": 4242 + n + nb, "tgt_seed": 77 + nb}
+        xs, ys, vs = workloads.request_sources(r)
+        tg = workloads.request_targets(r, xs, ys)
+        if kind == "potential":
+            data = dict(xs=xs, ys=ys, vs=vs, xt=tg[0], yt=tg[1])
+            grid = dict(orders=(6, 8, 10, 12, 16, 24), leaves=(64, 128, 256, 512))
+        else:
+            data = dict(xs=xs, ys=ys, vs=vs, x0s=tg[0], y0s=tg[1], hs=tg[2])
+            grid = dict(orders=(6, 8, 10, 12, 16, 24), leaves=(64, 128, 192, 384))
+        for theta in SERVE_THETAS:
+            res = tuner.tune(kind, data, theta=theta, precision=1, err_target=SERVE_ERR, reps=2, **grid)
+            log("  %s N=%d blocks=%d theta=%.1f: order %d leaf %d (%.3f ms, err %.1e)" % (
+                kind, n, nb, theta, res["order"], res["leaf"], res["ms"], res["err"]))
+            done.append({"kind": kind, "n": n, "blocks": nb, "theta": theta, "order": res["order"],
+                         "leaf": res["leaf"], "ms": res["ms"], "err": res["err"]})
+    return done
+
+
+def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned=True):
     """Replay the served mixed-size stream: Poisson arrivals, 70% potential / 30% force,
     25% of requests reuse the previous request's sources (the server keeps the tree).
-    Latency is measured by the server from submission to completion."""
+    With `tuned`, each request carries the order and leaf capacity of the nearest
+    autotuned size bucket (FP32); others use the reference's.  Latency is measured
+    by the server from submission to completion."""
     import threading
     reqs = workloads.request_stream(nreq, rate_hz=rate_hz, seed=seed)
     pools = workloads.request_pools(seed=seed)
+    tuner = None
+    if tuned and precision == 1:
+        from treecodes2d_amd import autotune
+        tuner = autotune.Tuner(pkg)
     prepared = []
+    ntuned = 0
     for r in reqs:
         xs, ys, vs = workloads.pool_sources(pools, r)
         tg = workloads.request_targets(r, xs, ys)
-        prepared.append((r, xs, ys, vs, tg))
+        ol = None
+        if tuner:
+            m = len(tg[0]) * (1 if r["kind"] == "potential" else 1024)
+            ol = tuner.nearest(r["kind"], 1, r["theta"], len(xs), m)
+        ntuned += ol is not None
+        prepared.append((r, xs, ys, vs, tg, ol or (0, 0)))
     srv = pkg.Server(slots=slots, hbm_budget_gib=budget_gib)
     info = [None] * nreq
     tickets = [None] * nreq
@@ -268,15 +314,17 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454):
     th = threading.Thread(target=waiter, daemon=True)
     th.start()
     t0 = time.perf_counter()
-    for i, (r, xs, ys, vs, tg) in enumerate(prepared):
+    for i, (r, xs, ys, vs, tg, (order, leaf)) in enumerate(prepared):
         now = time.perf_counter() - t0
         if r["arrival"] > now:
             time.sleep(r["arrival"] - now)
         sid = r["src_seed"]
         if r["kind"] == "potential":
-            tickets[i] = srv.submit_potential(r["theta"], xs, ys, vs, *tg, precision=precision, source_id=sid)
+            tickets[i] = srv.submit_potential(r["theta"], xs, ys, vs, *tg, precision=precision, source_id=sid,
+                                              order=order, leaf=leaf)
         else:
-            tickets[i] = srv.submit_force(r["theta"], xs, ys, vs, *tg, precision=precision, source_id=sid)
+            tickets[i] = srv.submit_force(r["theta"], xs, ys, vs, *tg, precision=precision, source_id=sid,
+                                          order=order, leaf=leaf)
     done.wait()
     makespan = time.perf_counter() - t0
     st = srv.stats()
@@ -288,7 +336,10 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454):
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_service_ms": float(svc.mean()), "makespan_s": makespan,
             "targets_per_s": targets / makespan, "reused_trees": int(st["reused"]),
-            "max_inflight": int(st["max_inflight"])}
+            "max_inflight": int(st["max_inflight"]),
+            "params": ("order / leaf of the nearest autotuned size bucket (2d-treecodes_amd/tuned.json) for %d of %d "
+                       "requests, the reference's for the rest" % (ntuned, nreq)) if tuner else
+                      "the reference's order / leaf (12 / 512 potential, 24 / 192 force)"}
 
 
 def main():
@@ -314,6 +365,10 @@ def main():
                     help="re-run the autotuner for order and leaf capacity per precision (rel err <= 5e-7)")
     ap.add_argument("--tune-cache", default="", metavar="PATH",
                     help="file the autotuner records its choices in (default: the package's tuned.json)")
+    ap.add_argument("--tune-serving", action="store_true",
+                    help="autotune the served stream's size buckets (FP32), record them and exit")
+    ap.add_argument("--serve-reference-params", action="store_true",
+                    help="serve every request with the reference's order / leaf instead of the tuned buckets")
     ap.add_argument("--reference-params", action="store_true",
                     help="use the reference's order/leaf (12/512 potential, 24/192 force) instead of tuned ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -325,6 +380,11 @@ def main():
         print("bench.py: this build measures one GPU; multi-GPU runs are not supported (--gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.tune_serving:
+        done = tune_serving(args.tune_cache)
+        print(json.dumps({"tuned_serving_buckets": done}))
+        return
 
     if args.copy_calibration:
         cctx = pkg.Context()
@@ -406,7 +466,8 @@ def main():
         log("served stream: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
         for c in cases.values():
             c.ctx.close()
-        serving = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, 64.0)
+        serving = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, 64.0,
+                               tuned=not args.serve_reference_params)
         log("  p50 %.2f ms p99 %.2f ms" % (serving["p50_ms"], serving["p99_ms"]))
     cpu = None
     if not args.no_cpu:

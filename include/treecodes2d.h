@@ -188,6 +188,9 @@ typedef struct tc_request {
     int nblocks;
     double *out0, *out1;              /* potential: out0[m]; force: out0/out1[1024*nblocks] */
     long long source_id;              /* nonzero: same id = same sources, the tree may be reused */
+    int order, leaf;                  /* expansion order and leaf capacity for this request, e.g. the
+                                       * autotuner's choice for its size bucket; 0 = the reference's
+                                       * (potential 12 / 512, force 24 / 192) */
 } tc_request;
 
 TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib);

@@ -1,0 +1,36 @@
+"""The autotuner's cache: bucket keys and the nearest-bucket lookup a server
+uses for request sizes that were not tuned (host logic, no GPU)."""
+import json
+
+
+def _tuner(pkg, tmp_path, entries):
+    from treecodes2d_amd import autotune
+    path = tmp_path / "tuned.json"
+    path.write_text(json.dumps(entries))
+    return autotune.Tuner(pkg, cache_path=str(path)), autotune
+
+
+def test_bucket_key_rounds_log2(pkg, tmp_path):
+    _, at = _tuner(pkg, tmp_path, {})
+    assert at.bucket_key("potential", 1, 0.5, 1 << 20, 1 << 20) == "potential/p1/t0.50/n20/m20"
+    assert at.bucket_key("force", 0, 0.3, 3000, 70000) == "force/p0/t0.30/n12/m16"
+
+
+def test_nearest_bucket(pkg, tmp_path):
+    e = lambda o, l: {"order": o, "leaf": l, "ms": 1.0, "err": 1e-7, "err_target": 2.5e-7}
+    tuner, _ = _tuner(pkg, tmp_path, {
+        "potential/p1/t0.50/n11/m11": e(8, 64), "potential/p1/t0.50/n17/m17": e(10, 128),
+        "potential/p1/t0.70/n17/m17": e(16, 128), "force/p1/t0.50/n17/m18": e(12, 192),
+        "potential/p0/t0.50/n17/m17": e(12, 64)})
+    # exact bucket, then nearest of the same kind / precision / theta
+    assert tuner.lookup("potential", 1, 0.5, 1 << 17, 1 << 17) == (10, 128)
+    assert tuner.nearest("potential", 1, 0.5, 1 << 16, 1 << 16) == (10, 128)
+    assert tuner.nearest("potential", 1, 0.5, 1 << 12, 1 << 12) == (8, 64)
+    assert tuner.nearest("potential", 1, 0.7, 1 << 17, 1 << 17) == (16, 128)
+    assert tuner.nearest("potential", 0, 0.5, 1 << 17, 1 << 17) == (12, 64)
+    assert tuner.nearest("force", 1, 0.5, 1 << 16, 1 << 18) == (12, 192)
+    # too far from every bucket, or no bucket for this theta: None (reference defaults)
+    assert tuner.nearest("potential", 1, 0.5, 1 << 14, 1 << 14) is None  # 3 + 3 from both buckets
+    assert tuner.nearest("potential", 1, 0.5, 1 << 24, 1 << 24) is None
+    assert tuner.nearest("potential", 1, 0.3, 1 << 17, 1 << 17) is None
+    assert tuner.nearest("force", 1, 0.5, 1 << 21, 1 << 14) is None

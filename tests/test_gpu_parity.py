@@ -27,7 +27,8 @@ def _device_sort(pkg, ctx, x, y):
     return box.numpy(), keys.numpy(), perm.numpy()
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 100003])
+# up to 16384 points the sort runs in one workgroup, above it in tiles of 4096
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 16384, 16385, 100003])
 def test_morton_sort_bit_exact(pkg, oracle, wl, ctx64, n):
     st = wl.Stream(100 + n)
     x, y = st.uniform(n), st.uniform(n)

@@ -85,3 +85,29 @@ def test_budget_rejects_oversized(pkg, wl):
         assert np.all(np.isfinite(out))
     finally:
         srv.close()
+
+
+def test_request_order_and_leaf(pkg, wl, oracle):
+    # per-request expansion order and leaf capacity (as a server sets them from
+    # the autotuner): results equal a context solve with the same options, and
+    # a resident tree built with other parameters is not reused
+    xs, ys, vs, xt, yt = wl.uniform_potential(30000, seed=44)
+    srv = pkg.Server(slots=1, hbm_budget_gib=4.0)
+    try:
+        t1 = srv.submit_potential(0.5, xs, ys, vs, xt, yt, source_id=9, order=8, leaf=64)
+        t2 = srv.submit_potential(0.5, xs, ys, vs, xt, yt, source_id=9)
+        t3 = srv.submit_potential(0.5, xs, ys, vs, xt[::3], yt[::3], source_id=9)
+        a, ia = srv.wait(t1)
+        b, ib = srv.wait(t2)
+        c, ic = srv.wait(t3)
+    finally:
+        srv.close()
+    assert not ia["reused"] and not ib["reused"] and ic["reused"]
+    ctx = pkg.Context(pot_order=8, pot_leaf=64)
+    assert np.array_equal(a, ctx.potential(0.5, xs, ys, vs, xt, yt))
+    ctx.close()
+    ref, _ = oracle.potential(0.5, xs, ys, vs, xt, yt)
+    assert rel_l2(b, ref) <= 1e-12
+    assert np.array_equal(b[::3], c)
+    ref8, _ = oracle.potential(0.5, xs, ys, vs, xt, yt, order=8, leafcap=64)
+    assert rel_l2(a, ref8) <= 1e-12
