@@ -17,6 +17,7 @@ constexpr int RS_NT = 256;     // radix sort block size (4 waves)
 constexpr int RS_ITEMS = 16;   // keys per thread per tile
 constexpr int RS_TILE = RS_NT * RS_ITEMS;
 constexpr int RS_BINS = 256;   // 8-bit digits: passes at bits 0, 8, 16, 24 (30-bit keys)
+constexpr int RS_DIRECT_TILES = 64;  // up to this many tiles, no separate scan per pass
 constexpr int SCAN_NT = 1024;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_CHUNK = SCAN_NT * SCAN_ITEMS;
@@ -291,9 +292,12 @@ __global__ __launch_bounds__(RS_NT) void k_rs_hist(const uint32_t *keys, int n, 
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// offs: the scanned digit-major table (global start of each (digit, tile));
+// with offs == nullptr the block derives its starts from the tile histograms
+// `hist` itself (few tiles: the whole table is a few KiB).
 __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                        uint32_t *vout, int n, int shift, const uint32_t *offs,
-                                                       int ntiles)
+                                                       const uint32_t *hist, int ntiles)
 {
     constexpr int NW = RS_NT / 64, WITEMS = 64 * RS_ITEMS;  // waves per tile, keys per wave
     __shared__ uint32_t s_k[RS_TILE];
@@ -308,7 +312,18 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     const int cnt = min(RS_TILE, n - base);
     for (int d = lane; d < RS_BINS; d += 64)
         s_wrun[wid][d] = 0;
-    s_gbase[tid] = offs[tid * ntiles + blockIdx.x];
+    if (offs) {
+        s_gbase[tid] = offs[tid * ntiles + blockIdx.x];
+    } else {
+        // digit tid: keys of smaller digits (all tiles) + this digit in earlier tiles
+        uint32_t tot = 0, pre = 0;
+        for (int t = 0; t < ntiles; ++t) {
+            const uint32_t c = hist[tid * ntiles + t];
+            pre += t < (int)blockIdx.x ? c : 0u;
+            tot += c;
+        }
+        s_gbase[tid] = block_scan_excl<RS_NT>(tot, s_w, nullptr) + pre;
+    }
     tc_wave_sync();
 
     // Each wave ranks its own contiguous 1024 keys (item j of lane l is key
@@ -1266,12 +1281,16 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
     uint32_t *vin[4] = {nullptr, ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>()};
     uint32_t *kout[4] = {ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>(), keys_out};
     uint32_t *vout[4] = {ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>(), perm_out};
+    // up to RS_DIRECT_TILES tiles the scatter blocks read the histograms
+    // themselves and no scan is launched
+    const bool direct = ntiles <= RS_DIRECT_TILES;
     for (int pass = 0; pass < 4; ++pass) {
         const int shift = 8 * pass;
         k_rs_hist<<<ntiles, RS_NT, 0, st>>>(kin[pass], n, shift, hist, ntiles);
-        tc_scan_exclusive_u32(hist, offs, RS_BINS * ntiles, nullptr, ws, st);
-        k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift, offs,
-                                                ntiles);
+        if (!direct)
+            tc_scan_exclusive_u32(hist, offs, RS_BINS * ntiles, nullptr, ws, st);
+        k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift,
+                                                direct ? nullptr : offs, hist, ntiles);
         TC_HIP(hipGetLastError());
     }
 }
