@@ -32,6 +32,11 @@ struct tc_context {
     int tree_precision = 0;
     TcTree tree;
     TcScratch ws;
+    // potential targets: sorted in their own scratch on a second stream while
+    // the source tree builds (both streams at the default priority)
+    TcScratch tws;
+    hipStream_t tstream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     DevBuf io, cnt;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_valid = false, eval_has_tsort = false;
@@ -224,10 +229,10 @@ void check_resident(tc_context *c, int kind)
 }
 
 void eval_potential(tc_context *c, double theta, const double *xt, const double *yt, int m, double *out,
-                    unsigned long long *cnt)
+                    unsigned long long *cnt, bool sort_targets)
 {
-    tc_eval_potential(c->tree, theta, xt, yt, m, out, c->precision, cnt, c->ws, c->stream,
-                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+    tc_eval_potential(c->tree, theta, xt, yt, m, out, c->precision, cnt, c->tws, c->stream,
+                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets);
 }
 
 int run_potential(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
@@ -267,17 +272,36 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
         }
         unsigned long long *cnt = prepare_counters(c);
-        record(c, 0);
-        if (rebuild)
-            build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL);
-        record(c, 1);
-        eval_potential(c, theta, dxt, dyt, m, dout, cnt);
+        if (rebuild) {
+            // the target sort (one workgroup for small m) runs beside the
+            // source build; the evaluation waits for both
+            TC_HIP(hipEventRecord(c->fork_ev, c->stream));
+            TC_HIP(hipStreamWaitEvent(c->tstream, c->fork_ev, 0));
+            tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
+            TC_HIP(hipEventRecord(c->join_ev, c->tstream));
+            record(c, 0);
+            try {
+                build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL);
+            } catch (...) {
+                // later work on the context stream (copies into the staging
+                // buffers) must not overtake the sort still reading them
+                (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
+                throw;
+            }
+            record(c, 1);
+            TC_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+            eval_potential(c, theta, dxt, dyt, m, dout, cnt, false);
+        } else {
+            record(c, 0);
+            record(c, 1);
+            eval_potential(c, theta, dxt, dyt, m, dout, cnt, true);
+        }
         c->ev_valid = c->profile;
         c->eval_has_tsort = true;
         if (!device_ptrs) {
             // the target sort's bounding-box pass flags non-finite coordinates
             uint32_t bad = 0;
-            TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->ws, m), sizeof(bad), hipMemcpyDeviceToHost,
+            TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->tws, m), sizeof(bad), hipMemcpyDeviceToHost,
                                   c->stream));
             copy_out(c, dout, out, m);
             if (bad)
@@ -458,8 +482,11 @@ TC_API tc_context *tc_context_create(int device)
             throw TcError(-1, "treecode: device " + std::to_string(device) + " out of range");
         TC_HIP(hipSetDevice(device));
         TC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        TC_HIP(hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking));
         for (auto &e : c->ev)
             TC_HIP(hipEventCreate(&e));
+        TC_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+        TC_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
         ensure_binomials(device, c->stream);
     } catch (const TcError &e) {
         set_error(e.what());
@@ -474,10 +501,13 @@ TC_API void tc_context_destroy(tc_context *c)
     if (!c)
         return;
     (void)hipSetDevice(c->device);
+    if (c->tstream)
+        (void)hipStreamSynchronize(c->tstream);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
     c->tree.release();
     c->ws.release();
+    c->tws.release();
     c->io.release();
     c->cnt.release();
     for (auto &e : c->ev)
@@ -489,6 +519,11 @@ TC_API void tc_context_destroy(tc_context *c)
         if (c->stage_ev[b])
             (void)hipEventDestroy(c->stage_ev[b]);
     }
+    for (hipEvent_t e : {c->fork_ev, c->join_ev})
+        if (e)
+            (void)hipEventDestroy(e);
+    if (c->tstream)
+        (void)hipStreamDestroy(c->tstream);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

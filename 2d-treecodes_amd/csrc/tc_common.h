@@ -233,9 +233,11 @@ void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, h
 // Device flag (uint32, nonzero = a non-finite coordinate) set by the last
 // tc_sort_targets call on m targets; valid until the next bounding-box pass.
 const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m);
+// sort_targets false: tc_sort_targets already ran on ws for these targets
+// (e.g. on another stream, ordered before st).
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
-                       hipEvent_t ev0, hipEvent_t ev1);
+                       hipEvent_t ev0, hipEvent_t ev1, bool sort_targets = true);
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
                    TcScratch &ws, int max_chunks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);

@@ -1504,13 +1504,14 @@ void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n,
 
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
-                       hipEvent_t ev0, hipEvent_t ev1)
+                       hipEvent_t ev0, hipEvent_t ev1, bool sort_targets)
 {
     if (m <= 0)
         return;
     if (precision == 1 && !t.has_fp32)
         throw TcError(-3, "tc_eval_potential: tree was built without FP32 mirrors");
-    tc_sort_targets(xt, yt, m, ws, st);
+    if (sort_targets)
+        tc_sort_targets(xt, yt, m, ws, st);
     const double th2 = theta * theta;
     const uint32_t *tperm = ws.tperm.as<uint32_t>();
     if (ev0)
