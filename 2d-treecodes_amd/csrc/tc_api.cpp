@@ -300,11 +300,11 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
         c->eval_has_tsort = true;
         if (!device_ptrs) {
             // the target sort's bounding-box pass flags non-finite coordinates
-            uint32_t bad = 0;
-            TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->tws, m), sizeof(bad), hipMemcpyDeviceToHost,
+            uint32_t *bad = static_cast<uint32_t *>(c->tws.host_pinned());
+            TC_HIP(hipMemcpyAsync(bad, tc_targets_bad_flag(c->tws, m), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
             copy_out(c, dout, out, m);
-            if (bad)
+            if (*bad)
                 throw TcError(-4, "treecode_potential: non-finite target coordinate");
         }
     });
@@ -416,12 +416,12 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
                               c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
         if (!device_ptrs) {
-            uint32_t bad = 0;
-            TC_HIP(hipMemcpyAsync(&bad, tc_targets_bad_flag(c->ws, m), sizeof(bad), hipMemcpyDeviceToHost,
+            uint32_t *bad = static_cast<uint32_t *>(c->ws.host_pinned());
+            TC_HIP(hipMemcpyAsync(bad, tc_targets_bad_flag(c->ws, m), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
             copy_out(c, dxr, xr, m);
             copy_out(c, dyr, yr, m);
-            if (bad)
+            if (*bad)
                 throw TcError(-4, "treecode_force: non-finite target coordinate");
         }
     });

@@ -9,6 +9,7 @@
 #include "tc_common.h"
 
 #include <cfloat>
+#include <cstring>
 #include <vector>
 
 namespace {
@@ -624,8 +625,13 @@ __global__ __launch_bounds__(256) void k_int_count(const uint32_t *keys, int n, 
             s_cnt[0][threadIdx.x] + s_cnt[1][threadIdx.x] + s_cnt[2][threadIdx.x] + s_cnt[3][threadIdx.x];
 }
 
+// Census record read back by the host in one copy: level starts, internal
+// count, non-finite flag, then (at int offset INFO_BOX) the bounding box.
+constexpr int INFO_BOX = 20;
+constexpr size_t INFO_BYTES = sizeof(int) * INFO_BOX + sizeof(double) * 4;
+
 __global__ void k_level_starts(const uint32_t *offs, int nblk, const uint32_t *total, const uint32_t *bad,
-                               int *info)
+                               const double *box, int *info)
 {
     const int L = threadIdx.x;
     if (L < TC_LMAX)
@@ -634,6 +640,8 @@ __global__ void k_level_starts(const uint32_t *offs, int nblk, const uint32_t *t
         info[TC_LMAX] = (int)*total;
     if (L == TC_LMAX + 1)
         info[TC_LMAX + 1] = (int)*bad;
+    if (L >= 32 && L < 36)
+        reinterpret_cast<double *>(info + INFO_BOX)[L - 32] = box[L - 32];
 }
 
 __global__ __launch_bounds__(256) void k_int_emit(const uint32_t *keys, int n, int cap, const uint32_t *offs,
@@ -1368,12 +1376,15 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     int *info = ws.info.as<int>();
     k_int_count<<<nblk, 256, 0, st>>>(keys, n, leafcap, cnt, nblk);
     tc_scan_exclusive_u32(cnt, offs, TC_LMAX * nblk, total, ws, st);
-    k_level_starts<<<1, 64, 0, st>>>(offs, nblk, total, tc_bbox_bad_flag(ws, n), info);
+    k_level_starts<<<1, 64, 0, st>>>(offs, nblk, total, tc_bbox_bad_flag(ws, n), box, info);
     TC_HIP(hipGetLastError());
-    int hinfo[TC_LMAX + 2];
-    TC_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
-    TC_HIP(hipMemcpyAsync(t.box, box, sizeof(t.box), hipMemcpyDeviceToHost, st));
+    static_assert(INFO_BYTES <= sizeof(int) * 32 && INFO_BYTES <= TcScratch::PINNED_BYTES, "census record size");
+    char *hp = static_cast<char *>(ws.host_pinned());
+    TC_HIP(hipMemcpyAsync(hp, info, INFO_BYTES, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
+    int hinfo[TC_LMAX + 2];
+    std::memcpy(hinfo, hp, sizeof(hinfo));
+    std::memcpy(t.box, hp + sizeof(int) * INFO_BOX, sizeof(t.box));
     if (hinfo[TC_LMAX + 1])
         throw TcError(-4, "treecode: non-finite source coordinate");
     const int T = hinfo[TC_LMAX];

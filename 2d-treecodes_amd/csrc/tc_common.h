@@ -207,11 +207,24 @@ struct TcTree {
 struct TcScratch {
     DevBuf k2, v2, hist, part, scan_tmp, istart, ilevel, info, tkeys, tperm, tsorted, bbox_part, ints, packed, work,
         flist;
+    // pinned host memory for the per-solve readbacks (tree census, flags): a
+    // copy into pageable memory costs a staged transfer of its own
+    void *pinned = nullptr;
+    static constexpr size_t PINNED_BYTES = 256;
+    void *host_pinned()
+    {
+        if (!pinned)
+            TC_HIP(hipHostMalloc(&pinned, PINNED_BYTES, hipHostMallocDefault));
+        return pinned;
+    }
     void release()
     {
         for (DevBuf *b : {&k2, &v2, &hist, &part, &scan_tmp, &istart, &ilevel, &info, &tkeys, &tperm, &tsorted,
                           &bbox_part, &ints, &packed, &work, &flist})
             b->release();
+        if (pinned)
+            (void)hipHostFree(pinned);
+        pinned = nullptr;
     }
 };
 
