@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <limits>
 #include <mutex>
 #include <string>
@@ -192,6 +193,74 @@ void copy_out(tc_context *c, const double *d, double *h, long long n)
     TC_HIP(hipStreamSynchronize(c->stream));
 }
 
+struct HostIn {
+    const double *h;
+    long long n;
+};
+struct HostOut {
+    double *h;
+    long long n;
+};
+
+inline size_t arr_bytes(long long n) { return sizeof(double) * (size_t)(n > 0 ? n : 0); }
+
+// Host arrays whose device copies lie back to back from d (in the io buffer).
+// Together up to one staging buffer, they are packed into pinned memory and
+// sent with one DMA (a copy from pageable memory costs a staged transfer of
+// its own); larger sets go array by array.
+void copy_in_all(tc_context *c, double *d, std::initializer_list<HostIn> arrs)
+{
+    size_t total = 0;
+    for (const HostIn &a : arrs)
+        total += arr_bytes(a.n);
+    if (!total)
+        return;
+    if (total > STAGE_BYTES) {
+        for (const HostIn &a : arrs) {
+            copy_in(c, a.h, d, a.n);
+            d += a.n > 0 ? a.n : 0;
+        }
+        return;
+    }
+    stage_ready(c);
+    stage_wait(c, 0);
+    char *s = static_cast<char *>(c->stage[0]);
+    for (const HostIn &a : arrs) {
+        par_memcpy(s, a.h, arr_bytes(a.n));
+        s += arr_bytes(a.n);
+    }
+    TC_HIP(hipMemcpyAsync(d, c->stage[0], total, hipMemcpyHostToDevice, c->stream));
+    TC_HIP(hipEventRecord(c->stage_ev[0], c->stream));
+    c->stage_busy[0] = true;
+}
+
+// Device block from d (arrays back to back) -> host arrays; returns with the
+// data in place (synchronises the stream).  Up to one staging buffer in total
+// it is one DMA into pinned memory.
+void copy_out_all(tc_context *c, const double *d, std::initializer_list<HostOut> arrs)
+{
+    size_t total = 0;
+    for (const HostOut &a : arrs)
+        total += arr_bytes(a.n);
+    if (total > STAGE_BYTES || !total) {
+        for (const HostOut &a : arrs) {
+            copy_out(c, d, a.h, a.n);
+            d += a.n > 0 ? a.n : 0;
+        }
+        TC_HIP(hipStreamSynchronize(c->stream));
+        return;
+    }
+    stage_ready(c);
+    stage_wait(c, 0);
+    TC_HIP(hipMemcpyAsync(c->stage[0], d, total, hipMemcpyDeviceToHost, c->stream));
+    TC_HIP(hipStreamSynchronize(c->stream));
+    const char *s = static_cast<const char *>(c->stage[0]);
+    for (const HostOut &a : arrs) {
+        par_memcpy(a.h, s, arr_bytes(a.n));
+        s += arr_bytes(a.n);
+    }
+}
+
 unsigned long long *prepare_counters(tc_context *c)
 {
     if (!c->count)
@@ -262,13 +331,10 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             double *bxs = base, *bys = base + ns, *bvs = base + 2 * ns;
             double *bxt = base + 3 * ns, *byt = bxt + m;
             dout = byt + m;
-            if (rebuild) {
-                copy_in(c, xs, bxs, n);
-                copy_in(c, ys, bys, n);
-                copy_in(c, vs, bvs, n);
-            }
-            copy_in(c, xt, bxt, m);
-            copy_in(c, yt, byt, m);
+            if (rebuild)
+                copy_in_all(c, bxs, {{xs, n}, {ys, n}, {vs, n}, {xt, m}, {yt, m}});
+            else
+                copy_in_all(c, bxt, {{xt, m}, {yt, m}});
             dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
         }
         unsigned long long *cnt = prepare_counters(c);
@@ -303,7 +369,7 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             uint32_t *bad = static_cast<uint32_t *>(c->tws.host_pinned());
             TC_HIP(hipMemcpyAsync(bad, tc_targets_bad_flag(c->tws, m), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
-            copy_out(c, dout, out, m);
+            copy_out_all(c, dout, {{out, m}});
             if (*bad)
                 throw TcError(-4, "treecode_potential: non-finite target coordinate");
         }
@@ -346,14 +412,10 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
             double *bx0 = base + 3 * ns, *by0 = bx0 + nb, *bh = by0 + nb;
             dxr = bh + nb;
             dyr = dxr + nt;
-            if (rebuild) {
-                copy_in(c, xs, bxs, n);
-                copy_in(c, ys, bys, n);
-                copy_in(c, vs, bvs, n);
-            }
-            copy_in(c, x0s, bx0, nb);
-            copy_in(c, y0s, by0, nb);
-            copy_in(c, hs, bh, nb);
+            if (rebuild)
+                copy_in_all(c, bxs, {{xs, n}, {ys, n}, {vs, n}, {x0s, nb}, {y0s, nb}, {hs, nb}});
+            else
+                copy_in_all(c, bx0, {{x0s, nb}, {y0s, nb}, {hs, nb}});
             dxs = bxs; dys = bys; dvs = bvs; dx0 = bx0; dy0 = by0; dh = bh;
         }
         unsigned long long *cnt = prepare_counters(c);
@@ -366,10 +428,8 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
                       c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
         c->eval_has_tsort = false;
-        if (!device_ptrs) {
-            copy_out(c, dxr, xr, (long long)nt);
-            copy_out(c, dyr, yr, (long long)nt);
-        }
+        if (!device_ptrs)
+            copy_out_all(c, dxr, {{xr, (long long)nt}, {yr, (long long)nt}});
     });
 }
 
@@ -401,11 +461,7 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
             double *bxt = base + 3 * (size_t)n, *byt = bxt + m;
             dxr = byt + m;
             dyr = dxr + m;
-            copy_in(c, xs, bxs, n);
-            copy_in(c, ys, bys, n);
-            copy_in(c, vs, bvs, n);
-            copy_in(c, xt, bxt, m);
-            copy_in(c, yt, byt, m);
+            copy_in_all(c, bxs, {{xs, n}, {ys, n}, {vs, n}, {xt, m}, {yt, m}});
             dxs = bxs; dys = bys; dvs = bvs; dxt = bxt; dyt = byt;
         }
         unsigned long long *cnt = prepare_counters(c);
@@ -419,8 +475,7 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
             uint32_t *bad = static_cast<uint32_t *>(c->ws.host_pinned());
             TC_HIP(hipMemcpyAsync(bad, tc_targets_bad_flag(c->ws, m), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
-            copy_out(c, dxr, xr, m);
-            copy_out(c, dyr, yr, m);
+            copy_out_all(c, dxr, {{xr, m}, {yr, m}});
             if (*bad)
                 throw TcError(-4, "treecode_force: non-finite target coordinate");
         }
