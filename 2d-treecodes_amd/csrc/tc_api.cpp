@@ -343,8 +343,16 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             // source build; the evaluation waits for both
             TC_HIP(hipEventRecord(c->fork_ev, c->stream));
             TC_HIP(hipStreamWaitEvent(c->tstream, c->fork_ev, 0));
-            tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
-            TC_HIP(hipEventRecord(c->join_ev, c->tstream));
+            try {
+                tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
+                TC_HIP(hipEventRecord(c->join_ev, c->tstream));
+            } catch (...) {
+                // whatever part of the sort was queued stays ordered before
+                // later work on the context stream
+                (void)hipEventRecord(c->join_ev, c->tstream);
+                (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
+                throw;
+            }
             record(c, 0);
             try {
                 build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL);
