@@ -357,8 +357,8 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             try {
                 build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL);
             } catch (...) {
-                // later work on the context stream (copies into the staging
-                // buffers) must not overtake the sort still reading them
+                // later work on the context stream (the next call's copies
+                // into the io buffer) must not overtake the sort reading it
                 (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
                 throw;
             }
