@@ -695,11 +695,18 @@ __global__ __launch_bounds__(256) void k_int_emit(const uint32_t *keys, int n, i
     }
 }
 
-__global__ void k_root_init(int n, int2 *se, int *level, int2 *cell)
+// Clears every node's child field (leaves keep 0; k_int_nodes then sets the
+// internal ones) and writes the root's range, level and cell.
+__global__ __launch_bounds__(256) void k_topo_init(uint32_t *child, int nn, int n, int2 *se, int *level,
+                                                   int2 *cell)
 {
-    se[0] = make_int2(0, n);
-    level[0] = 0;
-    cell[0] = make_int2(0, 0);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nn; i += gridDim.x * 256)
+        child[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        se[0] = make_int2(0, n);
+        level[0] = 0;
+        cell[0] = make_int2(0, 0);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_int_nodes(const uint32_t *keys, int n, int cap, const int *istart,
@@ -1411,8 +1418,7 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     int *lev = reinterpret_cast<int *>(se + nn);
     int2 *cell = reinterpret_cast<int2 *>(lev + ((nn + 1) & ~1));
 
-    tc_fill_u32(child, 0u, (size_t)nn, st);
-    k_root_init<<<1, 1, 0, st>>>(n, se, lev, cell);
+    k_topo_init<<<std::min(tiles_of(nn, 256), 4096), 256, 0, st>>>(child, nn, n, se, lev, cell);
     if (T > 0) {
         k_int_emit<<<nblk, 256, 0, st>>>(keys, n, leafcap, offs, nblk, ws.istart.as<int>(), ws.ilevel.as<int>());
         k_int_nodes<<<tiles_of(T, 256), 256, 0, st>>>(keys, n, leafcap, ws.istart.as<int>(), ws.ilevel.as<int>(),
