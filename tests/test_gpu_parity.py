@@ -270,6 +270,37 @@ def test_host_arrays_staged_copies(pkg, wl, ctx32):
     assert np.array_equal(fx, xr.numpy()) and np.array_equal(fy, yr.numpy())
 
 
+def test_host_arrays_packed_copies(pkg, wl, ctx32):
+    # a call whose host inputs fit one 16 MiB staging buffer together sends them in one DMA
+    # (and its outputs come back in one): results equal the device-array path, for small
+    # calls and for ones just under the buffer (5 x 400,000 doubles = 16.0 MB in; force
+    # outputs 2 x 921,600 doubles = 14.7 MB out)
+    for n in (3000, 400_000):
+        xs, ys, vs, xt, yt = wl.uniform_potential(n, seed=73)
+        host = ctx32.potential(0.5, xs, ys, vs, xt, yt)
+        d = {k: ctx32.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt).items()}
+        out = ctx32.empty(len(xt))
+        ctx32.potential(0.5, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
+        ctx32.synchronize()
+        assert np.array_equal(host, out.numpy()), n
+        res = ctx32.potential_resident(0.5, xt, yt)
+        assert np.array_equal(host, res), n
+        fx, fy = ctx32.force_targets(0.5, xs, ys, vs, xt, yt)
+        gx, gy = ctx32.empty(len(xt)), ctx32.empty(len(xt))
+        ctx32.force_targets(0.5, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], xr=gx, yr=gy, device=True)
+        ctx32.synchronize()
+        assert np.array_equal(fx, gx.numpy()) and np.array_equal(fy, gy.numpy()), n
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(200_000, nbx=32, h=1.0 / 2048, seed=74)
+    for nb in (7, 900):
+        bx, by, bh = x0s[:nb].copy(), y0s[:nb].copy(), hs[:nb].copy()
+        fx, fy = ctx32.force(0.5, xs, ys, vs, bx, by, bh)
+        d = {k: ctx32.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, x0s=bx, y0s=by, hs=bh).items()}
+        xr, yr = ctx32.empty(nb * 1024), ctx32.empty(nb * 1024)
+        ctx32.force(0.5, d["xs"], d["ys"], d["vs"], d["x0s"], d["y0s"], d["hs"], xr=xr, yr=yr, device=True)
+        ctx32.synchronize()
+        assert np.array_equal(fx, xr.numpy()) and np.array_equal(fy, yr.numpy()), nb
+
+
 def test_deterministic_and_resident(pkg, wl, ctx64):
     xs, ys, vs, xt, yt = wl.uniform_potential(65536, seed=14)
     a = ctx64.potential(0.5, xs, ys, vs, xt, yt)
