@@ -293,6 +293,24 @@ __global__ __launch_bounds__(RS_NT) void k_rs_hist(const uint32_t *keys, int n, 
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Lanes of the wave holding the same 8-bit digit d as this lane (among lanes
+// with ok set): a lane differs from this one where, for some bit, its ballot
+// bit differs from this lane's own bit.  The per-lane 0 / ~0 of each bit is a
+// sign-extended bit field, so each bit costs a ballot and four 32-bit logic ops.
+TC_DEV unsigned long long wave_digit_peers(uint32_t d, bool ok)
+{
+    uint32_t diff_lo = 0, diff_hi = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const unsigned long long bb = tc_ballot((d >> b) & 1u);
+        const uint32_t own = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        diff_lo |= (uint32_t)bb ^ own;
+        diff_hi |= (uint32_t)(bb >> 32) ^ own;
+    }
+    const unsigned long long diff = ((unsigned long long)diff_hi << 32) | diff_lo;
+    return tc_ballot(ok) & ~diff;
+}
+
 // offs: the scanned digit-major table (global start of each (digit, tile));
 // with offs == nullptr the block derives its starts from the tile histograms
 // `hist` itself (few tiles: the whole table is a few KiB).
@@ -344,12 +362,7 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     for (int j = 0; j < RS_ITEMS; ++j) {
         const bool ok = base + wid * WITEMS + j * 64 + lane < n;
         const uint32_t d = (k[j] >> shift) & (RS_BINS - 1);
-        unsigned long long peers = tc_ballot(ok);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const unsigned long long bb = tc_ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bb : ~bb;
-        }
+        const unsigned long long peers = wave_digit_peers(d, ok);
         const uint32_t lower = __popcll(peers & tc_lanemask_lt());
         const uint32_t run = ok ? s_wrun[wid][d] : 0u;
         tc_wave_sync();
@@ -504,12 +517,7 @@ __global__ __launch_bounds__(SS_NT) void k_sort_small(const double *__restrict__
         for (int j = 0; j < SS_ITEMS; ++j) {
             const bool ok = wid * WITEMS + j * 64 + lane < n;
             const uint32_t d = (k[j] >> shift) & (RS_BINS - 1);
-            unsigned long long peers = tc_ballot(ok);
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const unsigned long long bb = tc_ballot((d >> b) & 1u);
-                peers &= ((d >> b) & 1u) ? bb : ~bb;
-            }
+            const unsigned long long peers = wave_digit_peers(d, ok);
             const uint32_t lower = __popcll(peers & tc_lanemask_lt());
             const uint32_t run = ok ? s_wrun[wid][d] : 0u;
             tc_wave_sync();
