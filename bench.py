@@ -480,7 +480,7 @@ def main():
         if args.tune:
             from treecodes2d_amd import autotune
             log("autotuning", prec)
-            grid = dict(orders=(8, 10, 12, 14), leaves=(64, 128, 256, 512)) if kind == "potential" else \
+            grid = dict(orders=(8, 10, 12, 14), leaves=(64, 128, 192, 256, 384, 512)) if kind == "potential" else \
                 dict(orders=(8, 10, 12, 16), leaves=(32, 64, 128, 192, 384))
             tuner = autotune.Tuner(pkg, cache_path=args.tune_cache or autotune.CACHE)
             res = tuner.tune(kind, data, theta=args.theta, precision=1 if prec == "f32" else 0,
