@@ -318,11 +318,18 @@ def test_deterministic_and_resident(pkg, wl, ctx64):
 def test_empty_and_error_paths(pkg, wl, ctx64):
     out = ctx64.potential(0.5, np.zeros(0), np.zeros(0), np.zeros(0), np.ones(5), np.ones(5))
     assert np.array_equal(out, np.zeros(5))
-    xs, ys, vs, xt, yt = wl.uniform_potential(1000, seed=15)
-    xs[10] = np.nan
-    with pytest.raises(pkg.TreecodeError):
-        ctx64.potential(0.5, xs, ys, vs, xt, yt)
-    # the context stays usable after an error
+    # a failed build (the target sort already queued on the context's second stream) leaves the
+    # context usable, and the next solve equals one made before the failure; 40,000 points use
+    # the tiled sorts, 1000 the one-workgroup ones
+    for n in (40_000, 1000):
+        xs, ys, vs, xt, yt = wl.uniform_potential(n, seed=15)
+        before = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+        x10 = xs[10]
+        xs[10] = np.nan
+        with pytest.raises(pkg.TreecodeError):
+            ctx64.potential(0.5, xs, ys, vs, xt, yt)
+        xs[10] = x10
+        assert np.array_equal(ctx64.potential(0.5, xs, ys, vs, xt, yt), before), n
     xs[10] = 0.5
     ctx64.potential(0.5, xs, ys, vs, xt, yt)
     # non-finite targets are reported too (flagged on the device by the target sort)
