@@ -9,6 +9,12 @@
 #include "tc_common.h"
 
 #include "../../include/treecodes2d.h"
+#ifdef TREECODES2D_FLOAT_DROPIN
+// the float build exports the drop-in pair with the float signatures of the
+// reference's real=float headers
+#include "../../include/float/2d-treecodes-force.h"
+#include "../../include/float/2d-treecodes-potential.h"
+#endif
 
 #include <cmath>
 #include <cstdio>
@@ -280,6 +286,12 @@ void build(tc_context *c, const double *xs, const double *ys, const double *vs, 
 {
     const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
     const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
+    // the resident tree is invalid from the first array tc_build_tree
+    // overwrites until it returns: a build that throws part-way (non-finite
+    // source, failed allocation) leaves no tree a resident call could use
+    c->tree_kind = TC_KIND_NONE;
+    c->tree.nnodes = 0;
+    c->tree.has_fp32 = false;
     tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws, c->stream);
     c->tree_kind = kind;
     c->tree_precision = c->precision;
@@ -526,6 +538,50 @@ void report_dropin_failure(const char *fn)
     }
 }
 
+// The float drop-in pair (the reference's real=float build, libraries.Makefile:13):
+// float arrays are widened, solved on the FP32-bulk path of a second default
+// context, and narrowed back.
+void dropin_potential_f32(const char *fn, const float theta, const float *const xsources,
+                          const float *const ysources, const float *const sourcevalues, const int nsources,
+                          const float *const xtargets, const float *const ytargets, const int ntargets,
+                          float *const targetvalues)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    const size_t n = nsources > 0 ? (size_t)nsources : 0, m = ntargets > 0 ? (size_t)ntargets : 0;
+    std::vector<double> xs = widen(xsources, n), ys = widen(ysources, n), vs = widen(sourcevalues, n);
+    std::vector<double> xt = widen(xtargets, m), yt = widen(ytargets, m), out(m);
+    tc_context *c = default_context_f32();
+    int rc = c ? tc_potential(c, (double)theta, xs.data(), ys.data(), vs.data(), nsources, xt.data(), yt.data(),
+                              ntargets, out.data(), 0)
+               : -1;
+    if (rc)
+        report_dropin_failure(fn);
+    for (size_t i = 0; i < m; ++i)
+        targetvalues[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)out[i];
+}
+
+void dropin_force_f32(const char *fn, const float theta, const float *const xsources, const float *const ysources,
+                      const float *const sourcevalues, const int nsources, const float *const x0s,
+                      const float *const y0s, const float *const hs, const int nblocks, float *const xresults,
+                      float *const yresults)
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    const size_t n = nsources > 0 ? (size_t)nsources : 0, nb = nblocks > 0 ? (size_t)nblocks : 0;
+    std::vector<double> xs = widen(xsources, n), ys = widen(ysources, n), vs = widen(sourcevalues, n);
+    std::vector<double> bx = widen(x0s, nb), by = widen(y0s, nb), bh = widen(hs, nb);
+    std::vector<double> xr(nb * 1024), yr(nb * 1024);
+    tc_context *c = default_context_f32();
+    int rc = c ? tc_force_mrag(c, (double)theta, xs.data(), ys.data(), vs.data(), nsources, bx.data(), by.data(),
+                               bh.data(), nblocks, xr.data(), yr.data(), 0)
+               : -1;
+    if (rc)
+        report_dropin_failure(fn);
+    for (size_t i = 0; i < nb * 1024; ++i) {
+        xresults[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)xr[i];
+        yresults[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)yr[i];
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -611,7 +667,7 @@ TC_API int tc_set_option(tc_context *c, int key, int value)
         c->pot_order = value;
         return 0;
     case TC_OPT_POT_LEAF:
-        if (value < 1) break;
+        if (value < 1 || value > TC_MAX_LEAF) break;
         c->pot_leaf = value;
         return 0;
     case TC_OPT_FORCE_ORDER:
@@ -619,7 +675,7 @@ TC_API int tc_set_option(tc_context *c, int key, int value)
         c->force_order = value;
         return 0;
     case TC_OPT_FORCE_LEAF:
-        if (value < 1) break;
+        if (value < 1 || value > TC_MAX_LEAF) break;
         c->force_leaf = value;
         return 0;
     case TC_OPT_PROFILE:
@@ -820,6 +876,7 @@ TC_API int tc_morton_sort_device(tc_context *c, const double *x, const double *y
     });
 }
 
+#ifndef TREECODES2D_FLOAT_DROPIN
 TC_API void treecode_potential(const double theta, const double *const xsources, const double *const ysources,
                                const double *const sourcevalues, const int nsources,
                                const double *const xtargets, const double *const ytargets, const int ntargets,
@@ -854,48 +911,6 @@ TC_API void treecode_force(const double theta, const double *const xsources, con
     }
 }
 
-TC_API void treecode_potential_float(const float theta, const float *const xsources, const float *const ysources,
-                                     const float *const sourcevalues, const int nsources,
-                                     const float *const xtargets, const float *const ytargets,
-                                     const int ntargets, float *const targetvalues)
-{
-    std::lock_guard<std::mutex> lk(g_default_mu);
-    const size_t n = nsources > 0 ? (size_t)nsources : 0, m = ntargets > 0 ? (size_t)ntargets : 0;
-    std::vector<double> xs = widen(xsources, n), ys = widen(ysources, n), vs = widen(sourcevalues, n);
-    std::vector<double> xt = widen(xtargets, m), yt = widen(ytargets, m), out(m);
-    tc_context *c = default_context_f32();
-    int rc = c ? tc_potential(c, (double)theta, xs.data(), ys.data(), vs.data(), nsources, xt.data(), yt.data(),
-                              ntargets, out.data(), 0)
-               : -1;
-    if (rc)
-        report_dropin_failure("treecode_potential_float");
-    for (size_t i = 0; i < m; ++i)
-        targetvalues[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)out[i];
-}
-
-TC_API void treecode_force_mrag_float(const float theta, const float *const xsources,
-                                      const float *const ysources, const float *const sourcevalues,
-                                      const int nsources, const float *const x0s, const float *const y0s,
-                                      const float *const hs, const int nblocks, float *const xresults,
-                                      float *const yresults)
-{
-    std::lock_guard<std::mutex> lk(g_default_mu);
-    const size_t n = nsources > 0 ? (size_t)nsources : 0, nb = nblocks > 0 ? (size_t)nblocks : 0;
-    std::vector<double> xs = widen(xsources, n), ys = widen(ysources, n), vs = widen(sourcevalues, n);
-    std::vector<double> bx = widen(x0s, nb), by = widen(y0s, nb), bh = widen(hs, nb);
-    std::vector<double> xr(nb * 1024), yr(nb * 1024);
-    tc_context *c = default_context_f32();
-    int rc = c ? tc_force_mrag(c, (double)theta, xs.data(), ys.data(), vs.data(), nsources, bx.data(), by.data(),
-                               bh.data(), nblocks, xr.data(), yr.data(), 0)
-               : -1;
-    if (rc)
-        report_dropin_failure("treecode_force_mrag_float");
-    for (size_t i = 0; i < nb * 1024; ++i) {
-        xresults[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)xr[i];
-        yresults[i] = rc ? std::numeric_limits<float>::quiet_NaN() : (float)yr[i];
-    }
-}
-
 TC_API void treecode_velocity_mrag(const double theta, const double *const xsources,
                                    const double *const ysources, const double *const vorticity,
                                    const int nsources, const double *const x0s, const double *const y0s,
@@ -926,5 +941,46 @@ TC_API void treecode_force_mrag(const double theta, const double *const xsources
             xresults[i] = yresults[i] = std::numeric_limits<double>::quiet_NaN();
     }
 }
+
+TC_API void treecode_potential_float(const float theta, const float *const xsources, const float *const ysources,
+                                     const float *const sourcevalues, const int nsources,
+                                     const float *const xtargets, const float *const ytargets,
+                                     const int ntargets, float *const targetvalues)
+{
+    dropin_potential_f32("treecode_potential_float", theta, xsources, ysources, sourcevalues, nsources, xtargets,
+                         ytargets, ntargets, targetvalues);
+}
+
+TC_API void treecode_force_mrag_float(const float theta, const float *const xsources,
+                                      const float *const ysources, const float *const sourcevalues,
+                                      const int nsources, const float *const x0s, const float *const y0s,
+                                      const float *const hs, const int nblocks, float *const xresults,
+                                      float *const yresults)
+{
+    dropin_force_f32("treecode_force_mrag_float", theta, xsources, ysources, sourcevalues, nsources, x0s, y0s, hs,
+                     nblocks, xresults, yresults);
+}
+
+#else  // TREECODES2D_FLOAT_DROPIN
+
+// drivers/treecode-potential.h:21-29 and drivers/treecode-force.h:21-31 with realtype = float
+TC_API void treecode_potential(const float theta, const float *const xsources, const float *const ysources,
+                               const float *const sourcevalues, const int nsources, const float *const xtargets,
+                               const float *const ytargets, const int ntargets, float *const targetvalues)
+{
+    dropin_potential_f32("treecode_potential", theta, xsources, ysources, sourcevalues, nsources, xtargets, ytargets,
+                         ntargets, targetvalues);
+}
+
+TC_API void treecode_force_mrag(const float theta, const float *const xsources, const float *const ysources,
+                                const float *const sourcevalues, const int nsources, const float *const x0s,
+                                const float *const y0s, const float *const hs, const int nblocks,
+                                float *const xresults, float *const yresults)
+{
+    dropin_force_f32("treecode_force_mrag", theta, xsources, ysources, sourcevalues, nsources, x0s, y0s, hs, nblocks,
+                     xresults, yresults);
+}
+
+#endif  // TREECODES2D_FLOAT_DROPIN
 
 }  // extern "C"

@@ -1359,8 +1359,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
 {
     if (n <= 0)
         throw TcError(-3, "tc_build_tree: no sources");
-    if (leafcap < 1)
-        throw TcError(-3, "tc_build_tree: leaf capacity must be >= 1");
+    if (leafcap < 1 || leafcap > TC_MAX_LEAF)
+        throw TcError(-3, "tc_build_tree: leaf capacity must be in [1, 2^30]");
     if (n >= (1 << 30))
         throw TcError(-3, "tc_build_tree: at most 2^30 - 1 sources per tree");
     if (!tc_order_supported(order))

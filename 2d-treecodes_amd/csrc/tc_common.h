@@ -22,6 +22,8 @@
 #define TC_LMAX 15
 #define TC_MAXORDER 32
 #define TC_STACK 48  // DFS stack: 3 * LMAX + 1 entries suffice
+// Largest leaf capacity: source index + capacity stays below 2^31 (n < 2^30).
+#define TC_MAX_LEAF (1 << 30)
 
 // Expansion orders with compiled kernels: X(p) for each.
 #define TC_ORDERS(X) X(6) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(32)

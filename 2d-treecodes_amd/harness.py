@@ -110,6 +110,37 @@ def write_dataset(path, xs, ys, vs, xdst=None, ydst=None, xref=None, yref=None, 
                 np.ascontiguousarray(yref, np.float64).tofile(f)
 
 
+def write_synthetic_datasets(root, seed=1457):
+    """Synthetic cases under the eleven dataset paths the reference's main.cpp reads
+    (main.cpp:331-343; the originals are remote and not available offline), so that
+    an unmodified main.cpp linked against this library runs every test it has:
+    three potential files (N = 400, 2000, 12000 uniform sources, targets a copy of
+    them), four velocity files whose targets form 32x32 blocks (main.cpp:155-173)
+    and four MRAG files, from the 4-Gaussian vortex generator.  Stored reference
+    values are zero: main.cpp replaces the sampled ones with its own direct sum."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, here)
+    import workloads
+    written = []
+    for k, (rel, mragfile, tt) in enumerate(REFERENCE_FILES):
+        path = os.path.join(root, rel)
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        if tt == "P":
+            n = int(rel.rsplit("N", 1)[1])
+            xs, ys, vs, xt, yt = workloads.uniform_potential(n, seed=seed + k)
+            write_dataset(path, xs, ys, vs, xdst=xt, ydst=yt)
+        else:
+            xs, ys, vs, x0s, y0s, hs = workloads.gaussian4_force(6000 + 3000 * k, seed=seed + k, nbx=3 + k % 3,
+                                                                 h=1.0 / (64 + 32 * (k % 4)))
+            if mragfile:
+                write_dataset(path, xs, ys, vs, x0s=x0s, y0s=y0s, hs=hs, yref=np.zeros(len(x0s) * BS2))
+            else:
+                tx, ty = mrag_points(x0s, y0s, hs)
+                write_dataset(path, xs, ys, vs, xdst=tx, ydst=ty, yref=np.zeros(len(tx)))
+        written.append(path)
+    return written
+
+
 def check(ref, res, tol):
     """main.cpp:31-58: -> (passed, linf_abs, linf_rel, l1_abs, l1_rel, failures)."""
     ref = np.asarray(ref, np.float64)

@@ -24,7 +24,11 @@ extern "C" {
 
 #define TC_API __attribute__((visibility("default")))
 
-/* ---- drop-in entry points (reference semantics, FP64, host memory, synchronous) ---- */
+/* ---- drop-in entry points (reference semantics, FP64, host memory, synchronous) ----
+ * TREECODES2D_FLOAT_DROPIN: defined by the float build (libtreecodes2d_float.so), whose
+ * treecode_potential / treecode_force_mrag take float arrays under the same names, as
+ * the reference's real=float build (include/float/2d-treecodes-*.h declare them). */
+#ifndef TREECODES2D_FLOAT_DROPIN
 
 /* drivers/treecode-potential.h:21-29: out[i] = sum_j v_j log|t_i - s_j| with the
  * multipole order 12 / leaf capacity 512 of the reference build. */
@@ -42,9 +46,10 @@ TC_API void treecode_force_mrag(const double theta, const double *const xsources
                                 double *const xresults, double *const yresults);
 
 /* The reference's `real=float` build (libraries.Makefile:13, realtype substituted by m4)
- * has the same two functions over float arrays.  Here they carry a _float suffix so both
- * variants live in one library; they run the FP32-bulk path (FP64 tree geometry and
- * translations, FP32 near field and multipole evaluation). */
+ * has the same two functions over float arrays.  libtreecodes2d_float.so exports them
+ * under the reference's names (headers in include/float/); in this library they carry a
+ * _float suffix so both variants can be called from one process.  Both run the FP32-bulk
+ * path (FP64 tree geometry and translations, FP32 near field and multipole evaluation). */
 TC_API void treecode_potential_float(const float theta, const float *const xsources, const float *const ysources,
                                      const float *const sourcevalues, const int nsources,
                                      const float *const xtargets, const float *const ytargets,
@@ -71,6 +76,7 @@ TC_API void treecode_velocity_mrag(const double theta, const double *const xsour
                                    const double *const ysources, const double *const vorticity,
                                    const int nsources, const double *const x0s, const double *const y0s,
                                    const double *const hs, const int nblocks, double *const u, double *const v);
+#endif /* TREECODES2D_FLOAT_DROPIN */
 
 /* ---- extensions (no counterpart in the reference) ---- */
 

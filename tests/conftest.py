@@ -63,3 +63,21 @@ def mrag_targets(x0s, y0s, hs):
     tx = (np.asarray(x0s)[:, None] + ix[None, :] * np.asarray(hs)[:, None]).ravel()
     ty = (np.asarray(y0s)[:, None] + iy[None, :] * np.asarray(hs)[:, None]).ravel()
     return tx, ty
+
+
+def compare_tree(t, ot):
+    """Exported GPU tree (Context.tree()) against an oracle.Tree: box, keys, permutation,
+    topology and node geometry bit-exact (geometry decides every accept/open test),
+    expansion coefficients within 1e-13 of each node's largest."""
+    assert np.array_equal(t["box"], ot.bbox)
+    assert np.array_equal(t["keys"], ot.keys)
+    assert np.array_equal(t["perm"], ot.perm)
+    nn = ot.nnodes
+    assert t["topo"].shape[0] == nn
+    assert np.array_equal(t["topo"][:, 0], ot.se[:, 0])
+    assert np.array_equal(t["topo"][:, 1], ot.se[:, 1])
+    assert np.array_equal(t["topo"][:, 2], ot.child)
+    assert np.array_equal(t["topo"][:, 3], ot.level)
+    assert np.array_equal(t["geom"], ot.geom)
+    scale = np.maximum(np.abs(ot.exp).max(axis=(1, 2), keepdims=True), 1e-300)
+    assert np.max(np.abs(t["exps"] - ot.exp) / scale) <= 1e-13

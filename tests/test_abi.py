@@ -111,3 +111,78 @@ def test_request_stream_reuse_is_consistent(wl):
             assert all(np.array_equal(a, b) for a, b in zip(by_id[key], src))
         by_id[key] = src
     assert sum(r["reuses_sources"] for r in reqs) > 50
+
+
+# ------------------------------------------------------------------ float build and reference clients
+REF = "/root/reference"
+HAVE_REF = os.path.exists(os.path.join(REF, "main.cpp"))
+FLOAT_HEADERS = ["2d-treecodes-potential.h", "2d-treecodes-force.h"]
+
+
+def _float_lib(pkg):
+    return os.path.join(os.path.dirname(pkg.LIB_PATH), "libtreecodes2d_float.so")
+
+
+def test_float_library_exports_reference_names(pkg):
+    # the real=float build exports the two reference names (float signatures) and nothing
+    # of the double library's extension API
+    out = subprocess.run(["nm", "-D", "--defined-only", _float_lib(pkg)], capture_output=True, text=True).stdout
+    text_syms = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert text_syms == {"treecode_potential", "treecode_force_mrag", "treecode_last_error"}
+    for alias in FLOAT_HEADERS:
+        p = os.path.join(os.path.dirname(pkg.LIB_PATH), "float", "lib" + alias[:-2] + ".so")
+        assert os.path.realpath(p) == os.path.realpath(_float_lib(pkg))
+
+
+def _decl_tokens(text):
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return re.findall(r"[A-Za-z_]\w*|\S", text)
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="reference sources absent")
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_headers_equal_reference_generation(real):
+    # libraries.Makefile:30-31,35-36: m4 -D realtype=<real> drivers/treecode-*.h, minus the
+    # typedef and attribute lines; the shipped headers must declare exactly the same tokens
+    for h, src in zip(FLOAT_HEADERS, ["treecode-potential.h", "treecode-force.h"]):
+        lines = open(os.path.join(REF, "drivers", src)).read().splitlines()
+        gen = "\n".join(l for l in lines if "typedef" not in l and "attribute" not in l)
+        gen = re.sub(r"\brealtype\b", real, gen)
+        mine = os.path.join(ROOT, "include", h) if real == "double" else os.path.join(ROOT, "include", "float", h)
+        assert _decl_tokens(open(mine).read()) == _decl_tokens(gen), (real, h)
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_client_links_both_builds(pkg, tmp_path, real):
+    # a C client of the reference headers links against the reference library names of
+    # either build (the float client only resolves against the float library)
+    inc = os.path.join(ROOT, "include") if real == "double" else os.path.join(ROOT, "include", "float")
+    libdir = os.path.dirname(pkg.LIB_PATH) if real == "double" else os.path.join(os.path.dirname(pkg.LIB_PATH),
+                                                                                   "float")
+    src = tmp_path / "client.c"
+    src.write_text('#include "2d-treecodes-potential.h"\n#include "2d-treecodes-force.h"\n'
+                   "int main(void) { %s a[1] = {0}; treecode_potential(0.5f, a, a, a, 0, a, a, 0, a);\n"
+                   "  treecode_force_mrag(0.5f, a, a, a, 0, a, a, a, 0, a, a); return 0; }\n" % real)
+    exe = tmp_path / "client"
+    r = subprocess.run(["gcc", "-Wall", "-Werror", "-I", inc, str(src), "-L", libdir, "-l2d-treecodes-potential",
+                        "-l2d-treecodes-force", "-Wl,-rpath," + libdir, "-o", str(exe)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="reference sources absent")
+def test_reference_main_links_unchanged(pkg, tmp_path):
+    # the reference's main.cpp, read in place, compiled with its Makefile's flags against
+    # include/ and linked against lib2d-treecodes-{potential,force}.so (Makefile:17-18,25-26);
+    # with none of its dataset files present it skips every case without calling the library
+    exe = tmp_path / "test"
+    libdir = os.path.dirname(pkg.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++11", "-fopenmp", "-DBLOCKSIZE=32", "-g", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(REF, "main.cpp"), "-L", libdir, "-l2d-treecodes-potential",
+                        "-l2d-treecodes-force", "-Wl,-rpath," + libdir, "-o", str(exe)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([str(exe)], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert run.returncode == 0
+    assert run.stdout.count("not found, i skip this test") == 11

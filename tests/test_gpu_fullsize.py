@@ -1,14 +1,19 @@
-"""Full-size checks at the benchmark shapes (the 2^24 potential and the
-16,384-block force), through properties that do not need a CPU solve of that
-size: error against the FP64 direct sum on sampled targets, identical
-interaction counts in FP32 and FP64, keys and permutation against the oracle's
-key function, and the tree invariants of the reference build (children
-partition their parent's range in quadrant order, leaf iff at most `cap`
-sources or level 15)."""
+"""Full-size checks at the benchmark shapes: the 2^24 potential (half uniform,
+half Plummer), the 2^23-source force on 16,384 mixed-spacing blocks and the
+2^20-source force on 64 x 64 blocks.
+
+- FP64 against the oracle: the whole tree bit-exact (keys, permutation,
+  topology, geometry) and results within 1e-12 on sampled targets / blocks at
+  theta 0.3, 0.5 and 0.7, with the reference's and the tuned order / leaf.
+- Both precisions against the FP64 direct sum on 4096 sampled targets (1e-6),
+  with identical interaction counts in FP32 and FP64.
+- Size-independent invariants of the reference build: children partition
+  their parent's range in quadrant order, leaf iff at most `cap` sources or
+  level 15."""
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import compare_tree, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -20,24 +25,26 @@ def c3(wl):
     return wl.mixed_plummer_potential(N24)
 
 
-def _solve_potential(pkg, data, precision, order, leaf):
+def _solve_potential(pkg, data, precision, order, leaf, theta=0.5):
     xs, ys, vs, xt, yt = data
     ctx = pkg.Context(precision=precision, pot_order=order, pot_leaf=leaf, counters=True)
     d = {k: ctx.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt).items()}
     out = ctx.empty(len(xt))
-    ctx.potential(0.5, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
+    ctx.potential(theta, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
     ctx.synchronize()
     return ctx, d, out.numpy()
 
 
-def test_potential_2e24_accuracy_and_counts(pkg, c3):
+# theta 0.3 / 0.5 / 0.7 with an order that keeps the error below 1e-6 at each
+@pytest.mark.parametrize("theta,order", [(0.3, 8), (0.5, 10), (0.7, 20)])
+def test_potential_2e24_accuracy_and_counts(pkg, c3, theta, order):
     idx = np.sort(np.random.default_rng(7).choice(N24, 4096, replace=False)).astype(np.int32)
     counts = {}
     for prec in (pkg.FP32, pkg.FP64):
-        ctx, d, out = _solve_potential(pkg, c3, prec, 10, 256)
+        ctx, d, out = _solve_potential(pkg, c3, prec, order, 256, theta)
         counts[prec] = ctx.stats()["counters"]
         ref = ctx.direct_potential(d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], ctx.to_device(idx, np.int32))
-        assert rel_l2(out[idx], ref) <= 1e-6
+        assert rel_l2(out[idx], ref) <= 1e-6, prec
         ctx.close()
     # the FP32 path takes exactly the FP64 path's accept/open decisions
     for k in ("p2p", "e2p"):
@@ -107,3 +114,78 @@ def test_force_16384_blocks_accuracy_and_counts(pkg, wl):
     assert counts[pkg.FP32]["l2p"] == m
     for k in ("p2p", "e2p", "e2l", "l2p"):
         assert counts[pkg.FP32][k] == counts[pkg.FP64][k]
+
+
+# ------------------------------------------------------------------ FP64 against the oracle at full size
+# The oracle builds the same 2^24-source tree on the CPU in a few seconds and evaluates sampled
+# targets (or blocks) against it; a target's (block's) result does not depend on the others, so
+# a sample pins the full-size solve.  Bars as at small sizes: tree bit-exact, FP64 results within
+# 1e-12 relative L2, interaction counts equal (counted by a resident solve of the sample alone,
+# whose results must equal the full solve's at those targets bit for bit).
+FP64_TOL = 1e-12
+
+
+def _sample(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, k, replace=False))
+
+
+# reference order / leaf (treecode-potential.cpp:99, Makefile:13) and the tuned FP32 / FP64 ones
+@pytest.mark.parametrize("order,leaf", [(12, 512), (10, 192), (10, 64)])
+def test_potential_2e24_fp64_vs_oracle(pkg, oracle, c3, order, leaf):
+    xs, ys, vs, xt, yt = c3
+    ctx = pkg.Context(precision=pkg.FP64, pot_order=order, pot_leaf=leaf, counters=True)
+    d = {k: ctx.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt).items()}
+    out = ctx.empty(len(xt))
+    ctx.potential(0.5, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
+    ctx.synchronize()
+    ot = oracle.Tree(xs, ys, vs, leaf, order)
+    compare_tree(ctx.tree(), ot)
+    idx = _sample(N24, 4096, 70 + order + leaf)
+    for theta in (0.3, 0.5, 0.7):
+        if theta != 0.5:
+            ctx.potential_resident(theta, d["xt"], d["yt"], out=out, device=True)
+            ctx.synchronize()
+        full = out.numpy()[idx]
+        ref, cnt = ot.potential(theta, xt[idx], yt[idx])
+        assert rel_l2(full, ref) <= FP64_TOL, theta
+        part = ctx.potential_resident(theta, xt[idx], yt[idx])
+        assert np.array_equal(part, full), theta
+        c = ctx.stats()["counters"]
+        assert (c["p2p"], c["e2p"]) == (cnt[0], cnt[1]), theta
+    del ot
+    ctx.close()
+
+
+def _force_vs_oracle(pkg, oracle, data, order, leaf, nsample, seed):
+    xs, ys, vs, x0s, y0s, hs = data
+    nb = len(x0s)
+    ctx = pkg.Context(precision=pkg.FP64, force_order=order, force_leaf=leaf, counters=True)
+    d = {k: ctx.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs).items()}
+    xr, yr = ctx.empty(nb * 1024), ctx.empty(nb * 1024)
+    ctx.force(0.5, d["xs"], d["ys"], d["vs"], d["x0s"], d["y0s"], d["hs"], xr=xr, yr=yr, device=True)
+    ctx.synchronize()
+    ot = oracle.Tree(xs, ys, vs, leaf, order)
+    compare_tree(ctx.tree(), ot)
+    b = _sample(nb, nsample, seed)
+    rows = (b[:, None] * 1024 + np.arange(1024)[None, :]).ravel()
+    fx, fy = xr.numpy()[rows], yr.numpy()[rows]
+    rx, ry, cnt = ot.force(0.5, x0s[b], y0s[b], hs[b])
+    assert rel_l2(fx, rx) <= FP64_TOL and rel_l2(fy, ry) <= FP64_TOL
+    gx, gy = ctx.force_resident(0.5, x0s[b], y0s[b], hs[b])
+    assert rel_l2(gx, fx) <= 1e-15 and rel_l2(gy, fy) <= 1e-15
+    c = ctx.stats()["counters"]
+    assert (c["p2p"], c["e2p"], c["e2l"], c["l2p"]) == tuple(int(v) for v in cnt)
+    del ot
+    ctx.close()
+
+
+# C5: 2^23 sources, 16,384 mixed-spacing blocks; reference 24 / 192 and the tuned FP64 8 / 64
+@pytest.mark.parametrize("order,leaf", [(24, 192), (8, 64)])
+def test_force_c5_fp64_vs_oracle(pkg, oracle, wl, order, leaf):
+    _force_vs_oracle(pkg, oracle, wl.gaussian16_force(1 << 23, nblocks=16384), order, leaf, 16, 90 + order)
+
+
+# C2: 2^20 sources (4 Gaussians), 64 x 64 blocks at h = 1/2048; reference and tuned FP64 8 / 32
+@pytest.mark.parametrize("order,leaf", [(24, 192), (8, 32)])
+def test_force_c2_fp64_vs_oracle(pkg, oracle, wl, order, leaf):
+    _force_vs_oracle(pkg, oracle, wl.gaussian4_force(1 << 20), order, leaf, 64, 95 + order)

@@ -5,10 +5,12 @@ coefficients within 1e-13 relative; FP64 potentials and velocities within
 1e-12 relative L2 with identical interaction counts; the FP32-bulk path within
 1e-6 relative L2 of the FP64 direct sum and of the FP64 treecode.
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import mrag_targets, rel_l2
+from conftest import compare_tree, mrag_targets, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -81,22 +83,7 @@ def test_morton_sort_cell_edges(pkg, oracle, ctx64):
 
 
 def _compare_tree(ctx, ot, order):
-    t = ctx.tree()
-    assert np.array_equal(t["box"], ot.bbox)
-    assert np.array_equal(t["keys"], ot.keys)
-    assert np.array_equal(t["perm"], ot.perm)
-    nn = ot.nnodes
-    assert t["topo"].shape[0] == nn
-    assert np.array_equal(t["topo"][:, 0], ot.se[:, 0])
-    assert np.array_equal(t["topo"][:, 1], ot.se[:, 1])
-    assert np.array_equal(t["topo"][:, 2], ot.child)
-    assert np.array_equal(t["topo"][:, 3], ot.level)
-    # geometry decides every accept/open test: bit-exact
-    assert np.array_equal(t["geom"], ot.geom)
-    ex = t["exps"]
-    oe = ot.exp
-    scale = np.maximum(np.abs(oe).max(axis=(1, 2), keepdims=True), 1e-300)
-    assert np.max(np.abs(ex - oe) / scale) <= 1e-13
+    compare_tree(ctx.tree(), ot)
 
 
 @pytest.mark.parametrize("case", ["uniform", "gauss", "small", "tiny", "plummer"])
@@ -343,6 +330,39 @@ def test_empty_and_error_paths(pkg, wl, ctx64):
     assert np.all(np.isfinite(good))
 
 
+def test_failed_build_leaves_no_resident_tree(pkg, wl):
+    # an FP64 tree is resident; a rebuild in FP32 fails at its census (non-finite source)
+    # after it has overwritten the tree's sizes and source arrays: resident calls and the
+    # export must refuse (no kernel runs on the half-built tree), and a new build works
+    ctx = pkg.Context(precision=pkg.FP64)
+    xs, ys, vs, xt, yt = wl.uniform_potential(20000, seed=18)
+    ctx.potential(0.5, xs, ys, vs, xt, yt)
+    ctx.set(pkg.OPT_PRECISION, pkg.FP32)
+    bad = xs.copy()
+    bad[3] = np.nan
+    with pytest.raises(pkg.TreecodeError, match="non-finite"):
+        ctx.potential(0.5, bad, ys, vs, xt, yt)
+    x0s, y0s, hs = np.array([0.2]), np.array([0.3]), np.array([1.0 / 256])
+    for call in (lambda: ctx.potential_resident(0.5, xt, yt), lambda: ctx.force_resident(0.5, x0s, y0s, hs),
+                 ctx.tree):
+        with pytest.raises(pkg.TreecodeError, match="no resident tree"):
+            call()
+    out = ctx.potential(0.5, xs, ys, vs, xt, yt)
+    assert np.array_equal(ctx.potential_resident(0.5, xt, yt), out)
+    ctx.close()
+
+
+def test_leaf_capacity_limit(pkg):
+    # source index + leaf capacity must stay below 2^31: capacities above 2^30 are refused
+    ctx = pkg.Context()
+    for key in (pkg.OPT_POT_LEAF, pkg.OPT_FORCE_LEAF):
+        ctx.set(key, 1 << 30)
+        for bad in ((1 << 30) + 1, (1 << 31) - 1, 0, -5):
+            with pytest.raises(pkg.TreecodeError, match="invalid option"):
+                ctx.set(key, bad)
+    ctx.close()
+
+
 def test_coincident_sources(pkg, oracle, ctx64, ctx32):
     # zero-radius nodes (all sources at one point) next to ordinary ones
     n = 2000
@@ -403,24 +423,72 @@ def test_gpu_direct_sums_vs_oracle(pkg, oracle, wl, ctx64):
     assert rel_l2(gx, rx) <= 1e-14 and rel_l2(gy, ry) <= 1e-14
 
 
+def _float_dropin(pkg):
+    """The float build's drop-in pair under the reference's names (float/lib2d-treecodes-*.so)."""
+    import ctypes
+    libdir = os.path.join(os.path.dirname(pkg.LIB_PATH), "float")
+    F = ctypes.POINTER(ctypes.c_float)
+    fl, it = ctypes.c_float, ctypes.c_int
+    pot = ctypes.CDLL(os.path.join(libdir, "lib2d-treecodes-potential.so")).treecode_potential
+    pot.argtypes = [fl, F, F, F, it, F, F, it, F]
+    pot.restype = None
+    frc = ctypes.CDLL(os.path.join(libdir, "lib2d-treecodes-force.so")).treecode_force_mrag
+    frc.argtypes = [fl, F, F, F, it, F, F, F, it, F, F]
+    frc.restype = None
+    return pot, frc
+
+
 def test_float_abi(pkg, oracle, wl):
-    # the reference's real=float build: float arrays in and out, FP32-bulk path inside
+    # the reference's real=float build: treecode_potential / treecode_force_mrag over float
+    # arrays, called by the reference names in the float library, and the _float-suffixed
+    # pair of the double library; both run the FP32-bulk path and give the same results
+    import ctypes
+    pot, frc = _float_dropin(pkg)
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
     xs, ys, vs, xt, yt = (a.astype(np.float32) for a in wl.uniform_potential(20000, seed=61))
-    out = pkg.treecode_potential_float(0.5, xs, ys, vs, xt, yt)
-    assert out.dtype == np.float32
+    out = np.empty(len(xt), np.float32)
+    pot(0.5, fp(xs), fp(ys), fp(vs), len(xs), fp(xt), fp(yt), len(xt), fp(out))
     ref, _ = oracle.potential(0.5, *(a.astype(np.float64) for a in (xs, ys, vs, xt, yt)))
     assert rel_l2(out, ref) <= FP32_TOL
+    assert np.array_equal(out, pkg.treecode_potential_float(0.5, xs, ys, vs, xt, yt))
     xs, ys, vs, x0s, y0s, hs = (a.astype(np.float32) for a in wl.gaussian4_force(20000, nbx=4, h=1.0 / 128, seed=62))
-    fx, fy = pkg.treecode_force_mrag_float(0.5, xs, ys, vs, x0s, y0s, hs)
+    fx, fy = np.empty(len(x0s) * 1024, np.float32), np.empty(len(x0s) * 1024, np.float32)
+    frc(0.5, fp(xs), fp(ys), fp(vs), len(xs), fp(x0s), fp(y0s), fp(hs), len(x0s), fp(fx), fp(fy))
     rx, ry, _ = oracle.force_mrag(0.5, *(a.astype(np.float64) for a in (xs, ys, vs, x0s, y0s, hs)))
     assert rel_l2(fx, rx) <= FP32_TOL and rel_l2(fy, ry) <= FP32_TOL
+    gx, gy = pkg.treecode_force_mrag_float(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert np.array_equal(fx, gx) and np.array_equal(fy, gy)
 
 
-def test_velocity_wrapper(pkg, wl):
+REF_MAIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                        "main_dropin")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_MAIN), reason="oracle/_ref/main_dropin not built (needs the reference)")
+def test_reference_main_runs_unchanged(pkg, tmp_path):
+    # the reference's main.cpp (built unchanged against this library by oracle/Makefile)
+    # on synthetic files at its eleven dataset paths: it calls treecode_potential /
+    # treecode_force_mrag and checks sampled results against its own FP64 direct sum
+    # with its own rule (tol 1e-8, velocities 1e-6; asserts abort on a failure)
+    import subprocess
+    from treecodes2d_amd import harness
+    harness.write_synthetic_datasets(str(tmp_path))
+    r = subprocess.run([REF_MAIN, "0.2", "1e-8"], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert r.stdout.count("TEST PASSED.") == 11, r.stdout[-2000:]
+    assert "not found" not in r.stdout
+
+
+def test_velocity_wrapper(pkg, oracle, wl):
+    # main.cpp:136-143 turns stored velocities into the library's convention by
+    # xref = -2 pi v, yref = 2 pi u; the wrapper inverts it: u = grad_y / (2 pi),
+    # v = -grad_x / (2 pi) of the oracle's treecode_force_mrag, within the FP64 bar
     xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(20000, nbx=4, h=1.0 / 128, seed=63)
     u, v = pkg.treecode_velocity_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
-    fx, fy = pkg.treecode_force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
-    assert np.array_equal(u, fy * (1.0 / (2 * np.pi))) and np.array_equal(v, -fx * (1.0 / (2 * np.pi)))
+    rx, ry, _ = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert rel_l2(u, ry / (2 * np.pi)) <= FP64_TOL and rel_l2(v, -rx / (2 * np.pi)) <= FP64_TOL
+    # and the harness's reading of a stored velocity file maps (u, v) back onto the force
+    assert rel_l2(-2 * np.pi * v, rx) <= FP64_TOL and rel_l2(2 * np.pi * u, ry) <= FP64_TOL
 
 
 def test_log64_accuracy(pkg, ctx64):
