@@ -33,8 +33,10 @@ EXPORTED = [
     "tc_morton_sort_device", "tc_device_alloc", "tc_device_free", "tc_copy_to_device", "tc_copy_to_host",
     "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
     "tc_server_wait", "tc_server_stats", "treecode_potential_float", "treecode_force_mrag_float",
-    "treecode_velocity_mrag", "tc_copy_kernel", "treecode_force", "tc_force_targets",
+    "treecode_velocity_mrag", "tc_copy_kernel", "treecode_force", "tc_force_targets", "tc_potential_noalloc",
 ]
+NOALLOC_MAX_POINTS = 1 << 18  # sources and targets per tc_potential_noalloc call
+NOALLOC_SCRATCH_BYTES = 64 << 20  # its static device scratch
 
 
 class TreecodeError(RuntimeError):
@@ -145,6 +147,8 @@ def lib():
         L.tc_force_targets.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P, _P, it]
     if hasattr(L, 'tc_copy_kernel'):
         L.tc_copy_kernel.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    if hasattr(L, 'tc_potential_noalloc'):
+        L.tc_potential_noalloc.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P]
     if hasattr(L, 'tc_direct_potential'):
         L.tc_direct_potential.argtypes = [_P, _P, _P, _P, it, _P, _P, _P, it, _P]
     if hasattr(L, 'tc_direct_force'):
@@ -323,6 +327,15 @@ class Context:
             out = np.empty(len(xt)) if out is None else out
         self._check(lib().tc_potential(self.h, float(theta), _ptr(xs), _ptr(ys), _ptr(vs), len(xs), _ptr(xt),
                                        _ptr(yt), len(xt), _ptr(out), int(device)))
+        return out
+
+    def potential_noalloc(self, theta, xs, ys, vs, xt, yt, out, n=None, m=None):
+        """tc_potential_noalloc: DeviceArrays only; the first n sources and m targets
+        (default: all); synchronous, allocates nothing on the device."""
+        n = len(xs) if n is None else n
+        m = len(xt) if m is None else m
+        self._check(lib().tc_potential_noalloc(self.h, float(theta), xs.ptr, ys.ptr, vs.ptr, int(n), xt.ptr, yt.ptr,
+                                               int(m), out.ptr))
         return out
 
     def potential_resident(self, theta, xt, yt, out=None, device=False):

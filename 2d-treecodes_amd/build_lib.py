@@ -31,8 +31,9 @@ LIB_FLOAT = os.path.join(HERE, "libtreecodes2d_float.so")
 ALIASES = ["lib2d-treecodes-potential.so", "lib2d-treecodes-force.so"]
 # (source, object name, extra flags)
 UNITS = [("tc_build.hip", "tc_build", []), ("tc_eval.hip", "tc_eval", []), ("tc_api.cpp", "tc_api", []),
-         ("tc_serve.cpp", "tc_serve", []), ("tc_api.cpp", "tc_api_float", ["-DTREECODES2D_FLOAT_DROPIN"])]
-DOUBLE_OBJS = ["tc_build", "tc_eval", "tc_api", "tc_serve"]
+         ("tc_serve.cpp", "tc_serve", []), ("tc_noalloc.hip", "tc_noalloc", []),
+         ("tc_api.cpp", "tc_api_float", ["-DTREECODES2D_FLOAT_DROPIN"])]
+DOUBLE_OBJS = ["tc_build", "tc_eval", "tc_api", "tc_serve", "tc_noalloc"]
 FLOAT_OBJS = ["tc_build", "tc_eval", "tc_api_float"]
 FLOAT_EXPORTS = ["treecode_potential", "treecode_force_mrag", "treecode_last_error"]
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")

@@ -48,11 +48,19 @@ struct tc_context {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_valid = false, eval_has_tsort = false;
     unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0, 0, 0};
+    bool counters_on_host = false;  // the last call left its counters in `counters` (no-allocation path)
     // pinned staging for host arrays: two buffers, so the CPU copy of one
     // chunk overlaps the DMA of the other (allocated on first large copy)
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_busy[2] = {false, false};
+    // tc_potential_noalloc: tree, source and target scratch and counters carved
+    // from the device's static scratch array (address looked up at creation)
+    void *na_base = nullptr;
+    TcArena na_arena;
+    TcTree na_tree;
+    TcScratch na_ws, na_tws;
+    DevBuf na_cnt;
 };
 
 namespace {
@@ -269,6 +277,7 @@ void copy_out_all(tc_context *c, const double *d, std::initializer_list<HostOut>
 
 unsigned long long *prepare_counters(tc_context *c)
 {
+    c->counters_on_host = false;
     if (!c->count)
         return nullptr;
     c->cnt.reserve(sizeof(unsigned long long) * TC_NCOUNTERS);
@@ -395,6 +404,82 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
         }
     });
 }
+
+#ifndef TREECODES2D_FLOAT_DROPIN
+constexpr int NOALLOC_MAX_POINTS = 1 << 18;  // sources and targets of one tc_potential_noalloc call
+
+// tc_potential with device pointers, every buffer carved from the static
+// scratch: the same kernels in the same order as run_potential, so the results
+// equal its results bit for bit.  Synchronous; on failure both streams are
+// drained before returning, so no kernel still reads the scratch.
+int run_potential_noalloc(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs,
+                          int n, const double *xt, const double *yt, int m, double *out)
+{
+    return guarded(ctx, [&] {
+        tc_context *c = ctx;
+        c->ev_valid = false;
+        if (!c->na_base)
+            throw TcError(-1, "tc_potential_noalloc: no scratch address for this device");
+        if (n > NOALLOC_MAX_POINTS || m > NOALLOC_MAX_POINTS)
+            throw TcError(-3, "tc_potential_noalloc: at most " + std::to_string(NOALLOC_MAX_POINTS) +
+                                  " sources and targets per call (got " + std::to_string(n) + " and " +
+                                  std::to_string(m) + ")");
+        if (m <= 0)
+            return;
+        if (n <= 0) {
+            tc_fill_u32(reinterpret_cast<uint32_t *>(out), 0u, 2 * (size_t)m, c->stream);
+            TC_HIP(hipStreamSynchronize(c->stream));
+            return;
+        }
+        // every call lays its buffers out afresh from the start of the array
+        c->na_arena.base = static_cast<char *>(c->na_base);
+        c->na_arena.cap = tc_noalloc_scratch_bytes();
+        c->na_arena.used = 0;
+        c->na_tree.bind(&c->na_arena);
+        c->na_ws.bind(&c->na_arena);
+        c->na_tws.bind(&c->na_arena);
+        c->na_cnt.bind(&c->na_arena);
+        unsigned long long *cnt = nullptr;
+        if (c->count) {
+            c->na_cnt.reserve(sizeof(unsigned long long) * TC_NCOUNTERS);
+            cnt = c->na_cnt.as<unsigned long long>();
+            tc_fill_u32(c->na_cnt.as<uint32_t>(), 0u, 2 * TC_NCOUNTERS, c->stream);
+        }
+        bool forked = false;
+        try {
+            TC_HIP(hipEventRecord(c->fork_ev, c->stream));
+            TC_HIP(hipStreamWaitEvent(c->tstream, c->fork_ev, 0));
+            forked = true;
+            tc_sort_targets(xt, yt, m, c->na_tws, c->tstream);
+            TC_HIP(hipEventRecord(c->join_ev, c->tstream));
+            record(c, 0);
+            const bool f32 = c->precision == 1;
+            tc_build_tree(c->na_tree, xs, ys, vs, n, c->pot_leaf, c->pot_order, f32, false, c->na_ws, c->stream);
+            record(c, 1);
+            TC_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+            tc_eval_potential(c->na_tree, theta, xt, yt, m, out, c->precision, cnt, c->na_tws, c->stream,
+                              c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, false);
+            TC_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            if (forked)
+                (void)hipStreamSynchronize(c->tstream);
+            (void)hipStreamSynchronize(c->stream);
+            throw;
+        }
+        c->ev_valid = c->profile;
+        c->eval_has_tsort = true;
+        if (cnt) {
+            // the counters leave the scratch now (a later call reuses it)
+            void *hp = c->na_tws.host_pinned();
+            TC_HIP(hipMemcpyAsync(hp, cnt, sizeof(unsigned long long) * TC_NCOUNTERS, hipMemcpyDeviceToHost,
+                                  c->stream));
+            TC_HIP(hipStreamSynchronize(c->stream));
+            std::memcpy(c->counters, hp, sizeof(unsigned long long) * TC_NCOUNTERS);
+            c->counters_on_host = true;
+        }
+    });
+}
+#endif
 
 int run_force(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
               const double *x0s, const double *y0s, const double *hs, int nb, double *xr, double *yr,
@@ -607,6 +692,13 @@ TC_API tc_context *tc_context_create(int device)
         TC_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
         TC_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
         ensure_binomials(device, c->stream);
+#ifndef TREECODES2D_FLOAT_DROPIN
+        // set-up of tc_potential_noalloc: the device address of the static
+        // scratch (device current) and the pinned readback buffers it uses
+        c->na_base = tc_noalloc_scratch_lookup(device);
+        c->na_ws.host_pinned();
+        c->na_tws.host_pinned();
+#endif
     } catch (const TcError &e) {
         set_error(e.what());
         tc_context_destroy(c);
@@ -629,6 +721,10 @@ TC_API void tc_context_destroy(tc_context *c)
     c->tws.release();
     c->io.release();
     c->cnt.release();
+    c->na_tree.release();
+    c->na_ws.release();
+    c->na_tws.release();
+    c->na_cnt.release();
     for (auto &e : c->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -707,6 +803,14 @@ TC_API int tc_potential_resident(tc_context *c, double theta, const double *xt, 
     return run_potential(c, theta, nullptr, nullptr, nullptr, 0, xt, yt, m, out, device_ptrs, false);
 }
 
+#ifndef TREECODES2D_FLOAT_DROPIN
+TC_API int tc_potential_noalloc(tc_context *c, double theta, const double *xs, const double *ys, const double *vs,
+                                int n, const double *xt, const double *yt, int m, double *out)
+{
+    return run_potential_noalloc(c, theta, xs, ys, vs, n, xt, yt, m, out);
+}
+#endif
+
 TC_API int tc_force_mrag(tc_context *c, double theta, const double *xs, const double *ys, const double *vs,
                          int n, const double *x0s, const double *y0s, const double *hs, int nblocks, double *xr,
                          double *yr, int device_ptrs)
@@ -752,7 +856,9 @@ TC_API int tc_get_stats(tc_context *c, double *times_ms, unsigned long long *cou
             }
         }
         if (counters) {
-            if (c->count && c->cnt.ptr)
+            if (c->count && c->counters_on_host)
+                std::memcpy(counters, c->counters, sizeof(unsigned long long) * TC_NCOUNTERS);
+            else if (c->count && c->cnt.ptr)
                 TC_HIP(hipMemcpy(counters, c->cnt.ptr, sizeof(unsigned long long) * TC_NCOUNTERS,
                                  hipMemcpyDeviceToHost));
             else

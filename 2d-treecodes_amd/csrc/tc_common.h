@@ -162,26 +162,59 @@ TC_DEV int tc_common_level(uint32_t a, uint32_t b)
 }
 
 // ------------------------------------------------------------------ host-side internals
+// Bump allocator over device memory the library does not own (the static
+// scratch array of the no-allocation entry point).  Buffers bound to an arena
+// carve their storage from it and never call hipMalloc or hipFree.
+struct TcArena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *carve(size_t bytes)
+    {
+        const size_t o = (used + 255) & ~size_t(255);
+        if (o > cap || bytes > cap - o)
+            throw TcError(-3, "treecode: the request needs more than the " + std::to_string(cap >> 20) +
+                                  " MiB of the no-allocation scratch");
+        used = o + bytes;
+        return base + o;
+    }
+};
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
+    TcArena *arena = nullptr;  // non-null: storage carved from the arena, never freed
     template <class T> T *as() const { return static_cast<T *>(ptr); }
     void reserve(size_t bytes)
     {
         if (bytes <= cap)
             return;
+        size_t b = bytes < 256 ? 256 : bytes;
+        if (arena) {
+            // a larger request takes a fresh piece; the old one stays unused
+            // until the arena is reset
+            ptr = arena->carve(b);
+            cap = b;
+            return;
+        }
         if (ptr)
             TC_HIP(hipFree(ptr));
         ptr = nullptr;
         cap = 0;
-        size_t b = bytes < 256 ? 256 : bytes;
         b = (b + 4095) & ~size_t(4095);
         TC_HIP(hipMalloc(&ptr, b));
         cap = b;
     }
+    // Binds the buffer to an arena (or back to hipMalloc with nullptr); the
+    // buffer must not own memory.
+    void bind(TcArena *a)
+    {
+        arena = a;
+        ptr = nullptr;
+        cap = 0;
+    }
     void release()
     {
-        if (ptr)
+        if (ptr && !arena)
             (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
@@ -202,6 +235,11 @@ struct TcTree {
     {
         for (DevBuf *b : {&keys, &perm, &src, &src32, &nodes, &exps, &exps32, &nodew, &selfid, &box_d})
             b->release();
+    }
+    void bind(TcArena *a)
+    {
+        for (DevBuf *b : {&keys, &perm, &src, &src32, &nodes, &exps, &exps32, &nodew, &selfid, &box_d})
+            b->bind(a);
     }
 };
 
@@ -228,7 +266,21 @@ struct TcScratch {
             (void)hipHostFree(pinned);
         pinned = nullptr;
     }
+    // Device buffers from the arena; the pinned readback buffer must already
+    // exist (host_pinned() at set-up), so no call made with this scratch allocates.
+    void bind(TcArena *a)
+    {
+        for (DevBuf *b : {&k2, &v2, &hist, &part, &scan_tmp, &istart, &ilevel, &info, &tkeys, &tperm, &tsorted,
+                          &bbox_part, &ints, &packed, &work, &flist})
+            b->bind(a);
+    }
 };
+
+// Static device scratch of the no-allocation entry point (tc_noalloc.hip):
+// its size, and its address on the current device (looked up once per device
+// with hipGetSymbolAddress, then cached).
+size_t tc_noalloc_scratch_bytes();
+void *tc_noalloc_scratch_lookup(int device);
 
 // Internal entry points (tc_build.hip, tc_eval.hip)
 void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st);

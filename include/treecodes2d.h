@@ -125,6 +125,24 @@ TC_API int tc_force_targets(tc_context *ctx, double theta, const double *xs, con
                             int device_ptrs);
 TC_API int tc_synchronize(tc_context *ctx);
 
+/* tc_potential with device pointers that allocates nothing.  No counterpart in the
+ * reference (whose library allocates per call, upward.cpp:289-300): it is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The sorted sources, the tree, the target
+ * order and the counters live in one static device array of 64 MiB per GPU (held
+ * until the process ends; its address is looked up when a context is created), so the
+ * call makes no hipMalloc, hipFree or other allocating call.  Limits: n and m at most
+ * 262,144 each (larger calls return -3 before any work is queued), and the tree must
+ * fit what the points leave of the 64 MiB (about 140 + 24*order bytes per node; a
+ * larger tree returns -3 at the build's census).  Order, leaf capacity, precision,
+ * counters and timers come from the context's options; results equal tc_potential's
+ * with device_ptrs = 1 on the same inputs.  Synchronous (returns with `out` written).
+ * Every call on one GPU shares that GPU's array and overwrites it: calls that overlap
+ * on one GPU (from several threads or contexts) race and must be serialised by the
+ * caller.  The resident tree of tc_potential_resident is neither used nor changed. */
+TC_API int tc_potential_noalloc(tc_context *ctx, double theta, const double *xs, const double *ys,
+                                const double *vs, int n, const double *xt, const double *yt, int m, double *out);
+
 /* Evaluate new targets against the tree left resident by the last tc_potential /
  * tc_force_mrag call of this context (the sources are not rebuilt).  The tree must
  * have been built for the same kind of solve (order, leaf capacity, precision). */
