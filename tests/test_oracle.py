@@ -387,3 +387,162 @@ def test_force_targets_vs_direct(oracle, wl):
     assert cnt[0] == len(xs) * len(xt)
     fx, fy, cnt = oracle.force_targets(0.5, xs, ys, vs, xt, yt)
     assert rel_l2(fx, dx) <= 1e-9 and rel_l2(fy, dy) <= 1e-9 and cnt[1] > 0
+
+
+# ------------------------------------------------------------------ reference-style traversals
+# The interaction lists (and so the counters the GPU must match) are pinned by a
+# literal restatement of the reference's own loops on a tree built the reference's
+# way: the recursive build above (reference numbering, not breadth-first), node
+# geometry from the leaf setup and parent update of upward.cpp:60-160 in numpy,
+# expansions by direct P2E of each node's sources about its centre of mass (equal
+# to the M2M translations for a finite order), then the depth-first loops of
+# treecode-potential.cpp:25-86 and treecode-force.cpp:94-199 with their
+# accept / E2L / E2P tests and push order.  Empty nodes contribute exactly zero
+# and are not counted (the oracle's convention).
+EPS_REG = 10 * np.finfo(float).eps
+
+
+def ref_style_geometry(keys, xs, ys, vs, cap, p, ext):
+    nodes = ref_style_tree(keys, len(keys), cap)
+    for nid in sorted(nodes, reverse=True):  # children carry larger ids than their parent
+        nd = nodes[nid]
+        s, e = nd["s"], nd["e"]
+        if e == s:
+            nd.update(w=0.0, wx=0.0, wy=0.0, mass=0.0)
+            continue
+        if not nd["child"]:
+            a = np.abs(vs[s:e])
+            nd.update(mass=float(np.sum(vs[s:e])), w=float(np.sum(a)), wx=float(np.sum(xs[s:e] * a)),
+                      wy=float(np.sum(ys[s:e] * a)))
+            nd["xcom"], nd["ycom"] = nd["wx"] / nd["w"], nd["wy"] / nd["w"]
+            nd["r"] = math.sqrt(float(np.max((xs[s:e] - nd["xcom"]) ** 2 + (ys[s:e] - nd["ycom"]) ** 2)))
+        else:
+            kids = [nodes[nd["child"] + c] for c in range(4)]
+            for k in ("mass", "w", "wx", "wy"):
+                nd[k] = sum(kid[k] for kid in kids)
+            nd["xcom"], nd["ycom"] = nd["wx"] / nd["w"], nd["wy"] / nd["w"]
+            rr = max(kid["r"] + math.hypot(nd["xcom"] - kid["xcom"], nd["ycom"] - kid["ycom"])
+                     for kid in kids if kid["w"])
+            nd["r"] = min(rr, float(np.float32(1.4143)) * ext / (1 << nd["l"]))
+        d = (xs[s:e] - nd["xcom"]) + 1j * (ys[s:e] - nd["ycom"])
+        k = np.arange(1, p + 1)
+        nd["a"] = -np.sum(vs[s:e, None] * d[:, None] ** k[None, :], axis=0) / k
+    return nodes
+
+
+def ref_style_potential(nodes, xs, ys, vs, theta, xt, yt):
+    out = np.zeros(len(xt))
+    npairs = ne2p = 0
+    for i, (x, y) in enumerate(zip(xt, yt)):
+        res = 0.0
+        stack = [0]
+        while stack:
+            nd = nodes[stack.pop()]
+            if nd["e"] == nd["s"]:
+                continue
+            r2 = (x - nd["xcom"]) ** 2 + (y - nd["ycom"]) ** 2
+            if nd["r"] * nd["r"] < theta * theta * r2:
+                z = complex(x - nd["xcom"], y - nd["ycom"])
+                u = (1 / z) ** np.arange(1, len(nd["a"]) + 1)
+                res += nd["mass"] * math.log(r2) / 2 + float(np.sum(nd["a"] * u).real)
+                ne2p += 1
+            elif not nd["child"]:
+                s, e = nd["s"], nd["e"]
+                dx, dy = x - xs[s:e], y - ys[s:e]
+                res += 0.5 * float(np.sum(np.log(dx * dx + dy * dy + EPS_REG) * vs[s:e]))
+                npairs += e - s
+            else:
+                stack += [nd["child"] + c for c in range(4)]
+        out[i] = res
+    return out, npairs, ne2p
+
+
+def ref_style_force_block(nodes, xs, ys, vs, theta, x0, y0, h, p):
+    tile = 8
+    fx, fy = np.zeros(1024), np.zeros(1024)
+    counts = np.zeros(4, np.int64)
+    rbrick = 1.4142135623730951 * h * (tile - 1) * 0.5
+    ix, iy = np.arange(64) % tile, np.arange(64) // tile
+    l = np.arange(1, p + 1)
+    binom = np.array([[math.comb(ll + k - 1, k - 1) for k in range(1, p + 1)] for ll in range(1, p + 1)], float)
+    for by in range(0, 32, tile):
+        for bx in range(0, 32, tile):
+            xb, yb = x0 + h * (bx + 0.5 * (tile - 1)), y0 + h * (by + 0.5 * (tile - 1))
+            rx, ry = np.zeros(64), np.zeros(64)
+            loc = np.zeros(p, complex)  # b_1..b_p
+            tx, ty = x0 + (bx + ix) * h, y0 + (by + iy) * h
+            stack = [0]
+            while stack:
+                nd = nodes[stack.pop()]
+                if nd["e"] == nd["s"]:
+                    continue
+                xc, yc, r = nd["xcom"], nd["ycom"], nd["r"]
+                dist = math.sqrt((xb - xc) ** 2 + (yb - yc) ** 2)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    ratio = np.float64(dist) / np.float64(r)  # IEEE: a one-source leaf has r = 0
+                if (ratio - 1) > (1 / theta) and rbrick <= r:
+                    z0 = complex(xc - xb, yc - yb)
+                    sgn = (-1.0) ** l
+                    inner = -nd["mass"] / l + binom @ (sgn * nd["a"] * z0 ** (-l.astype(float)))
+                    loc += z0 ** (-l.astype(float)) * inner
+                    counts[2] += 1
+                    continue
+                xt = max(x0 + bx * h, min(x0 + (bx + tile - 1) * h, xc))
+                yt = max(y0 + by * h, min(y0 + (by + tile - 1) * h, yc))
+                if r * r < theta * theta * ((xt - xc) ** 2 + (yt - yc) ** 2):
+                    z = (tx - xc) + 1j * (ty - yc)
+                    dphi = nd["mass"] / z - np.sum(l * nd["a"] * z[:, None] ** (-(l + 1.0)), axis=1)
+                    rx += dphi.real
+                    ry -= dphi.imag
+                    counts[1] += 64
+                elif not nd["child"]:
+                    s, e = nd["s"], nd["e"]
+                    dx, dy = tx[:, None] - xs[None, s:e], ty[:, None] - ys[None, s:e]
+                    den = dx * dx + (dy * dy + EPS_REG)
+                    rx += np.sum(vs[None, s:e] * dx / den, axis=1)
+                    ry += np.sum(vs[None, s:e] * dy / den, axis=1)
+                    counts[0] += 64 * (e - s)
+                else:
+                    stack += [nd["child"] + c for c in range(4)]
+            zl = (tx - xb) + 1j * (ty - yb)
+            d = np.sum(l * loc * zl[:, None] ** (l - 1.0), axis=1)
+            rx += d.real
+            ry -= d.imag
+            counts[3] += 64
+            k = bx + ix + 32 * (by + iy)
+            fx[k], fy[k] = rx, ry
+    return fx, fy, counts
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_potential_reference_style_traversal(oracle, wl, theta):
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(3000, seed=23)
+    p, cap = 12, 24
+    t = oracle.Tree(xs, ys, vs, cap, p)
+    n = t.n
+    xsrt, ysrt, vsrt = t.xyv[:n], t.xyv[n:2 * n], t.xyv[2 * n:]
+    nodes = ref_style_geometry([int(k) for k in t.keys], xsrt, ysrt, vsrt, cap, p, t.bbox[2])
+    sel = np.arange(0, len(xt), 11)
+    ref, npairs, ne2p = ref_style_potential(nodes, xsrt, ysrt, vsrt, theta, xt[sel], yt[sel])
+    out, cnt = t.potential(theta, xt[sel], yt[sel])
+    assert (int(cnt[0]), int(cnt[1])) == (npairs, ne2p)
+    assert rel_l2(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_force_reference_style_traversal(oracle, wl, theta):
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian16_force(4000, nblocks=6, seed=24)
+    p, cap = 12, 16
+    t = oracle.Tree(xs, ys, vs, cap, p)
+    n = t.n
+    xsrt, ysrt, vsrt = t.xyv[:n], t.xyv[n:2 * n], t.xyv[2 * n:]
+    nodes = ref_style_geometry([int(k) for k in t.keys], xsrt, ysrt, vsrt, cap, p, t.bbox[2])
+    fx, fy, cnt = t.force(theta, x0s, y0s, hs)
+    total = np.zeros(4, np.int64)
+    for b in range(len(x0s)):
+        rx, ry, c = ref_style_force_block(nodes, xsrt, ysrt, vsrt, theta, x0s[b], y0s[b], hs[b], p)
+        total += c
+        sl = slice(1024 * b, 1024 * (b + 1))
+        assert rel_l2(fx[sl], rx) <= 1e-12 and rel_l2(fy[sl], ry) <= 1e-12, b
+    assert tuple(int(c) for c in cnt) == tuple(int(c) for c in total)
+    assert total[2] > 0 and total[1] > 0 and total[0] > 0
