@@ -378,11 +378,12 @@ class Context:
 
     def stats(self):
         t = np.zeros(4)
-        c = (ctypes.c_ulonglong * 6)()
+        c = (ctypes.c_ulonglong * 8)()
         info = np.zeros(4, np.int32)
         self._check(lib().tc_get_stats(self.h, _dp(t), c, info.ctypes.data_as(_I)))
         return {"times_ms": {"build": t[0], "tsort": t[1], "eval": t[2], "total": t[3]},
-                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3], "p2p_slots": c[4], "e2p_slots": c[5]},
+                "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3], "p2p_slots": c[4], "e2p_slots": c[5],
+                             "m2l_wave": c[6], "e2p_wave": c[7]},
                 "n": int(info[0]), "nnodes": int(info[1]), "ninternal": int(info[2]), "order": int(info[3])}
 
     def tree(self):

@@ -19,7 +19,7 @@ import numpy as np
 
 DEFAULT = {"potential": (12, 512), "force": (24, 192)}
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
-ORDERS = (6, 8, 10, 12, 14, 16, 20, 24, 32)
+ORDERS = (6, 8, 9, 10, 12, 14, 16, 20, 24, 32)
 LEAVES = (64, 128, 192, 256, 384, 512, 768, 1024)
 
 

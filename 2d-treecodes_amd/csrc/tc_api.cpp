@@ -47,7 +47,7 @@ struct tc_context {
     DevBuf io, cnt;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_valid = false, eval_has_tsort = false;
-    unsigned long long counters[TC_NCOUNTERS] = {0, 0, 0, 0, 0, 0};
+    unsigned long long counters[TC_NCOUNTERS] = {0};
     bool counters_on_host = false;  // the last call left its counters in `counters` (no-allocation path)
     // pinned staging for host arrays: two buffers, so the CPU copy of one
     // chunk overlaps the DMA of the other (allocated on first large copy)

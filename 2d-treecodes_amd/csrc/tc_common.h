@@ -26,7 +26,7 @@
 #define TC_MAX_LEAF (1 << 30)
 
 // Expansion orders with compiled kernels: X(p) for each.
-#define TC_ORDERS(X) X(6) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(32)
+#define TC_ORDERS(X) X(6) X(8) X(9) X(10) X(12) X(14) X(16) X(20) X(24) X(32)
 inline bool tc_order_supported(int p)
 {
 #define TC_ORDER_EQ(P) || p == P
@@ -43,10 +43,13 @@ static_assert(sizeof(TcNode) == 64, "TcNode must stay 64 B (one s_load_dwordx16)
 
 // P2P_SLOTS = 64 x leaf size per wave-level leaf visit and E2P_SLOTS = 64 per
 // wave-level multipole evaluation: the lane slots the masked loops occupy
-// (P2P / P2P_SLOTS and E2P / E2P_SLOTS are their SIMD efficiencies)
+// (P2P / P2P_SLOTS and E2P / E2P_SLOTS are their SIMD efficiencies).
+// M2L_WAVE: FP32 potential only, multipole-to-local translations about a wave's
+// target centre; E2P_WAVE: the (target, node) interactions they evaluate (a
+// part of E2P, which counts every accepted (target, node) pair in any mode).
 enum TcCounter {
     TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_CNT_P2P_SLOTS = 4, TC_CNT_E2P_SLOTS = 5,
-    TC_NCOUNTERS = 6
+    TC_CNT_M2L_WAVE = 6, TC_CNT_E2P_WAVE = 7, TC_NCOUNTERS = 8
 };
 
 struct TcError : std::runtime_error {
@@ -87,6 +90,15 @@ TC_DEV double tc_uniform(double v)
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
     const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Value of v in lane l (wave-uniform result).
+TC_DEV double tc_lane_value(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 

@@ -176,10 +176,94 @@ TC_DEV void e2p_force32(float zr, float zi, const float2 *__restrict__ ex, float
     gy = -fmaf(ur, vi, ui * vr);
 }
 
+// ------------------------------------------------------------------ FP32 wave-level local expansions
+// A node that every target of a wave accepts, and that lies far from the
+// wave's targets compared with their spread, is evaluated for the whole wave
+// at once: its multipole expansion is translated into a local (Taylor)
+// expansion about the centre c of the wave's targets (multipole-to-local),
+// the wave sums these local expansions, and each lane evaluates the sum at
+// its own target once at the end.  The accept decisions -- and so every
+// target's interaction list -- are exactly those of the per-target walk; only
+// the evaluation of those far interactions changes.
+//
+// Local coefficients, scaled by the wave radius R (max |t - c|) so that the
+// evaluation point zeta = (t - c) / R has |zeta| <= 1, with z0 = com - c,
+// w = s / z0 (s: the node's coefficient scale), v = R / z0, b_k the node's
+// scaled coefficients (a_k = b_k s^k):
+//   L_0 = Q log|z0| + Re sum_k (-1)^k b_k w^k
+//   L_l = v^l (-Q / l + sum_k (-1)^k C(l + k - 1, l) b_k w^k),   l = 1 .. 15
+// (the Taylor coefficients of Q log(z - com) + sum_k a_k (z - com)^-k about c).
+// The series converges for |t - c| < |z0| - r; a node qualifies when
+// R <= 0.3 (|z0| - r), so truncating at l = 15 leaves at most 0.3^16 ~ 4e-9 of
+// the node's field.  A node qualifies only if the whole wave reaches it and
+// |z0| - R > r / theta' with theta' = theta (1 - 1e-9): then every target's
+// distance d >= |z0| - R satisfies r < theta' d, so every lane's own FP64
+// test r^2 < theta^2 d^2 accepts it (the margin covers the roundings).
+constexpr int M2L_G = 16;            // lanes per node; local order M2L_G - 1
+constexpr double M2L_RHO = 0.3;      // R <= M2L_RHO (|z0| - r)
+constexpr double M2L_SLACK = 1.0 + 1e-9;  // relative margin of the squared distance test
+constexpr int M2L_Q = 128;           // queued translations per wave (a full queue: nodes take the per-lane path)
+
+// One pass of four queued nodes: lane (g, l) = (lane / 16, lane % 16) adds
+// L_l of node g (coefficients staged in co, z0 in z) to (lre, lim); qs is
+// (2 / ln 2) / l, so cq.y qs = Q / l (0 on lanes l = 0).
+template <int P>
+TC_DEV void m2l_wave_pass(int nq, const float2 *z, const float2 *co, float rw, float qs, const float *s_beta,
+                          double &lre, double &lim)
+{
+#pragma clang fp contract(fast)
+    const int lane = tc_lane();
+    const int g = lane / M2L_G, l = lane % M2L_G;
+    if (g >= nq)
+        return;
+    const float2 zz = z[g];
+    const float z2 = fmaf(zz.x, zz.x, zz.y * zz.y);
+    const float inv = fast_rcp(z2);
+    const float ur = zz.x * inv, ui = -zz.y * inv;
+    co += g * (P + 1);
+    const float2 cq = co[P];  // (s, Q ln2 / 2)
+    const float wr = cq.x * ur, wi = cq.x * ui;
+    const float *be = s_beta + l * P;
+    float tr = 0.f, ti = 0.f;
+#pragma unroll 2
+    for (int k = P; k >= 1; --k) {
+        const float ar = fmaf(be[k - 1], co[k - 1].x, tr), ai = fmaf(be[k - 1], co[k - 1].y, ti);
+        tr = fmaf(ar, wr, -ai * wi);
+        ti = fmaf(ar, wi, ai * wr);
+    }
+    float re, im;
+    if (l == 0) {
+        re = fmaf(cq.y, fast_log2(z2), tr);
+        im = 0.f;
+    } else {
+        // v^l by binary powering, v = R / z0
+        float br = rw * ur, bi = rw * ui, pr = 1.f, pi = 0.f;
+#pragma unroll
+        for (int e = 1; e < M2L_G; e <<= 1) {
+            if (l & e) {
+                const float t = fmaf(pr, br, -pi * bi);
+                pi = fmaf(pr, bi, pi * br);
+                pr = t;
+            }
+            const float t = fmaf(br, br, -bi * bi);
+            bi = 2.f * br * bi;
+            br = t;
+        }
+        const float cr = fmaf(-cq.y, qs, tr);
+        re = fmaf(pr, cr, -pi * ti);
+        im = fmaf(pr, ti, pi * cr);
+    }
+    lre += (double)re;
+    lim += (double)im;
+}
+
 // EvaluatePotential::evaluate (treecode-potential.cpp:25-86) for targets
 // tperm[64 w .. 64 w + 63] on wave w: masked DFS with the accept test
 // r^2 < theta^2 d^2 per lane, E2P for accepting lanes, P2P at leaves.
-template <int P, bool F32>
+// FP32: nodes that qualify (above) go through the wave's local expansion.
+// CNT: count interactions (a separate instance, so uncounted solves carry no
+// counter arithmetic)
+template <int P, bool F32, bool CNT>
 __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                    const float2 *__restrict__ exps32,
                                                    const double4 *__restrict__ src,
@@ -191,6 +275,21 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     __shared__ int s_node[4][TC_STACK];
     __shared__ unsigned long long s_mask[4][TC_STACK];
     __shared__ double4 s_src[4][64];
+    // FP32 wave-level local expansions: s_beta[l P + k - 1] = (-1)^k C(l + k - 1, l),
+    // queued node ids and their staged coefficients per wave
+    __shared__ float s_beta[F32 ? M2L_G * P : 1];
+    __shared__ int s_q[4][F32 ? M2L_Q : 1];
+    __shared__ double4 s_wc[4];  // FP32: {centre x, centre y, radius} of each wave's targets, 1 / theta'
+    __shared__ float2 s_qz[4][F32 ? M2L_Q : 1];  // FP32: z0 = com - centre of each queued node
+    static_assert(!F32 || 4 * (P + 1) * sizeof(float2) <= sizeof(s_src[0]), "staged coefficients fit a source tile");
+    if constexpr (F32) {
+        for (int i = threadIdx.x; i < M2L_G * P; i += 256) {
+            const int l = i / P, kk = i - l * P + 1;
+            const double c = c_binomT[l * 64 + l + kk - 1];
+            s_beta[i] = (float)((kk & 1) ? -c : c);
+        }
+        __syncthreads();
+    }
     const int wid = threadIdx.x >> 6, lane = tc_lane();
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool valid = k < m;
@@ -199,6 +298,29 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     const unsigned long long live = tc_ballot(valid);
     if (!live)
         return;
+    // FP32: centre (bounding-box middle) and radius of the wave's targets
+    int nq = 0;
+    unsigned long long nm2l = 0, ne2pw = 0;
+    if constexpr (F32) {
+        const int first = __builtin_ctzll(live);
+        const double fx = __shfl(x, first, 64), fy = __shfl(y, first, 64);
+        const double lx = valid ? x : fx, ly = valid ? y : fy;
+        double mnx = lx, mxx = lx, mny = ly, mxy = ly;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double a = __shfl_xor(mnx, off, 64), b = __shfl_xor(mxx, off, 64);
+            const double c = __shfl_xor(mny, off, 64), d = __shfl_xor(mxy, off, 64);
+            mnx = a < mnx ? a : mnx;
+            mxx = b > mxx ? b : mxx;
+            mny = c < mny ? c : mny;
+            mxy = d > mxy ? d : mxy;
+        }
+        const double cwx = 0.5 * (mnx + mxx), cwy = 0.5 * (mny + mxy);
+        const double ddx = lx - cwx, ddy = ly - cwy;
+        const double rwave = sqrt(tc_wave_max(ddx * ddx + ddy * ddy));
+        if (lane == 0)
+            s_wc[wid] = make_double4(cwx, cwy, rwave, th2 > 0 ? 1.0 / (sqrt(th2) * (1.0 - 1e-9)) : 0.0);
+    }
     double acc = 0;
     // FP32 E2P results are summed in FP32 over 8 of the wave's evaluations, then
     // flushed into the FP64 accumulator (each term is already an FP32 value)
@@ -214,13 +336,15 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         tc_wave_sync();
         const int id = tc_uniform(s_node[wid][sp]);
         const unsigned long long mask = tc_uniform64(s_mask[wid][sp]);
+        // no early exit for empty nodes: all fields of the record are needed
+        // before any branch, so they arrive in one round of scalar loads (an
+        // empty node leaves no lane accepting or opening it)
         const TcNode nd = nodes[id];
-        if (nd.s == nd.e)
-            continue;
-        const bool in = (mask >> lane) & 1ull;
+        const bool nonempty = nd.s != nd.e;
+        const bool in = ((mask >> lane) & 1ull) && nonempty;
         const double dx = x - nd.xcom, dy = y - nd.ycom;
         const double r2 = dx * dx + dy * dy;
-        const bool acc_ok = in && (nd.r * nd.r < th2 * r2);
+        const bool acc_ok = in & (nd.r * nd.r < th2 * r2);  // no short circuit: r arrives with the rest
         const unsigned long long amask = tc_ballot(acc_ok);
         if (acc_ok) {
             if (F32)
@@ -233,22 +357,26 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
             acc_e2p = 0.f;
             n_e2p = 0;
         }
-        ne2p += __popcll(amask);
-        neslots += amask ? 64ull : 0ull;
-        const unsigned long long omask = mask & ~amask;
+        if constexpr (CNT) {
+            ne2p += __popcll(amask);
+            neslots += amask ? 64ull : 0ull;
+        }
+        const unsigned long long omask = nonempty ? mask & ~amask : 0ull;
         if (!omask)
             continue;
         if (nd.child == 0) {
             const bool act = (omask >> lane) & 1ull;
             const int nact = __popcll(omask), nsrc = nd.e - nd.s, nchunk = (nsrc + 63) >> 6;
-            np2p += (unsigned long long)nact * (unsigned long long)nsrc;
+            if constexpr (CNT)
+                np2p += (unsigned long long)nact * (unsigned long long)nsrc;
             // Few lanes need this leaf: lanes take sources instead and the wave
             // loops over the active targets (8 at a time), one reduction per
             // target.  Chosen when k (C 9.5 + 16.5) < n 9.8, the measured
             // nanoseconds per SIMD of the two forms (FP32 op 1.18 ns, log 3.6 ns;
             // C chunks of 64 of the n sources, k active lanes).
             if (F32 && (long long)nact * (nchunk * 19 + 33) < (long long)nsrc * 20) {
-                nslots += 64ull * (unsigned long long)nchunk * (unsigned long long)nact;
+                if constexpr (CNT)
+                    nslots += 64ull * (unsigned long long)nchunk * (unsigned long long)nact;
                 const float txl = (float)(x - nd.xcom), tyl = (float)(y - nd.ycom);
                 unsigned long long rest = omask;
                 double part = 0;
@@ -299,7 +427,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 // which keeps the LDS well below the VALU's rate even when few
                 // waves are resident.  Partner sums are exchanged at the end of
                 // the leaf.
-                nslots += 64ull * (unsigned long long)nsrc;
+                if constexpr (CNT)
+                    nslots += 64ull * (unsigned long long)nsrc;
                 const float tx = (float)(x - nd.xcom), ty = (float)(y - nd.ycom);
                 const float ux = __shfl_xor(tx, 32, 64), uy = __shfl_xor(ty, 32, 64);
                 const int half = lane >> 5;
@@ -346,7 +475,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 if (act)
                     acc += leafsum * (double)LN2_HALF;
             } else {
-                nslots += 64ull * (unsigned long long)nsrc;
+                if constexpr (CNT)
+                    nslots += 64ull * (unsigned long long)nsrc;
                 double leafsum = 0, leafsum2 = 0;
                 for (int b0 = nd.s; b0 < nd.e; b0 += 64) {
                     const int cnt = min(64, nd.e - b0);
@@ -374,6 +504,57 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 acc += (leafsum + leafsum2) * 0.5;
             }
         } else {
+            if constexpr (F32) {
+                if (omask == live) {
+                    // the whole wave opens this node: children that qualify for the
+                    // wave's local expansion are queued, the others pushed (order
+                    // 0..3 kept), empty ones dropped (they contribute nothing)
+                    bool m2l = false, push = false;
+                    float2 z0 = make_float2(0.f, 0.f);
+                    tc_wave_sync();
+                    if (lane < 4) {
+                        const double4 wc = s_wc[wid];
+                        const TcNode *cn = nodes + nd.child + lane;
+                        const int2 cse = *reinterpret_cast<const int2 *>(&cn->s);
+                        const double4 cg = *reinterpret_cast<const double4 *>(cn);
+                        if (cse.x < cse.y) {
+                            // |z0| > R + r / theta' and |z0| > r + R / rho, compared squared
+                            // with a relative margin (wc.w = 1 / theta')
+                            const double ddx = cg.x - wc.x, ddy = cg.y - wc.y;
+                            const double d2 = ddx * ddx + ddy * ddy;
+                            const double a = fmax(wc.z + cg.z * wc.w, cg.z + wc.z * (1.0 / M2L_RHO));
+                            m2l = th2 > 0 && d2 > a * a * M2L_SLACK;  // theta 0 accepts nothing
+                            push = !m2l;
+                            z0 = make_float2((float)ddx, (float)ddy);
+                        }
+                    }
+                    // a full queue sends the rest down the per-lane path
+                    if (m2l && nq + 4 > M2L_Q) {
+                        m2l = false;
+                        push = true;
+                    }
+                    const unsigned long long mm = tc_ballot(m2l), pm = tc_ballot(push);
+                    if (m2l) {
+                        const int qi = nq + __popcll(mm & tc_lanemask_lt());
+                        s_q[wid][qi] = nd.child + lane;
+                        s_qz[wid][qi] = z0;
+                    }
+                    if (push) {
+                        const int pos = sp + __popcll(pm & tc_lanemask_lt());
+                        s_node[wid][pos] = nd.child + lane;
+                        s_mask[wid][pos] = omask;
+                    }
+                    sp += __popcll(pm);
+                    const int nadd = __popcll(mm);
+                    nq += nadd;
+                    if constexpr (CNT) {
+                        nm2l += nadd;
+                        ne2pw += (unsigned long long)nadd * __popcll(live);
+                        ne2p += (unsigned long long)nadd * __popcll(live);
+                    }
+                    continue;
+                }
+            }
             // children pushed in order 0..3, popped 3..0 (treecode-potential.cpp:76-79)
             if (lane < 4) {
                 s_node[wid][sp + lane] = nd.child + lane;
@@ -382,13 +563,70 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
             sp += 4;
         }
     }
+    if constexpr (F32) {
+        if (nq > 0) {
+            // the queued translations, four nodes per pass (the source tile's
+            // LDS stages their coefficients)
+            tc_wave_sync();
+            float2 *s_coef = reinterpret_cast<float2 *>(&s_src[wid][0]);
+            const double4 wc = s_wc[wid];
+            const double cwx = wc.x, cwy = wc.y, rwave = wc.z;
+            const int ll = lane % M2L_G;
+            const float qs = ll ? (1.f / LN2_HALF) / (float)ll : 0.f;
+            double lre = 0, lim = 0;
+            // coefficients of the next pass are fetched while the current one is computed
+            constexpr int NST = (4 * (P + 1) + 63) / 64;
+            float2 st[NST];
+            auto fetch = [&](int q0) {
+                const int nqq = min(4, nq - q0);
+#pragma unroll
+                for (int j = 0; j < NST; ++j) {
+                    const int i = lane + 64 * j, g = i / (P + 1), kk = i - g * (P + 1);
+                    st[j] = g < nqq ? exps32[(size_t)(P + 1) * s_q[wid][q0 + g] + kk] : make_float2(0.f, 0.f);
+                }
+            };
+            fetch(0);
+            for (int q0 = 0; q0 < nq; q0 += 4) {
+#pragma unroll
+                for (int j = 0; j < NST; ++j)
+                    if (lane + 64 * j < 4 * (P + 1))
+                        s_coef[lane + 64 * j] = st[j];
+                tc_wave_sync();
+                if (q0 + 4 < nq)
+                    fetch(q0 + 4);
+                m2l_wave_pass<P>(min(4, nq - q0), s_qz[wid] + q0, s_coef, (float)rwave, qs, s_beta, lre, lim);
+                tc_wave_sync();
+            }
+            // sum the four node groups' partial coefficients, then every lane
+            // evaluates sum_l L_l zeta^l at its own target (Horner, FP64)
+            lre += __shfl_xor(lre, 16, 64);
+            lre += __shfl_xor(lre, 32, 64);
+            lim += __shfl_xor(lim, 16, 64);
+            lim += __shfl_xor(lim, 32, 64);
+            const double sc = rwave > 0 ? 1.0 / rwave : 0.0;
+            const double zr = (x - cwx) * sc, zi = (y - cwy) * sc;
+            double hr = 0, hi = 0;
+#pragma unroll 1
+            for (int l = M2L_G - 1; l >= 0; --l) {
+                const double cr = tc_lane_value(lre, l), ci = tc_lane_value(lim, l);
+                const double t = hr * zr - hi * zi + cr;
+                hi = hr * zi + hi * zr + ci;
+                hr = t;
+            }
+            acc += hr;
+        }
+    }
     if (valid)
         out[ti] = acc + (double)acc_e2p;
-    if (counters && lane == 0) {
+    if (CNT && lane == 0) {
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_E2P], ne2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], nslots);
         atomicAdd(&counters[TC_CNT_E2P_SLOTS], neslots);
+        if (F32) {
+            atomicAdd(&counters[TC_CNT_M2L_WAVE], nm2l);
+            atomicAdd(&counters[TC_CNT_E2P_WAVE], ne2pw);
+        }
     }
 }
 
@@ -1203,10 +1441,12 @@ __global__ __launch_bounds__(256) void k_force_targets(const TcNode *__restrict_
         tc_wave_sync();
         const int id = tc_uniform(s_node[wid][sp]);
         const unsigned long long mask = tc_uniform64(s_mask[wid][sp]);
+        // no early exit for empty nodes: all fields of the record are needed
+        // before any branch, so they arrive in one round of scalar loads (an
+        // empty node leaves no lane accepting or opening it)
         const TcNode nd = nodes[id];
-        if (nd.s == nd.e)
-            continue;
-        const bool in = (mask >> lane) & 1ull;
+        const bool nonempty = nd.s != nd.e;
+        const bool in = ((mask >> lane) & 1ull) && nonempty;
         const double dx = x - nd.xcom, dy = y - nd.ycom;
         const double r2 = dx * dx + dy * dy;
         const bool acc_ok = in && (nd.r * nd.r < th2 * r2);
@@ -1223,7 +1463,7 @@ __global__ __launch_bounds__(256) void k_force_targets(const TcNode *__restrict_
         }
         ne2p += __popcll(amask);
         neslots += amask ? 64ull : 0ull;
-        const unsigned long long omask = mask & ~amask;
+        const unsigned long long omask = nonempty ? mask & ~amask : 0ull;
         if (!omask)
             continue;
         if (nd.child == 0) {
@@ -1348,14 +1588,20 @@ void launch_potential(const TcTree &t, double th2, const double *xt, const doubl
                       int m, double *out, int precision, unsigned long long *counters, hipStream_t st)
 {
     const int grid = tiles_of(m, 256);
-    if (precision == 1)
-        k_potential<P, true><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(),
-                                                   t.exps32.as<float2>(), t.src.as<double4>(),
-                                                   t.src32.as<float4>(), xt, yt, tperm, m, th2, out, counters);
+    auto go = [&](auto kern, const float2 *e32, const float4 *s32) {
+        kern<<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), e32, t.src.as<double4>(), s32, xt, yt,
+                                   tperm, m, th2, out, counters);
+    };
+    const float2 *e32 = t.exps32.as<float2>();
+    const float4 *s32 = t.src32.as<float4>();
+    if (precision == 1 && counters)
+        go(k_potential<P, true, true>, e32, s32);
+    else if (precision == 1)
+        go(k_potential<P, true, false>, e32, s32);
+    else if (counters)
+        go(k_potential<P, false, true>, nullptr, nullptr);
     else
-        k_potential<P, false><<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), nullptr,
-                                                    t.src.as<double4>(), nullptr, xt, yt, tperm, m, th2, out,
-                                                    counters);
+        go(k_potential<P, false, false>, nullptr, nullptr);
     TC_HIP(hipGetLastError());
 }
 

@@ -151,11 +151,13 @@ TC_API int tc_potential_resident(tc_context *ctx, double theta, const double *xt
 TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x0s, const double *y0s,
                                   const double *hs, int nblocks, double *xr, double *yr, int device_ptrs);
 
-/* Times of the last call's phases (requires TC_OPT_PROFILE), counters[6] (requires
+/* Times of the last call's phases (requires TC_OPT_PROFILE), counters[8] (requires
  * TC_OPT_COUNTERS): {target-source pairs evaluated directly, (target, node) multipole
- * evaluations, (brick, node) multipole-to-local shifts, targets given a local expansion,
- * lane slots occupied by the direct loops, lane slots occupied by multipole evaluations},
- * and tree_info = {n, nnodes, ninternal, order}. */
+ * interactions (the accepted pairs, equal in both precisions), (brick, node) multipole-to-local
+ * shifts, targets given a local expansion, lane slots occupied by the direct loops, lane slots
+ * occupied by per-target multipole evaluations, multipole-to-local translations about a
+ * wavefront's target centre (FP32 potential), (target, node) interactions those translations
+ * evaluate}, and tree_info = {n, nnodes, ninternal, order}. */
 TC_API int tc_get_stats(tc_context *ctx, double *times_ms, unsigned long long *counters, int *tree_info);
 
 /* Copy the resident tree to host memory (any pointer may be NULL):

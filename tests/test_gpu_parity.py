@@ -118,7 +118,7 @@ def test_potential_fp64_vs_oracle(pkg, oracle, wl, ctx64, theta):
     assert (st["p2p"], st["e2p"]) == (cnt[0], cnt[1])
 
 
-@pytest.mark.parametrize("order,leaf", [(6, 256), (10, 256), (14, 128), (20, 64), (32, 512)])
+@pytest.mark.parametrize("order,leaf", [(6, 256), (9, 192), (10, 256), (14, 128), (20, 64), (32, 512)])
 def test_potential_orders_vs_oracle(pkg, oracle, wl, order, leaf):
     # every compiled expansion order (tuned configurations use 10 / 256 and 10 / 64)
     ctx = pkg.Context(counters=True, pot_order=order, pot_leaf=leaf)
@@ -131,7 +131,7 @@ def test_potential_orders_vs_oracle(pkg, oracle, wl, order, leaf):
     ctx.close()
 
 
-@pytest.mark.parametrize("order,leaf", [(8, 64), (12, 128), (12, 192), (16, 192), (20, 128)])
+@pytest.mark.parametrize("order,leaf", [(8, 64), (9, 96), (12, 128), (12, 192), (16, 192), (20, 128)])
 def test_force_orders_vs_oracle(pkg, oracle, wl, order, leaf):
     # E2L runs four nodes per wave for orders <= 16 and two above
     ctx = pkg.Context(counters=True, force_order=order, force_leaf=leaf)
@@ -397,6 +397,33 @@ def test_fp32_potential_accuracy(pkg, oracle, wl, ctx32, ctx64):
     assert rel_l2(out32, out64) <= FP32_TOL
 
 
+def test_fp32_wave_local_expansions(pkg, oracle, wl, ctx32, ctx64):
+    # FP32: far nodes that a whole wave accepts are evaluated through one local expansion
+    # about the wave's target centre; interaction counts stay those of the per-target walk
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(1 << 18, seed=21)
+    out32 = ctx32.potential(0.5, xs, ys, vs, xt, yt)
+    c32 = ctx32.stats()["counters"]
+    out64 = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+    c64 = ctx64.stats()["counters"]
+    assert (c32["p2p"], c32["e2p"]) == (c64["p2p"], c64["e2p"])
+    assert c32["m2l_wave"] > 0 and 4 * c32["e2p_wave"] > c32["e2p"]
+    assert c64["m2l_wave"] == 0 and c64["e2p_wave"] == 0
+    assert rel_l2(out32, out64) <= FP32_TOL
+    # degenerate waves: a single target, 64 copies of one point (zero radius), a wave of
+    # targets outside the box, a ring of targets around the sources
+    st = wl.Stream(22)
+    ang = 2 * np.pi * st.uniform(3000)
+    cases = [(xt[:1], yt[:1]), (np.full(64, 0.3), np.full(64, 0.7)),
+             (1.6 + 0.01 * st.uniform(500), -0.4 + 0.01 * st.uniform(500)),
+             (0.5 + 0.9 * np.cos(ang), 0.5 + 0.9 * np.sin(ang))]
+    for i, (tx, ty) in enumerate(cases):
+        a = ctx32.potential(0.5, xs, ys, vs, tx, ty)
+        b = ctx64.potential(0.5, xs, ys, vs, tx, ty)
+        assert rel_l2(a, b) <= FP32_TOL, i
+        d = oracle.direct_potential(xs, ys, vs, tx, ty, np.arange(len(tx), dtype=np.int32))
+        assert rel_l2(a, d) <= FP32_TOL, i
+
+
 def test_fp32_force_accuracy(pkg, oracle, wl, ctx32, ctx64):
     xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(1 << 18, nbx=16, h=1.0 / 512, seed=17)
     fx32, fy32 = ctx32.force(0.5, xs, ys, vs, x0s, y0s, hs)
@@ -523,9 +550,11 @@ def test_force_targets(pkg, oracle, wl, ctx64, ctx32, precision):
 def test_target_order_invariance(pkg, wl, ctx64, ctx32):
     # FP64: each lane accumulates in its own target's DFS order, so results do not depend on
     # how targets are grouped into waves: a permutation of the inputs permutes the outputs
-    # bit for bit.  FP32: a leaf that few lanes of a wave need is summed source-parallel
-    # (another rounding order), and which leaves those are depends on the grouping, so a
-    # permutation (which regroups targets whose sort keys tie) may move the last bits.
+    # bit for bit.  FP32: a leaf that few lanes of a wave need is summed source-parallel, and
+    # far nodes a whole wave accepts go through the wave's local expansion; both depend on
+    # the grouping, so a permutation (which regroups targets whose sort keys tie) changes the
+    # rounding of those terms: bounded by FP32 rounding of the far field, 1e-7 (a tenth of the
+    # FP32 path's error bar).
     xs, ys, vs, xt, yt = wl.mixed_plummer_potential(50000, seed=66)
     perm = np.random.default_rng(1).permutation(len(xt))
     a = ctx64.potential(0.5, xs, ys, vs, xt, yt)
@@ -533,4 +562,5 @@ def test_target_order_invariance(pkg, wl, ctx64, ctx32):
     assert np.array_equal(a[perm], b)
     a = ctx32.potential(0.5, xs, ys, vs, xt, yt)
     b = ctx32.potential(0.5, xs, ys, vs, xt[perm], yt[perm])
-    assert rel_l2(b, a[perm]) < 1e-9
+    print("FP32 permutation difference %.3e" % rel_l2(b, a[perm]))
+    assert rel_l2(b, a[perm]) < 1e-7
