@@ -34,6 +34,7 @@ EXPORTED = [
     "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
     "tc_server_wait", "tc_server_stats", "treecode_potential_float", "treecode_force_mrag_float",
     "treecode_velocity_mrag", "tc_copy_kernel", "treecode_force", "tc_force_targets", "tc_potential_noalloc",
+    "tc_get_profile",
 ]
 NOALLOC_MAX_POINTS = 1 << 18  # sources and targets per tc_potential_noalloc call
 NOALLOC_SCRATCH_BYTES = 64 << 20  # its static device scratch
@@ -147,6 +148,8 @@ def lib():
         L.tc_force_targets.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P, _P, it]
     if hasattr(L, 'tc_copy_kernel'):
         L.tc_copy_kernel.argtypes = [_P, _P, _P, ctypes.c_size_t]
+    if hasattr(L, 'tc_get_profile'):
+        L.tc_get_profile.argtypes = [_P, _D, ctypes.POINTER(ctypes.c_ulonglong)]
     if hasattr(L, 'tc_potential_noalloc'):
         L.tc_potential_noalloc.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it, _P]
     if hasattr(L, 'tc_direct_potential'):
@@ -385,6 +388,18 @@ class Context:
                 "counters": {"p2p": c[0], "e2p": c[1], "e2l": c[2], "l2p": c[3], "p2p_slots": c[4], "e2p_slots": c[5],
                              "m2l_wave": c[6], "e2p_wave": c[7]},
                 "n": int(info[0]), "nnodes": int(info[1]), "ninternal": int(info[2]), "order": int(info[3])}
+
+    def profile(self):
+        """Build phases (ms) and the traversal's clock cycles per category (tc_get_profile)."""
+        b = np.zeros(4)
+        c = (ctypes.c_ulonglong * 5)()
+        self._check(lib().tc_get_profile(self.h, _dp(b), c))
+        cyc = dict(zip(("p2p", "e2p", "local", "l2p", "total"), (int(v) for v in c)))
+        walk = cyc["total"] - cyc["p2p"] - cyc["e2p"] - cyc["local"] - cyc["l2p"]
+        shares = {k: (v / cyc["total"] if cyc["total"] else 0.0) for k, v in cyc.items() if k != "total"}
+        shares["walk"] = walk / cyc["total"] if cyc["total"] else 0.0
+        return {"build_ms": dict(zip(("keys", "sort", "topology", "upward"), (float(v) for v in b))),
+                "cycles": cyc, "cycle_shares": shares}
 
     def tree(self):
         st = self.stats()

@@ -125,14 +125,22 @@ class Tuner:
         base = [t for t in table if t["order"] == p0 and t["leaf"] == l0]
         if err_target is None:
             err_target = min(1e-6, max(base[0]["err"], 1e-15) * 1.01) if base else 1e-6
-        ok = [t for t in table if t["err"] <= err_target] or table
+        ok = [t for t in table if t["err"] <= err_target]
+        met = bool(ok)
+        if not met:
+            # nothing in the grid meets the target: keep the reference's parameters (if
+            # measured, else the most accurate cell) and say so, rather than the fastest
+            ok = base or [min(table, key=lambda t: t["err"])]
+            if log:
+                log("tune %s: no cell within %.2e; keeping order %d leaf %d" % (kind, err_target, ok[0]["order"],
+                                                                              ok[0]["leaf"]))
         best = min(ok, key=lambda t: t["ms"])
         res = {"order": best["order"], "leaf": best["leaf"], "ms": best["ms"], "err": best["err"],
-               "err_target": err_target, "table": table}
+               "err_target": err_target, "met_target": met, "table": table}
         n = len(data["xs"])
         if save:
             self.cache[bucket_key(kind, precision, theta, n, m)] = {k: res[k] for k in ("order", "leaf", "ms", "err",
-                                                                                         "err_target")}
+                                                                                         "err_target", "met_target")}
             self._save()
         ctx.close()
         return res

@@ -46,7 +46,9 @@ struct tc_context {
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     DevBuf io, cnt;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool ev_valid = false, eval_has_tsort = false;
+    // build phase boundaries: after the keys, the sorted records, the topology, the upward pass
+    hipEvent_t bev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ev_valid = false, eval_has_tsort = false, bev_valid = false;
     unsigned long long counters[TC_NCOUNTERS] = {0};
     bool counters_on_host = false;  // the last call left its counters in `counters` (no-allocation path)
     // pinned staging for host arrays: two buffers, so the CPU copy of one
@@ -285,6 +287,18 @@ unsigned long long *prepare_counters(tc_context *c)
     return c->cnt.as<unsigned long long>();
 }
 
+// All counters of the last call (zeros when counting was off).
+void read_counters(tc_context *c, unsigned long long *all)
+{
+    if (c->count && c->counters_on_host)
+        std::memcpy(all, c->counters, sizeof(unsigned long long) * TC_NCOUNTERS);
+    else if (c->count && c->cnt.ptr)
+        TC_HIP(hipMemcpy(all, c->cnt.ptr, sizeof(unsigned long long) * TC_NCOUNTERS, hipMemcpyDeviceToHost));
+    else
+        for (int i = 0; i < TC_NCOUNTERS; ++i)
+            all[i] = 0;
+}
+
 void record(tc_context *c, int i)
 {
     if (c->profile)
@@ -301,7 +315,10 @@ void build(tc_context *c, const double *xs, const double *ys, const double *vs, 
     c->tree_kind = TC_KIND_NONE;
     c->tree.nnodes = 0;
     c->tree.has_fp32 = false;
-    tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws, c->stream);
+    c->bev_valid = false;
+    tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws, c->stream,
+                  c->profile ? c->bev : nullptr);
+    c->bev_valid = c->profile;
     c->tree_kind = kind;
     c->tree_precision = c->precision;
 }
@@ -341,8 +358,10 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             c->tree_kind = TC_KIND_NONE;
             return;
         }
-        if (!rebuild)
+        if (!rebuild) {
             check_resident(c, TC_KIND_POTENTIAL);
+            c->bev_valid = false;
+        }
         const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
         double *dout = out;
         if (!device_ptrs) {
@@ -502,8 +521,10 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
             c->tree_kind = TC_KIND_NONE;
             return;
         }
-        if (!rebuild)
+        if (!rebuild) {
             check_resident(c, TC_KIND_FORCE);
+            c->bev_valid = false;
+        }
         const double *dxs = xs, *dys = ys, *dvs = vs, *dx0 = x0s, *dy0 = y0s, *dh = hs;
         double *dxr = xr, *dyr = yr;
         if (!device_ptrs) {
@@ -689,6 +710,8 @@ TC_API tc_context *tc_context_create(int device)
         TC_HIP(hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking));
         for (auto &e : c->ev)
             TC_HIP(hipEventCreate(&e));
+        for (auto &e : c->bev)
+            TC_HIP(hipEventCreate(&e));
         TC_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
         TC_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
         ensure_binomials(device, c->stream);
@@ -726,6 +749,9 @@ TC_API void tc_context_destroy(tc_context *c)
     c->na_tws.release();
     c->na_cnt.release();
     for (auto &e : c->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    for (auto &e : c->bev)
         if (e)
             (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b) {
@@ -856,20 +882,40 @@ TC_API int tc_get_stats(tc_context *c, double *times_ms, unsigned long long *cou
             }
         }
         if (counters) {
-            if (c->count && c->counters_on_host)
-                std::memcpy(counters, c->counters, sizeof(unsigned long long) * TC_NCOUNTERS);
-            else if (c->count && c->cnt.ptr)
-                TC_HIP(hipMemcpy(counters, c->cnt.ptr, sizeof(unsigned long long) * TC_NCOUNTERS,
-                                 hipMemcpyDeviceToHost));
-            else
-                for (int i = 0; i < TC_NCOUNTERS; ++i)
-                    counters[i] = 0;
+            unsigned long long all[TC_NCOUNTERS];
+            read_counters(c, all);
+            std::memcpy(counters, all, sizeof(unsigned long long) * TC_NPUBLIC_COUNTERS);
         }
         if (tree_info) {
             tree_info[0] = c->tree.n;
             tree_info[1] = c->tree.nnodes;
             tree_info[2] = c->tree.ninternal;
             tree_info[3] = c->tree.order;
+        }
+    });
+}
+
+TC_API int tc_get_profile(tc_context *c, double *build_ms, unsigned long long *cycles)
+{
+    return guarded(c, [&] {
+        TC_HIP(hipStreamSynchronize(c->stream));
+        if (build_ms) {
+            for (int i = 0; i < 4; ++i)
+                build_ms[i] = 0;
+            if (c->ev_valid && c->bev_valid) {
+                const hipEvent_t marks[5] = {c->ev[0], c->bev[0], c->bev[1], c->bev[2], c->bev[3]};
+                for (int i = 0; i < 4; ++i) {
+                    float ms = 0;
+                    TC_HIP(hipEventElapsedTime(&ms, marks[i], marks[i + 1]));
+                    build_ms[i] = ms;
+                }
+            }
+        }
+        if (cycles) {
+            unsigned long long all[TC_NCOUNTERS];
+            read_counters(c, all);
+            for (int i = 0; i < 5; ++i)
+                cycles[i] = all[TC_CNT_CYC_P2P + i];
         }
     });
 }

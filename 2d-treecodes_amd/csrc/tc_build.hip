@@ -1258,7 +1258,8 @@ static const uint32_t *tc_bbox_bad_flag(TcScratch &ws, int n)
 }
 
 void tc_sort_points(const double *x, const double *y, const double *v, int n, double *box_d, uint32_t *keys_out,
-                    uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert)
+                    uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert,
+                    const hipEvent_t *ev)
 {
     if (n <= SS_N) {
         ws.bbox_part.reserve(sizeof(double4) + 256);
@@ -1268,6 +1269,11 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
         else
             k_sort_small<false><<<1, SS_NT, 0, st>>>(x, y, v, n, box_d, bad, keys_out, perm_out, src_out);
         TC_HIP(hipGetLastError());
+        if (ev) {
+            // one kernel does keys and sort: both boundaries fall after it
+            TC_HIP(hipEventRecord(ev[0], st));
+            TC_HIP(hipEventRecord(ev[1], st));
+        }
         return;
     }
     tc_bbox(x, y, n, box_d, ws, st);
@@ -1276,16 +1282,18 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
         ws.packed.reserve(sizeof(double4) * (size_t)n);
         packed = ws.packed.as<double4>();
     }
-    tc_morton_sort(x, y, v, n, box_d, keys_out, perm_out, packed, ws, st, hilbert);
+    tc_morton_sort(x, y, v, n, box_d, keys_out, perm_out, packed, ws, st, hilbert, ev);
     if (src_out) {
         k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(packed, perm_out, n, src_out);
         TC_HIP(hipGetLastError());
     }
+    if (ev)
+        TC_HIP(hipEventRecord(ev[1], st));
 }
 
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
-                    bool hilbert)
+                    bool hilbert, const hipEvent_t *ev)
 {
     ws.k2.reserve(sizeof(uint32_t) * (size_t)n);
     ws.v2.reserve(sizeof(uint32_t) * (size_t)n);
@@ -1294,6 +1302,8 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
     else
         k_keys<<<tiles_of(n, 256), 256, 0, st>>>(x, y, v, n, box_d, keys_out, packed);
     TC_HIP(hipGetLastError());
+    if (ev)
+        TC_HIP(hipEventRecord(ev[0], st));
     const int ntiles = tiles_of(n, RS_TILE);
     ws.hist.reserve(sizeof(uint32_t) * (size_t)RS_BINS * ntiles * 2);
     uint32_t *hist = ws.hist.as<uint32_t>();
@@ -1355,7 +1365,7 @@ static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, cons
 }
 
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
-                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st)
+                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev)
 {
     if (n <= 0)
         throw TcError(-3, "tc_build_tree: no sources");
@@ -1379,7 +1389,7 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
 
     double *box = t.box_d.as<double>();
     uint32_t *keys = t.keys.as<uint32_t>(), *perm = t.perm.as<uint32_t>();
-    tc_sort_points(x, y, v, n, box, keys, perm, t.src.as<double4>(), ws, st, false);
+    tc_sort_points(x, y, v, n, box, keys, perm, t.src.as<double4>(), ws, st, false, ev);
 
     // census of internal cells
     const int nblk = tiles_of(n, 4 * TB_CHUNK);
@@ -1433,6 +1443,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
                                                       info, T, t.selfid.as<int>(), child, se, lev, cell);
     }
     TC_HIP(hipGetLastError());
+    if (ev)
+        TC_HIP(hipEventRecord(ev[2], st));
 
     const int *sid = t.selfid.as<int>();
     switch (order) {
@@ -1441,6 +1453,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
         TC_ORDERS(TC_CASE)
 #undef TC_CASE
     }
+    if (ev)
+        TC_HIP(hipEventRecord(ev[3], st));
 }
 
 const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m) { return tc_bbox_bad_flag(ws, m); }

@@ -47,9 +47,17 @@ static_assert(sizeof(TcNode) == 64, "TcNode must stay 64 B (one s_load_dwordx16)
 // M2L_WAVE: FP32 potential only, multipole-to-local translations about a wave's
 // target centre; E2P_WAVE: the (target, node) interactions they evaluate (a
 // part of E2P, which counts every accepted (target, node) pair in any mode).
+// CYC_*: wavefront clock cycles (shader clock, summed over waves) spent in
+// each category of a traversal -- direct sums, per-target multipole
+// evaluations, local-expansion translations (E2L / wave-level M2L), local
+// evaluation (L2P) -- and in the whole traversal; the per-category shares of
+// the reference's TimeDistrib (treecode-force.cpp:26-48).  Counted with the
+// interaction counters.
 enum TcCounter {
     TC_CNT_P2P = 0, TC_CNT_E2P = 1, TC_CNT_E2L = 2, TC_CNT_L2P = 3, TC_CNT_P2P_SLOTS = 4, TC_CNT_E2P_SLOTS = 5,
-    TC_CNT_M2L_WAVE = 6, TC_CNT_E2P_WAVE = 7, TC_NCOUNTERS = 8
+    TC_CNT_M2L_WAVE = 6, TC_CNT_E2P_WAVE = 7, TC_NPUBLIC_COUNTERS = 8,
+    TC_CNT_CYC_P2P = 8, TC_CNT_CYC_E2P = 9, TC_CNT_CYC_LOCAL = 10, TC_CNT_CYC_L2P = 11, TC_CNT_CYC_TOTAL = 12,
+    TC_NCOUNTERS = 13
 };
 
 struct TcError : std::runtime_error {
@@ -73,6 +81,9 @@ struct TcError : std::runtime_error {
 #endif
 
 TC_DEV int tc_lane() { return __lane_id(); }
+
+// Shader clock for the per-category cycle counters (one scalar instruction).
+TC_DEV unsigned long long tc_clock() { return __builtin_readcyclecounter(); }
 
 // Orders LDS traffic between lanes of one wavefront (no instruction, only a
 // code-motion and memory-model fence at wavefront scope).
@@ -299,15 +310,19 @@ void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &
 // Bounding box, keys (Morton, or Hilbert for targets) and stable sort of n
 // points; with src_out also the sorted records {x, y, v, 0}.  One workgroup
 // for n <= 16384, else the multi-kernel path.
+// ev (optional, two events): recorded after the keys and after the sorted records.
 void tc_sort_points(const double *x, const double *y, const double *v, int n, double *box_d, uint32_t *keys_out,
-                    uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert);
+                    uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert,
+                    const hipEvent_t *ev = nullptr);
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
-                    bool hilbert = false);
+                    bool hilbert = false, const hipEvent_t *ev = nullptr);
 // fp32: also build the FP32 mirrors; deriv32: the coefficient mirror holds
 // k b_k (force trees) instead of b_k (potential trees).
+// ev (optional, four events): recorded after the Morton keys, the sorted
+// records, the topology and the upward pass (the build's phase boundaries).
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
-                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st);
+                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev = nullptr);
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);
 // Device flag (uint32, nonzero = a non-finite coordinate) set by the last
 // tc_sort_targets call on m targets; valid until the next bounding-box pass.

@@ -327,6 +327,11 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     float acc_e2p = 0.f;
     int n_e2p = 0;
     unsigned long long np2p = 0, ne2p = 0, nslots = 0, neslots = 0;
+    // CNT: wave clock cycles per category (direct sums, per-target multipoles,
+    // wave-level translations, local evaluation) and in total
+    unsigned long long cyc0 = 0, cp2p = 0, ce2p = 0, cloc = 0, cl2p = 0, tc0 = 0;
+    if constexpr (CNT)
+        cyc0 = tc_clock();
     int sp = 0;
     s_node[wid][0] = 0;
     s_mask[wid][0] = live;
@@ -346,6 +351,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         const double r2 = dx * dx + dy * dy;
         const bool acc_ok = in & (nd.r * nd.r < th2 * r2);  // no short circuit: r arrives with the rest
         const unsigned long long amask = tc_ballot(acc_ok);
+        if constexpr (CNT)
+            tc0 = tc_clock();
         if (acc_ok) {
             if (F32)
                 acc_e2p += e2p_pot32<P>((float)dx, (float)dy, exps32 + (size_t)(P + 1) * id);
@@ -360,11 +367,15 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         if constexpr (CNT) {
             ne2p += __popcll(amask);
             neslots += amask ? 64ull : 0ull;
+            if (amask)
+                ce2p += tc_clock() - tc0;
         }
         const unsigned long long omask = nonempty ? mask & ~amask : 0ull;
         if (!omask)
             continue;
         if (nd.child == 0) {
+            if constexpr (CNT)
+                tc0 = tc_clock();
             const bool act = (omask >> lane) & 1ull;
             const int nact = __popcll(omask), nsrc = nd.e - nd.s, nchunk = (nsrc + 63) >> 6;
             if constexpr (CNT)
@@ -503,6 +514,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 }
                 acc += (leafsum + leafsum2) * 0.5;
             }
+            if constexpr (CNT)
+                cp2p += tc_clock() - tc0;
         } else {
             if constexpr (F32) {
                 if (omask == live) {
@@ -585,6 +598,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                     st[j] = g < nqq ? exps32[(size_t)(P + 1) * s_q[wid][q0 + g] + kk] : make_float2(0.f, 0.f);
                 }
             };
+            if constexpr (CNT)
+                tc0 = tc_clock();
             fetch(0);
             for (int q0 = 0; q0 < nq; q0 += 4) {
 #pragma unroll
@@ -596,6 +611,11 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                     fetch(q0 + 4);
                 m2l_wave_pass<P>(min(4, nq - q0), s_qz[wid] + q0, s_coef, (float)rwave, qs, s_beta, lre, lim);
                 tc_wave_sync();
+            }
+            if constexpr (CNT) {
+                const unsigned long long t1 = tc_clock();
+                cloc += t1 - tc0;
+                tc0 = t1;
             }
             // sum the four node groups' partial coefficients, then every lane
             // evaluates sum_l L_l zeta^l at its own target (Horner, FP64)
@@ -614,6 +634,8 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                 hr = t;
             }
             acc += hr;
+            if constexpr (CNT)
+                cl2p += tc_clock() - tc0;
         }
     }
     if (valid)
@@ -627,6 +649,11 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
             atomicAdd(&counters[TC_CNT_M2L_WAVE], nm2l);
             atomicAdd(&counters[TC_CNT_E2P_WAVE], ne2pw);
         }
+        atomicAdd(&counters[TC_CNT_CYC_P2P], cp2p);
+        atomicAdd(&counters[TC_CNT_CYC_E2P], ce2p);
+        atomicAdd(&counters[TC_CNT_CYC_LOCAL], cloc);
+        atomicAdd(&counters[TC_CNT_CYC_L2P], cl2p);
+        atomicAdd(&counters[TC_CNT_CYC_TOTAL], tc_clock() - cyc0);
     }
 }
 
@@ -1136,7 +1163,8 @@ TC_DEV void force_skip_p2p(const int *list, int n, int &skip, const TcNode *__re
 // outputs and hands further P2P leaves to k_force_chunks (see TcNearLists).
 // With NEAR = true the waves take the bricks whose leaves did not fit there,
 // repeat the same traversal and add all their P2P sums.
-template <int P, bool F32, bool NEAR>
+// CNT: the instance launched with counters, which also times the categories
+template <int P, bool F32, bool NEAR, bool CNT>
 __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
@@ -1167,6 +1195,10 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
     int *stack = s_stack[wid];
     int *lst_l = s_list[wid][0], *lst_e = s_list[wid][1], *lst_p = s_list[wid][2];
     unsigned long long np2p = 0, ne2p = 0, ne2l = 0, nl2p = 0;
+    // with counters: wave clock cycles per category and in total
+    constexpr bool timing = CNT;
+    unsigned long long cp2p = 0, ce2p = 0, cloc = 0, cl2p = 0, tc0 = 0;
+    const unsigned long long cyc0 = timing ? tc_clock() : 0ull;
     for (uint32_t iter = 0;; ++iter) {
         uint32_t grab = 0;
         if (NEAR)
@@ -1250,36 +1282,59 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
             // a list that might not take another 64 ids is flushed now
             if (nl > FQ_CAP - 64) {
                 ne2l += nl;
+                if (timing)
+                    tc0 = tc_clock();
                 force_flush_e2l<P, F32>(lst_l, nl, nodes, exps, s_buf[wid], bk.x0b, bk.y0b, s_binom, s_ninv,
                                         s_coef[wid], lre, lim);
+                if (timing)
+                    cloc += tc_clock() - tc0;
                 nl = 0;
             }
             if (ne > FQ_CAP - 64) {
                 ne2p += 64ull * ne;
+                if (timing)
+                    tc0 = tc_clock();
                 force_flush_e2p<P, F32>(lst_e, ne, nodes, exps, exps32, s_buf[wid], xb, yb, ix, iy, h, fx, fy);
+                if (timing)
+                    ce2p += tc_clock() - tc0;
                 ne = 0;
             }
             if (np > FQ_CAP - 64) {
+                if (timing)
+                    tc0 = tc_clock();
                 if constexpr (NEAR)
                     force_skip_p2p<F32>(lst_p, np, skip, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
                 else
                     force_near_p2p<F32>(lst_p, np, nlist, (int)wave, ninl, prev, ovfl, nodes, src, src32,
                                         s_buf[wid], xt, yt, fx, fy, np2p);
+                if (timing)
+                    cp2p += tc_clock() - tc0;
                 np = 0;
             }
         }
         if constexpr (NEAR) {
+            if (timing)
+                tc0 = tc_clock();
             if (np)
                 force_skip_p2p<F32>(lst_p, np, skip, nodes, src, src32, s_buf[wid], xt, yt, fx, fy, np2p);
+            if (timing)
+                cp2p += tc_clock() - tc0;
             const size_t o = (size_t)b * 1024 + (size_t)(bx + ix) + 32 * (size_t)(by + iy);
             xres[o] += fx;
             yres[o] += fy;
             tc_wave_sync();
             continue;
         }
+        if (timing)
+            tc0 = tc_clock();
         if (np)
             force_near_p2p<F32>(lst_p, np, nlist, (int)wave, ninl, prev, ovfl, nodes, src, src32, s_buf[wid], xt,
                                 yt, fx, fy, np2p);
+        if (timing) {
+            const unsigned long long t1 = tc_clock();
+            cp2p += t1 - tc0;
+            tc0 = t1;
+        }
         if (lane == 0) {
             nlist.pmark[wave] = ovfl ? 1u : 0u;
             if (ovfl)
@@ -1287,8 +1342,14 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
         }
         if (ne) {
             ne2p += 64ull * ne;
+            if (timing)
+                tc0 = tc_clock();
             force_flush_e2p<P, F32>(lst_e, ne, nodes, exps, exps32, s_buf[wid], xb, yb, ix, iy, h, fx, fy);
+            if (timing)
+                ce2p += tc_clock() - tc0;
         }
+        if (timing)
+            tc0 = tc_clock();
         if (nl) {
             ne2l += nl;
             force_flush_e2l<P, F32>(lst_l, nl, nodes, exps, s_buf[wid], bk.x0b, bk.y0b, s_binom, s_ninv,
@@ -1299,6 +1360,11 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
         for (int off = 32; off >= E2L_G<P>; off >>= 1) {
             lre += __shfl_xor(lre, off, 64);
             lim += __shfl_xor(lim, off, 64);
+        }
+        if (timing) {
+            const unsigned long long t1 = tc_clock();
+            cloc += t1 - tc0;
+            tc0 = t1;
         }
         // downward_l2p_8x8 (force-kernels.m4:157-191) about the brick centre
         if (lane < P)
@@ -1327,6 +1393,8 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
         xres[o] = fx;
         yres[o] = fy;
         nl2p += 64;
+        if (timing)
+            cl2p += tc_clock() - tc0;
         tc_wave_sync();
     }
     if (counters && lane == 0) {
@@ -1336,6 +1404,11 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
         atomicAdd(&counters[TC_CNT_L2P], nl2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
         atomicAdd(&counters[TC_CNT_E2P_SLOTS], ne2p);
+        atomicAdd(&counters[TC_CNT_CYC_P2P], cp2p);
+        atomicAdd(&counters[TC_CNT_CYC_E2P], ce2p);
+        atomicAdd(&counters[TC_CNT_CYC_LOCAL], cloc);
+        atomicAdd(&counters[TC_CNT_CYC_L2P], cl2p);
+        atomicAdd(&counters[TC_CNT_CYC_TOTAL], tc_clock() - cyc0);
     }
 }
 
@@ -1353,10 +1426,11 @@ __global__ __launch_bounds__(256) void k_force_chunks(const TcNode *__restrict__
     const int wid = threadIdx.x >> 6, lane = tc_lane();
     const int nch = min((int)nlist.ctr[FN_CHUNKS], nlist.maxc);
     unsigned long long np2p = 0;
+    const unsigned long long cyc0 = counters ? tc_clock() : 0ull;
     for (int c = blockIdx.x * 4 + wid; c < nch; c += gridDim.x * 4) {
         const int4 d = nlist.cdesc[c];
         if (nlist.pmark[__builtin_amdgcn_readfirstlane(d.x)])
-            continue;  // summed by k_force<P, F32, true>
+            continue;  // summed by k_force<P, F32, true, *>
         const int gbrick = first + __builtin_amdgcn_readfirstlane(d.x);
         const int b = gbrick >> 4, brick = gbrick & 15;
         const int by = (brick >> 2) * TILE, bx = (brick & 3) * TILE;
@@ -1371,8 +1445,12 @@ __global__ __launch_bounds__(256) void k_force_chunks(const TcNode *__restrict__
         nlist.cpart[(size_t)c * 64 + lane] = make_double2(fx, fy);
     }
     if (counters && lane == 0) {
+        // a chunk wave does direct sums only
+        const unsigned long long cyc = tc_clock() - cyc0;
         atomicAdd(&counters[TC_CNT_P2P], np2p);
         atomicAdd(&counters[TC_CNT_P2P_SLOTS], np2p);
+        atomicAdd(&counters[TC_CNT_CYC_P2P], cyc);
+        atomicAdd(&counters[TC_CNT_CYC_TOTAL], cyc);
     }
 }
 
@@ -1655,14 +1733,24 @@ void launch_force_pass(const TcTree &t, double theta, const double *x0s, const d
     const float4 *src32 = F32 ? t.src32.as<float4>() : nullptr;
     tc_fill_u32(work, 0u, 1, st);
     tc_fill_u32(nl.ctr, 0u, FN_NCTR, st);
-    const int grid = std::min(tiles_of(nb, 4), resident_blocks<k_force<P, F32, false>>(device));
-    k_force<P, F32, false><<<grid, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb, theta,
-                                                 xr, yr, counters, work, nl);
+    if (counters) {
+        const int grid = std::min(tiles_of(nb, 4), resident_blocks<k_force<P, F32, false, true>>(device));
+        k_force<P, F32, false, true><<<grid, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb,
+                                                           theta, xr, yr, counters, work, nl);
+    } else {
+        const int grid = std::min(tiles_of(nb, 4), resident_blocks<k_force<P, F32, false, false>>(device));
+        k_force<P, F32, false, false><<<grid, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first,
+                                                            nb, theta, xr, yr, counters, work, nl);
+    }
     // the chunk and overflow counts are on the device: persistent grids
     k_force_chunks<F32><<<512, 256, 0, st>>>(nodes, src, src32, x0s, y0s, hs, first, nl, counters);
     k_force_combine<<<64, 256, 0, st>>>(x0s, y0s, hs, first, nl, xr, yr);
-    k_force<P, F32, true><<<256, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb,
-                                                    theta, xr, yr, counters, work, nl);
+    if (counters)
+        k_force<P, F32, true, true><<<256, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb,
+                                                         theta, xr, yr, counters, work, nl);
+    else
+        k_force<P, F32, true, false><<<256, 256, 0, st>>>(nodes, exps, exps32, src, src32, x0s, y0s, hs, first, nb,
+                                                          theta, xr, yr, counters, work, nl);
     TC_HIP(hipGetLastError());
 }
 

@@ -156,9 +156,12 @@ class GpuCase:
         return best
 
     def counters(self):
+        """A solve with interaction counters (and per-category cycles) on: the counts, and the
+        build's phases and the traversal's cycle shares (tc_get_profile)."""
         self.ctx.set(pkg.OPT_COUNTERS, 1)
         self.step()
         st = self.ctx.stats()
+        st["profile"] = self.ctx.profile()
         self.ctx.set(pkg.OPT_COUNTERS, 0)
         return st
 
@@ -225,8 +228,8 @@ def cpu_baseline(kind, data, theta, order, leaf, budget_s):
                 break
             k = int(min(m, k * max(2.0, 0.5 * budget_s / max(te, 1e-3))))
         per_target = te / len(idx)
-        sample = "oracle tree build on all %d sources (%.1f s, 1 thread) + %d random targets (%.1f s, %d threads)" % (
-            len(data["xs"]), t_build, len(idx), te, threads)
+        sample = "oracle tree build on all %d sources (%.1f s, %d threads) + %d random targets (%.1f s, %d threads)" \
+            % (len(data["xs"]), t_build, threads, len(idx), te, threads)
     else:
         nb = len(data["x0s"])
         k = 16
@@ -240,8 +243,8 @@ def cpu_baseline(kind, data, theta, order, leaf, budget_s):
             k = int(min(nb, k * max(2.0, 0.5 * budget_s / max(te, 1e-3))))
         m = nb * 1024
         per_target = te / (len(bidx) * 1024)
-        sample = "oracle tree build on all %d sources (%.1f s, 1 thread) + %d random blocks (%.1f s, %d threads)" % (
-            len(data["xs"]), t_build, len(bidx), te, threads)
+        sample = "oracle tree build on all %d sources (%.1f s, %d threads) + %d random blocks (%.1f s, %d threads)" \
+            % (len(data["xs"]), t_build, threads, len(bidx), te, threads)
     total = t_build + per_target * m
     return {"value": m / total / 1e6, "unit": "Mtargets/s", "cores": threads, "kind": "port",
             "sample": sample + "; extrapolated to %d targets" % m}
@@ -322,7 +325,9 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
     last = {}  # (kind, source seed) -> (order, leaf) of its latest request
     for r in reqs:
         xs, ys, vs = workloads.pool_sources(pools, r)
-        tg = workloads.request_targets(r, xs, ys)
+        # potential targets are the sources' own coordinates: passed as the same
+        # (read-only) host arrays rather than 10,000 copies held at once
+        tg = (xs, ys) if r["kind"] == "potential" else workloads.request_targets(r, xs, ys)
         ol = None
         if tuner:
             m = len(tg[0]) * (1 if r["kind"] == "potential" else 1024)
@@ -406,6 +411,130 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
                       "the reference's order / leaf (12 / 512 potential, 24 / 192 force)"}
 
 
+def executed_flops_potential(cnt, p):
+    # what the kernel actually evaluates: per-lane E2P for the interactions not taken by a
+    # wavefront's local expansion, 16 local coefficients per wave-level translation (the
+    # Horner over p terms, 8 flops each, plus about 24 for the powers and the log term)
+    # and 16 complex multiply-adds per target for the local evaluation
+    e2p_lane = cnt["e2p"] - cnt.get("e2p_wave", 0)
+    m2l = cnt.get("m2l_wave", 0)
+    l2p = 128.0 * (cnt.get("targets", 0) if m2l else 0)
+    return 9.0 * cnt["p2p"] + (10.0 * p + 6.0) * e2p_lane + 16.0 * (8.0 * p + 24.0) * m2l + l2p
+
+
+def measure(cfg, theta, args, cpu_budget, host=False, n=0):
+    """Time one workload in both precisions (each at its tuned order / leaf unless given),
+    check both against the FP64 direct sum, and build the result record."""
+    log("generating workload", cfg, "theta", theta)
+    kind, data, desc = make_inputs(cfg, n)
+    cases, timings, tuned = {}, {}, {}
+    for prec in (["f32", "f64"] if args.only is None else [args.only]):
+        order, leaf = args.order, args.leaf
+        from treecodes2d_amd import autotune
+        if not (args.tune or args.reference_params or order or leaf):
+            m_t = len(data["xt"]) if kind == "potential" else len(data["x0s"]) * 1024
+            hit = autotune.Tuner(pkg).lookup(kind, 1 if prec == "f32" else 0, theta, len(data["xs"]), m_t)
+            if hit:
+                order, leaf = hit
+                tuned[prec] = {"order": order, "leaf": leaf, "source": "2d-treecodes_amd/tuned.json"}
+                log("  tuned parameters for %s: order %d leaf %d" % (prec, order, leaf))
+        if args.tune:
+            log("autotuning", prec)
+            grid = dict(orders=(8, 9, 10, 12, 14, 16, 20, 24), leaves=(64, 128, 192, 256, 384, 512)) \
+                if kind == "potential" else dict(orders=(8, 10, 12, 16), leaves=(32, 64, 128, 192, 384))
+            tuner = autotune.Tuner(pkg, cache_path=args.tune_cache or autotune.CACHE)
+            res = tuner.tune(kind, data, theta=theta, precision=1 if prec == "f32" else 0,
+                             err_target=args.err_target, reps=2, **grid)
+            order, leaf = res["order"], res["leaf"]
+            tuned[prec] = {"order": order, "leaf": leaf, "err_sampled": res["err"],
+                           "table": [(t["order"], t["leaf"], round(t["ms"], 3), t["err"]) for t in res["table"]]}
+            log("  chose order %d leaf %d (%.3f ms, err %.2e)" % (order, leaf, res["ms"], res["err"]))
+        log("timing", prec)
+        c = GpuCase(kind, data, 1 if prec == "f32" else 0, theta, order, leaf)
+        dt, phases = c.timed(args.steps, args.warmup)
+        cases[prec] = c
+        timings[prec] = (dt, phases)
+        log("  %s: %.3f ms/step, phases %s" % (prec, dt * 1e3, {k: round(v, 3) for k, v in phases.items()}))
+
+    head = "f32" if "f32" in cases else "f64"
+    p = cases[head].order
+    dt, phases = timings[head]
+    m = cases[head].m
+    st = cases[head].counters()
+    cnt = dict(st["counters"])
+    fl = flops_potential(cnt, p) if kind == "potential" else flops_force(cnt, p)
+    eval_ms = phases["eval"]
+    achieved = fl / (eval_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[head]
+    roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "traffic_source": None,
+            "kernel": "k_potential (traversal)" if kind == "potential" else "k_force + k_force_chunks (traversal)",
+            "flops_model": "per interaction: P2P %d, E2P %s" % ((9, "10p+6") if kind == "potential" else
+                                                                (11, "16p+11, E2L 4p^2+21p-1, L2P 16p-12")),
+            "note": "VALU-bound; peak is the vector %s rate (no MFMA in this path)" % head.upper()}
+    if kind == "potential" and cnt.get("m2l_wave"):
+        cnt_t = dict(cnt, targets=m)
+        ex = executed_flops_potential(cnt_t, p) / (eval_ms * 1e-3) / 1e12
+        roof["executed_tflops"] = ex
+        roof["note"] += ("; %.0f%% of the (target, node) multipole interactions are evaluated through "
+                         "wavefront-level local expansions, so the algorithmic rate (achieved) exceeds the "
+                         "executed one (executed_tflops)" % (100.0 * cnt["e2p_wave"] / max(cnt["e2p"], 1)))
+    tfile = os.path.join(ROOT, "profiles", "potential_2e24_traffic.json")
+    if kind == "potential" and head == "f32" and os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        if tj["workload"] == desc and tj["order"] == p and tj["theta"] == theta and tj["leaf"] == cases[head].leaf:
+            roof["traffic"] = tj["hbm_bytes_per_launch"]
+            roof["traffic_source"] = ("copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this "
+                                      "workload, from profiles/potential_2e24_hbm_traffic.md (not measured in "
+                                      "this run)")
+    errs = None
+    if not args.no_error and "f32" in cases and "f64" in cases:
+        log("direct-sum error check")
+        errs = sampled_error(cases["f32"], cases["f64"], kind, data)
+    hostr = None
+    if host:
+        log("host-array C ABI timing")
+        hdt = cases[head].host_timed(data)
+        hostr = {"ms": hdt * 1e3, "mtargets_per_s": m / hdt / 1e6, "dtype": head,
+                 "path": "end-to-end C ABI with host arrays in and out (pageable memory): H2D copies, build, "
+                         "solve, D2H copy; PCIe-inclusive (value is the HBM-resident rate)"}
+        log("  %.3f ms -> %.1f Mtargets/s" % (hdt * 1e3, hostr["mtargets_per_s"]))
+    for c in cases.values():
+        c.ctx.close()
+    cpu = None
+    if cpu_budget > 0:
+        log("cpu baseline")
+        cpu = cpu_baseline(kind, data, theta, args.order, args.leaf, cpu_budget)
+    res = {"value": m / dt / 1e6, "ms_per_step": dt * 1e3, "dtype": head,
+           "config": {"workload": desc, "theta": theta, "order": p, "leaf": cases[head].leaf,
+                      "precision": "FP32 P2P/E2P on leaf-relative offsets, FP64 decisions/P2E/E2E/E2L"
+                      if head == "f32" else "plain FP64", "inputs": "resident in HBM"},
+           "phases_ms": {k: round(v, 4) for k, v in phases.items()},
+           "counters": {k: int(v) for k, v in cnt.items()},
+           "profile": {"build_ms": {k: round(v, 4) for k, v in st["profile"]["build_ms"].items()},
+                       "traversal_cycle_shares": {k: round(v, 4) for k, v in
+                                                  st["profile"]["cycle_shares"].items()},
+                       "note": "from the solve with counters on (build phases on the context stream, the "
+                               "target sort beside them; cycle shares of the traversal's wavefront time)"},
+           "roofline": roof, "cpu_baseline": cpu, "host_abi": hostr}
+    if "f32" in timings and "f64" in timings:
+        res["fp64_value"] = m / timings["f64"][0] / 1e6
+        res["fp64_ms_per_step"] = timings["f64"][0] * 1e3
+        res["speedup_fp32_vs_fp64"] = timings["f64"][0] / timings["f32"][0]
+        res["fp64_phases_ms"] = {k: round(v, 4) for k, v in timings["f64"][1].items()}
+    if errs:
+        res["rel_err_vs_direct"] = errs
+    if tuned:
+        res["autotune"] = tuned
+    if "f64" in cases:
+        res["fp64_config"] = {"order": cases["f64"].order, "leaf": cases["f64"].leaf}
+    return res
+
+
+# the other benchmark workloads, measured after the headline one (theta, CPU-baseline seconds)
+LEGS = [("c3", 0.3, 0.0), ("c3", 0.7, 0.0), ("c5", 0.5, 8.0), ("c2", 0.5, 0.0), ("c1", 0.5, 4.0)]
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,23 +549,26 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-array (PCIe-inclusive) timing")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="only the selected workload (default: also C1, C2, C5 and C3 at theta 0.3 / 0.7)")
     ap.add_argument("--only", choices=["f32", "f64"], default=None, help="time one precision only (profiling)")
-    ap.add_argument("--serve-requests", type=int, default=2000,
-                    help="requests of the served stream replayed after the solve timing (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests of the replayed served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate of the stream (1/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--tune", action="store_true",
-                    help="re-run the autotuner for order and leaf capacity per precision (rel err <= 5e-7)")
+                    help="re-run the autotuner for this workload (grid of order x leaf) before timing")
+    ap.add_argument("--err-target", type=float, default=5e-7, help="autotuner error target (default 5e-7)")
     ap.add_argument("--tune-cache", default="", metavar="PATH",
-                    help="file the autotuner records its choices in (default: the package's tuned.json)")
+                    help="autotuner cache to write (default 2d-treecodes_amd/tuned.json)")
     ap.add_argument("--tune-serving", action="store_true",
-                    help="autotune the served stream's size buckets (FP32), record them and exit")
+                    help="autotune the served stream's size buckets and exit")
     ap.add_argument("--serve-reference-params", action="store_true",
-                    help="serve every request with the reference's order / leaf instead of the tuned buckets")
+                    help="serve every request at the reference's order / leaf instead of the tuned buckets")
     ap.add_argument("--reference-params", action="store_true",
-                    help="use the reference's order/leaf (12/512 potential, 24/192 force) instead of tuned ones")
+                    help="time both precisions at the reference's order / leaf (no autotuned parameters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="also run a plain copy kernel of this size (HBM counter calibration)")
+                    help="also run a plain copy kernel of this size (MiB), the reference for HBM counters")
     args = ap.parse_args()
     if args.theta is None:
         args.theta = 0.0 if args.config == "near" else 0.5
@@ -461,119 +593,42 @@ def main():
         b.free()
         cctx.close()
 
-    log("generating workload", args.config)
-    kind, data, desc = make_inputs(args.config, args.n)
-    p = args.order or (12 if kind == "potential" else 24)
-    cases = {}
-    timings = {}
-    tuned = {}
-    for prec in (["f32", "f64"] if args.only is None else [args.only]):
-        order, leaf = args.order, args.leaf
-        from treecodes2d_amd import autotune
-        if not (args.tune or args.reference_params or order or leaf):
-            m_t = len(data["xt"]) if kind == "potential" else len(data["x0s"]) * 1024
-            hit = autotune.Tuner(pkg).lookup(kind, 1 if prec == "f32" else 0, args.theta, len(data["xs"]), m_t)
-            if hit:
-                order, leaf = hit
-                tuned[prec] = {"order": order, "leaf": leaf, "source": "2d-treecodes_amd/tuned.json"}
-                log("  tuned parameters for %s: order %d leaf %d" % (prec, order, leaf))
-        if args.tune:
-            from treecodes2d_amd import autotune
-            log("autotuning", prec)
-            grid = dict(orders=(8, 10, 12, 14), leaves=(64, 128, 192, 256, 384, 512)) if kind == "potential" else \
-                dict(orders=(8, 10, 12, 16), leaves=(32, 64, 128, 192, 384))
-            tuner = autotune.Tuner(pkg, cache_path=args.tune_cache or autotune.CACHE)
-            res = tuner.tune(kind, data, theta=args.theta, precision=1 if prec == "f32" else 0,
-                             err_target=5e-7, reps=2, **grid)
-            order, leaf = res["order"], res["leaf"]
-            tuned[prec] = {"order": order, "leaf": leaf, "err_sampled": res["err"],
-                           "table": [(t["order"], t["leaf"], round(t["ms"], 3), t["err"]) for t in res["table"]]}
-            log("  chose order %d leaf %d (%.3f ms, err %.2e)" % (order, leaf, res["ms"], res["err"]))
-        log("timing", prec)
-        c = GpuCase(kind, data, 1 if prec == "f32" else 0, args.theta, order, leaf)
-        dt, phases = c.timed(args.steps, args.warmup)
-        cases[prec] = c
-        timings[prec] = (dt, phases)
-        log("  %s: %.3f ms/step, phases %s" % (prec, dt * 1e3, {k: round(v, 3) for k, v in phases.items()}))
-
-    head = "f32" if "f32" in cases else "f64"
-    p = cases[head].order
-    dt, phases = timings[head]
-    m = cases[head].m
-    st = cases[head].counters()
-    cnt = st["counters"]
-    fl = flops_potential(cnt, p) if kind == "potential" else flops_force(cnt, p)
-    eval_ms = phases["eval"]
-    achieved = fl / (eval_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[head]
-    traffic, traffic_src = None, None
-    tfile = os.path.join(ROOT, "profiles", "potential_2e24_traffic.json")
-    if kind == "potential" and head == "f32" and os.path.exists(tfile):
-        tj = json.load(open(tfile))
-        if tj["workload"] == desc and tj["order"] == p and tj["theta"] == args.theta and \
-                tj["leaf"] == (cases[head].leaf):
-            traffic, traffic_src = tj["hbm_bytes_per_launch"], "profiles/potential_2e24_hbm_traffic.md"
-
-    errs = None
-    if not args.no_error and "f32" in cases and "f64" in cases:
-        log("direct-sum error check")
-        errs = sampled_error(cases["f32"], cases["f64"], kind, data)
-    host = None
-    if not args.no_host:
-        log("host-array C ABI timing")
-        hdt = cases[head].host_timed(data)
-        host = {"ms": hdt * 1e3, "mtargets_per_s": m / hdt / 1e6, "dtype": head,
-                "path": "context solve with host arrays (pageable): H2D copies, build, solve, D2H copy"}
-        log("  %.3f ms -> %.1f Mtargets/s" % (hdt * 1e3, host["mtargets_per_s"]))
+    head = measure(args.config, args.theta, args, 0.0 if args.no_cpu else args.cpu_budget, host=not args.no_host,
+                   n=args.n)
+    legs = {}
+    if not args.no_legs and args.only is None and not args.tune:
+        for cfg, theta, budget in LEGS:
+            if (cfg, theta) == (args.config, args.theta):
+                continue
+            key = "%s_theta%.1f" % (cfg, theta)
+            legs[key] = measure(cfg, theta, args, 0.0 if args.no_cpu else budget)
+            log("  leg %s: %.1f Mtargets/s (%.3f ms), FP32/FP64 %.2fx" % (
+                key, legs[key]["value"], legs[key]["ms_per_step"], legs[key].get("speedup_fp32_vs_fp64", 0)))
     serving = None
     if args.serve_requests > 0:
         log("served stream: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
-        for c in cases.values():
-            c.ctx.close()
         serving = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, 64.0,
                                tuned=not args.serve_reference_params)
         log("  p50 %.2f ms p99 %.2f ms" % (serving["p50_ms"], serving["p99_ms"]))
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline")
-        cpu = cpu_baseline(kind, data, args.theta, args.order, args.leaf, args.cpu_budget)
 
     res = {
         "metric": "Mtargets/s at rel err <=1e-6, FP32-bulk vs FP64 plain",
-        "value": m / dt / 1e6,
+        "value": head["value"],
         "unit": "Mtargets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": head,
+        "dtype": head["dtype"],
         "data": "synthetic",
-        "config": {"workload": desc, "theta": args.theta, "order": p, "leaf": cases[head].leaf,
-                   "precision": "FP32 P2P/E2P on leaf-relative offsets, FP64 decisions/P2E/E2E/E2L"
-                   if head == "f32" else "plain FP64", "inputs": "resident in HBM"},
-        "phases_ms": {k: round(v, 4) for k, v in phases.items()},
-        "counters": {k: int(v) for k, v in cnt.items()},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_potential (traversal)" if kind == "potential" else "k_force (traversal)",
-                     "note": "VALU-bound: peak is the vector %s rate (no MFMA in this path)" % head.upper()},
-        "cpu_baseline": cpu,
-        "serving": serving,
-        "host_abi": host,
     }
-    if "f32" in timings and "f64" in timings:
-        res["fp64_value"] = m / timings["f64"][0] / 1e6
-        res["speedup_fp32_vs_fp64"] = timings["f64"][0] / timings["f32"][0]
-        res["fp64_phases_ms"] = {k: round(v, 4) for k, v in timings["f64"][1].items()}
-    if errs:
-        res["rel_err_vs_direct"] = errs
-    if tuned:
-        res["autotune"] = tuned
-        if "f64" in cases:
-            res["fp64_config"] = {"order": cases["f64"].order, "leaf": cases["f64"].leaf}
+    res.update({k: v for k, v in head.items() if k not in ("value", "ms_per_step", "dtype")})
+    res["serving"] = serving
+    if legs:
+        res["legs"] = legs
     print(json.dumps(res), flush=True)
 
 

@@ -160,6 +160,16 @@ TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x
  * evaluate}, and tree_info = {n, nnodes, ninternal, order}. */
 TC_API int tc_get_stats(tc_context *ctx, double *times_ms, unsigned long long *counters, int *tree_info);
 
+/* Finer profile of the last call, the counterpart of the reference's per-thread TimeDistrib
+ * (treecode-force.cpp:26-48).  build_ms[4] (requires TC_OPT_PROFILE and a call that built a
+ * tree): bounding box + Morton keys, radix sort + sorted records, topology (census, its
+ * readback, nodes), upward pass (P2E, E2E, FP32 mirrors); zeros otherwise.  cycles[5]
+ * (requires TC_OPT_COUNTERS): wavefront clock cycles of the traversal summed over waves, spent
+ * in {direct sums, per-target multipole evaluations, local-expansion translations (force E2L,
+ * FP32 potential wave-level M2L), local evaluation (L2P), the whole traversal}; the rest of the
+ * total is the walk itself (decisions, stack, list handling).  Either pointer may be NULL. */
+TC_API int tc_get_profile(tc_context *ctx, double *build_ms, unsigned long long *cycles);
+
 /* Copy the resident tree to host memory (any pointer may be NULL):
  * keys[n], perm[n] (sorted -> input index), box[3] (xmin, ymin, ext),
  * geom[4*nn] (xcom, ycom, r, mass), topo[4*nn] (s, e, child, level),

@@ -80,6 +80,11 @@ def max_threads():
     return lib().orc_max_threads()
 
 
+def set_threads(n):
+    """Threads of the following OpenMP regions of the oracle (tree builds included)."""
+    lib().orc_set_threads(int(n))
+
+
 def bbox(x, y):
     x, y = _f64(x), _f64(y)
     out = np.zeros(3)

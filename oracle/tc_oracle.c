@@ -74,6 +74,9 @@ static void set_threads(int nthreads)
 #endif
 }
 
+/* Threads of the following parallel regions (tree builds included). */
+void orc_set_threads(int nthreads) { set_threads(nthreads); }
+
 /* std::max / std::min semantics: max(a,b) = a<b ? b : a, min(a,b) = b<a ? b : a */
 static inline double smax(double a, double b) { return a < b ? b : a; }
 static inline double smin(double a, double b) { return b < a ? b : a; }
@@ -124,6 +127,7 @@ static inline int quantise(double v, double vmin, double scale)
 void orc_keys(const double *x, const double *y, int n, double xmin, double ymin, double ext, int *keys)
 {
     const double scale = (1.0 / ext) * (double)(1 << LMAX);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         const int qx = quantise(x[i], xmin, scale);
         const int qy = quantise(y[i], ymin, scale);
@@ -132,26 +136,52 @@ void orc_keys(const double *x, const double *y, int n, double xmin, double ymin,
 }
 
 /* Stable LSD counting sort on 30-bit keys: the unique order of the
- * lexicographic pair sort at sort-sources.cpp:112 (ties by original index). */
+ * lexicographic pair sort at sort-sources.cpp:112 (ties by original index).
+ * Each pass is parallel and stays stable: thread t counts, then scatters, its
+ * own contiguous chunk, and the output runs of digit d are laid out in thread
+ * order. */
 void orc_sort(const int *keys, int n, int *perm, int *keys_sorted)
 {
     int *pa = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
     int *pb = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    const int nt = n < 65536 ? 1 : orc_max_threads();
+    int *count = (int *)calloc((size_t)nt * 1024, sizeof(int));
+#pragma omp parallel for schedule(static) num_threads(nt)
     for (int i = 0; i < n; ++i)
         pa[i] = i;
     for (int shift = 0; shift < 30; shift += 10) {
-        int count[1025];
-        memset(count, 0, sizeof(count));
-        for (int i = 0; i < n; ++i)
-            count[((keys[pa[i]] >> shift) & 1023) + 1]++;
-        for (int d = 0; d < 1024; ++d)
-            count[d + 1] += count[d];
-        for (int i = 0; i < n; ++i)
-            pb[count[(keys[pa[i]] >> shift) & 1023]++] = pa[i];
-        int *t = pa;
+#pragma omp parallel num_threads(nt)
+        {
+#ifdef _OPENMP
+            const int t = omp_get_thread_num();
+#else
+            const int t = 0;
+#endif
+            const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+            int *c = count + (size_t)t * 1024;
+            memset(c, 0, sizeof(int) * 1024);
+            for (int i = lo; i < hi; ++i)
+                c[(keys[pa[i]] >> shift) & 1023]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                int run = 0;
+                for (int d = 0; d < 1024; ++d)
+                    for (int tt = 0; tt < nt; ++tt) {
+                        const int v = count[(size_t)tt * 1024 + d];
+                        count[(size_t)tt * 1024 + d] = run;
+                        run += v;
+                    }
+            }
+            for (int i = lo; i < hi; ++i)
+                pb[c[(keys[pa[i]] >> shift) & 1023]++] = pa[i];
+        }
+        int *tmp = pa;
         pa = pb;
-        pb = t;
+        pb = tmp;
     }
+    free(count);
+#pragma omp parallel for schedule(static) num_threads(nt)
     for (int i = 0; i < n; ++i) {
         if (perm)
             perm[i] = pa[i];
@@ -371,6 +401,7 @@ orc_tree *orc_tree_build(const double *xsrc, const double *ysrc, const double *v
     orc_keys(xsrc, ysrc, n, t->xmin, t->ymin, t->ext, raw);
     orc_sort(raw, n, t->perm, t->keys);
     free(raw);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         t->xs[i] = xsrc[t->perm[i]];
         t->ys[i] = ysrc[t->perm[i]];
@@ -381,16 +412,24 @@ orc_tree *orc_tree_build(const double *xsrc, const double *ysrc, const double *v
     tree_grow(t, 1);
     t->s[0] = 0; t->e[0] = n; t->child[0] = 0; t->level[0] = 0; t->cx[0] = 0; t->cy[0] = 0;
     int nnodes = 1, lvl_begin = 0, lvl_end = 1;
+    int lvl_start[LMAX + 2];
+    int nlevels = 0;
     for (int l = 0; lvl_begin < lvl_end; ++l) {
+        lvl_start[nlevels++] = lvl_begin;
+        /* child bases in id order (serial, one add per node), then the four
+         * children of every internal node of the level in parallel */
+        for (int id = lvl_begin; id < lvl_end; ++id) {
+            const int leaf = t->e[id] - t->s[id] <= leafcap || l + 1 > LMAX;
+            t->child[id] = leaf ? 0 : nnodes;
+            nnodes += leaf ? 0 : 4;
+        }
+        tree_grow(t, nnodes);
+#pragma omp parallel for schedule(dynamic, 64)
         for (int id = lvl_begin; id < lvl_end; ++id) {
             const int s = t->s[id], e = t->e[id];
-            const int leaf = e - s <= leafcap || l + 1 > LMAX;
-            if (leaf)
+            const int cb = t->child[id];
+            if (!cb)
                 continue;
-            tree_grow(t, nnodes + 4);
-            const int cb = nnodes;
-            nnodes += 4;
-            t->child[id] = cb;
             /* mask = Morton prefix of the cell, rebuilt from its coordinates */
             const int shift_l = 2 * (LMAX - l);
             const int mask = (shift_l >= 30) ? 0
@@ -413,10 +452,12 @@ orc_tree *orc_tree_build(const double *xsrc, const double *ysrc, const double *v
         lvl_begin = lvl_end;
         lvl_end = nnodes;
     }
+    lvl_start[nlevels] = nnodes;
     t->nnodes = nnodes;
 
-    /* upward: leaves */
+    /* upward: leaves (independent) */
     const int p = order;
+#pragma omp parallel for schedule(dynamic, 16)
     for (int id = 0; id < nnodes; ++id) {
         if (t->child[id])
             continue;
@@ -440,8 +481,11 @@ orc_tree *orc_tree_build(const double *xsrc, const double *ysrc, const double *v
         orc_p2e(t->xs + s, t->ys + s, t->vs + s, e - s, t->xcom[id], t->ycom[id], p,
                 t->exp + (size_t)2 * p * id, t->exp + (size_t)2 * p * id + p);
     }
-    /* upward: parents (upward.cpp:95-218), children ids > parent id */
-    for (int id = nnodes - 1; id >= 0; --id) {
+    /* upward: parents (upward.cpp:95-218), deepest level first; the parents of
+     * one level are independent */
+    for (int lv = nlevels - 1; lv >= 0; --lv)
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int id = lvl_start[lv]; id < lvl_start[lv + 1]; ++id) {
         if (!t->child[id])
             continue;
         const int cb = t->child[id], l = t->level[id];

@@ -115,6 +115,7 @@ void orc_l2p(double rz, double iz, const double *lre, const double *lim, int ord
 double orc_binomial(int n, int k);
 
 int orc_max_threads(void);
+void orc_set_threads(int nthreads);
 
 #ifdef __cplusplus
 }

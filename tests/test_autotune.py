@@ -34,3 +34,32 @@ def test_nearest_bucket(pkg, tmp_path):
     assert tuner.nearest("potential", 1, 0.5, 1 << 24, 1 << 24) is None
     assert tuner.nearest("potential", 1, 0.3, 1 << 17, 1 << 17) is None
     assert tuner.nearest("force", 1, 0.5, 1 << 21, 1 << 14) is None
+
+
+# ------------------------------------------------------------------ selection rule on the GPU
+import numpy as np  # noqa: E402
+import pytest  # noqa: E402
+
+
+@pytest.mark.gpu
+def test_tuner_selection_rule(pkg, wl, tmp_path):
+    # the chosen cell meets the error target and is the fastest cell that does (so it is
+    # no slower than the reference's own order / leaf when that cell qualifies); with a
+    # target no cell can meet, the reference's cell is kept and flagged
+    from treecodes2d_amd import autotune
+    tuner = autotune.Tuner(pkg, cache_path=str(tmp_path / "t.json"))
+    xs, ys, vs, xt, yt = wl.uniform_potential(1 << 14, seed=31)
+    data = dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt)
+    res = tuner.tune("potential", data, theta=0.5, precision=1, orders=(8, 12), leaves=(128, 512),
+                     err_target=1e-6, reps=2)
+    ok = [t for t in res["table"] if t["err"] <= 1e-6]
+    assert res["met_target"] and res["err"] <= 1e-6
+    assert res["ms"] == min(t["ms"] for t in ok)
+    ref = [t for t in res["table"] if (t["order"], t["leaf"]) == (12, 512)][0]
+    if ref["err"] <= 1e-6:
+        assert res["ms"] <= ref["ms"]
+    cached = tuner.lookup("potential", 1, 0.5, 1 << 14, 1 << 14)
+    assert cached == (res["order"], res["leaf"])
+    res = tuner.tune("potential", data, theta=0.5, precision=1, orders=(8, 12), leaves=(128, 512),
+                     err_target=1e-30, reps=1, save=False)
+    assert not res["met_target"] and (res["order"], res["leaf"]) == (12, 512)

@@ -564,3 +564,38 @@ def test_target_order_invariance(pkg, wl, ctx64, ctx32):
     b = ctx32.potential(0.5, xs, ys, vs, xt[perm], yt[perm])
     print("FP32 permutation difference %.3e" % rel_l2(b, a[perm]))
     assert rel_l2(b, a[perm]) < 1e-7
+
+
+def test_profile_phases_and_cycles(pkg, wl):
+    # tc_get_profile: the build's phases (ms) after a call that built a tree, and the
+    # traversal's clock cycles per category; the parts never exceed the whole
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(200000, seed=71)
+    for prec in (pkg.FP64, pkg.FP32):
+        ctx = pkg.Context(precision=prec, counters=True, profile=True, pot_order=10, pot_leaf=128)
+        ctx.potential(0.5, xs, ys, vs, xt, yt)
+        pr, st = ctx.profile(), ctx.stats()
+        b = pr["build_ms"]
+        assert all(v > 0 for v in b.values()), b
+        assert sum(b.values()) <= st["times_ms"]["build"] + 1e-3
+        c = pr["cycles"]
+        assert c["p2p"] > 0 and c["e2p"] > 0 and c["total"] > 0
+        assert c["p2p"] + c["e2p"] + c["local"] + c["l2p"] <= c["total"]
+        if prec == pkg.FP32:
+            assert c["local"] > 0 and c["l2p"] > 0  # wave-level local expansions
+        else:
+            assert c["local"] == 0 and c["l2p"] == 0
+        # a resident solve builds nothing: no build phases
+        ctx.potential_resident(0.5, xt[:5000], yt[:5000])
+        assert all(v == 0 for v in ctx.profile()["build_ms"].values())
+        ctx.close()
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(100000, nbx=8, h=1.0 / 256, seed=72)
+    ctx = pkg.Context(counters=True, profile=True, force_order=12, force_leaf=64)
+    ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    c = ctx.profile()["cycles"]
+    assert min(c["p2p"], c["e2p"], c["local"], c["l2p"]) > 0
+    assert c["p2p"] + c["e2p"] + c["local"] + c["l2p"] <= c["total"]
+    # counting off: no cycles
+    ctx.set(pkg.OPT_COUNTERS, 0)
+    ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    assert ctx.profile()["cycles"]["total"] == 0
+    ctx.close()

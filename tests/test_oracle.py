@@ -546,3 +546,19 @@ def test_force_reference_style_traversal(oracle, wl, theta):
         assert rel_l2(fx[sl], rx) <= 1e-12 and rel_l2(fy[sl], ry) <= 1e-12, b
     assert tuple(int(c) for c in cnt) == tuple(int(c) for c in total)
     assert total[2] > 0 and total[1] > 0 and total[0] > 0
+
+
+def test_parallel_build_equals_serial(oracle, wl):
+    # the tree build runs its sort, topology levels, leaves and parent levels in
+    # parallel; every step is per element or per node, so any thread count gives
+    # the serial build bit for bit
+    xs, ys, vs, _, _ = wl.mixed_plummer_potential(300000, seed=25)
+    n0 = oracle.max_threads()
+    trees = []
+    for nt in (1, 4):
+        oracle.set_threads(nt)
+        trees.append(oracle.Tree(xs, ys, vs, 48, 12))
+    oracle.set_threads(n0)
+    a, b = trees
+    for k in ("keys", "perm", "se", "child", "level", "geom", "w", "exp"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
