@@ -34,8 +34,9 @@ EXPORTED = [
     "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
     "tc_server_wait", "tc_server_stats", "treecode_potential_float", "treecode_force_mrag_float",
     "treecode_velocity_mrag", "tc_copy_kernel", "treecode_force", "tc_force_targets", "tc_potential_noalloc",
-    "tc_get_profile",
+    "tc_get_profile", "tc_server_create_ex", "tc_server_stats_ex",
 ]
+SERVER_BATCH = 1  # tc_server_create_ex flag: batch concurrent potential traversals
 NOALLOC_MAX_POINTS = 1 << 18  # sources and targets per tc_potential_noalloc call
 NOALLOC_SCRATCH_BYTES = 64 << 20  # its static device scratch
 
@@ -141,6 +142,11 @@ def lib():
         L.tc_server_wait.argtypes = [_P, ctypes.c_longlong, _D]
     if hasattr(L, 'tc_server_stats'):
         L.tc_server_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong)]
+    if hasattr(L, 'tc_server_create_ex'):
+        L.tc_server_create_ex.argtypes = [it, it, dbl, it]
+        L.tc_server_create_ex.restype = _P
+    if hasattr(L, 'tc_server_stats_ex'):
+        L.tc_server_stats_ex.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong), it]
     if hasattr(L, 'treecode_force'):
         L.treecode_force.argtypes = [dbl, _D, _D, _D, it, _D, _D, it, _D, _D]
         L.treecode_force.restype = None
@@ -456,10 +462,15 @@ class Context:
 
 
 class Server:
-    """Continuous-batching request server (tc_server): slots x (stream + thread + resident tree)."""
+    """Continuous-batching request server (tc_server): slots of (streams + host thread), one
+    source-keyed tree cache that every slot reads, HBM admission against a budget; with
+    batch=True the potential traversals of concurrent requests run as one kernel launch."""
 
-    def __init__(self, device=0, slots=4, hbm_budget_gib=64.0):
-        self.h = lib().tc_server_create(device, slots, float(hbm_budget_gib))
+    STATS = ["completed", "reused", "failed", "max_inflight", "peak_bytes", "budget_bytes", "queued", "batches",
+             "batched", "evicted", "trimmed", "cached_trees", "held_bytes", "estimate_misses", "estimate_excess"]
+
+    def __init__(self, device=0, slots=4, hbm_budget_gib=64.0, batch=False):
+        self.h = lib().tc_server_create_ex(device, slots, float(hbm_budget_gib), SERVER_BATCH if batch else 0)
         if not self.h:
             raise TreecodeError("tc_server_create failed: " + last_error())
         self._keep = {}
@@ -495,16 +506,15 @@ class Server:
         st = lib().tc_server_wait(self.h, ticket, _dp(times))
         inputs, outputs = self._keep.pop(ticket)
         if st != 0:
-            raise TreecodeError("request %d failed (status %d)" % (ticket, st))
+            raise TreecodeError("request %d failed (status %d): %s" % (ticket, st, last_error()))
         info = {"queue_ms": times[0], "service_ms": times[1], "latency_ms": times[2], "reused": bool(times[3]),
                 "slot": int(times[4])}
         return (outputs[0] if len(outputs) == 1 else outputs), info
 
     def stats(self):
-        out = (ctypes.c_longlong * 7)()
-        lib().tc_server_stats(self.h, out)
-        keys = ["completed", "reused", "failed", "max_inflight", "peak_bytes", "budget_bytes", "queued"]
-        return dict(zip(keys, list(out)))
+        out = (ctypes.c_longlong * len(self.STATS))()
+        n = lib().tc_server_stats_ex(self.h, out, len(self.STATS))
+        return dict(zip(self.STATS[:n], list(out)[:n]))
 
     def close(self):
         if getattr(self, "h", None):

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <limits>
 #include <mutex>
@@ -305,8 +306,31 @@ void record(tc_context *c, int i)
         TC_HIP(hipEventRecord(c->ev[i], c->stream));
 }
 
-void build(tc_context *c, const double *xs, const double *ys, const double *vs, int n, int kind)
+// A tree outside the context (the request server's shared cache): built into
+// by this call (build), or built earlier by another context, whose stream
+// recorded `ready` after the build.  eval, when set, replaces the one-request
+// traversal launch of a potential solve (the server's batcher).
+struct ExtTree {
+    TcTree *tree = nullptr;
+    bool build = false;
+    hipEvent_t ready = nullptr;
+    const std::function<void(const TcPotBatchItem &, hipStream_t)> *eval = nullptr;
+};
+
+void build(tc_context *c, const double *xs, const double *ys, const double *vs, int n, int kind, const ExtTree &ext)
 {
+    if (ext.tree) {
+        const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
+        const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
+        ext.tree->nnodes = 0;
+        ext.tree->has_fp32 = false;
+        tc_build_tree(*ext.tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws,
+                      c->stream);
+        if (ext.ready)
+            TC_HIP(hipEventRecord(ext.ready, c->stream));
+        return;
+    }
+
     const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
     const int leaf = kind == TC_KIND_POTENTIAL ? c->pot_leaf : c->force_leaf;
     // the resident tree is invalid from the first array tc_build_tree
@@ -342,8 +366,34 @@ void eval_potential(tc_context *c, double theta, const double *xt, const double 
                       c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets);
 }
 
+// An external tree's evaluation: the target sort (unless done beside the
+// build) and the traversal, or the batcher's launch in place of the latter.
+void eval_potential_ext(tc_context *c, const TcTree &t, const ExtTree &ext, double theta, const double *xt,
+                        const double *yt, int m, double *out, unsigned long long *cnt, bool sort_targets)
+{
+    if (!ext.eval) {
+        tc_eval_potential(t, theta, xt, yt, m, out, c->precision, cnt, c->tws, c->stream,
+                          c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets);
+        return;
+    }
+    if (sort_targets)
+        tc_sort_targets(xt, yt, m, c->tws, c->stream);
+    record(c, 2);
+    TcPotBatchItem it;
+    it.tree = &t;
+    it.theta = theta;
+    it.xt = xt;
+    it.yt = yt;
+    it.tperm = c->tws.tperm.as<uint32_t>();
+    it.m = m;
+    it.out = out;
+    (*ext.eval)(it, c->stream);
+    record(c, 3);
+}
+
 int run_potential(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
-                  const double *xt, const double *yt, int m, double *out, int device_ptrs, bool rebuild)
+                  const double *xt, const double *yt, int m, double *out, int device_ptrs, bool rebuild,
+                  const ExtTree &ext = ExtTree())
 {
     return guarded(ctx, [&] {
         tc_context *c = ctx;
@@ -358,10 +408,12 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             c->tree_kind = TC_KIND_NONE;
             return;
         }
-        if (!rebuild) {
+        if (!rebuild && !ext.tree) {
             check_resident(c, TC_KIND_POTENTIAL);
             c->bev_valid = false;
         }
+        if (!rebuild && ext.tree && ext.ready)
+            TC_HIP(hipStreamWaitEvent(c->stream, ext.ready, 0));
         const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
         double *dout = out;
         if (!device_ptrs) {
@@ -395,7 +447,7 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             }
             record(c, 0);
             try {
-                build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL);
+                build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL, ext);
             } catch (...) {
                 // later work on the context stream (the next call's copies
                 // into the io buffer) must not overtake the sort reading it
@@ -404,11 +456,17 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             }
             record(c, 1);
             TC_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
-            eval_potential(c, theta, dxt, dyt, m, dout, cnt, false);
+            if (ext.tree)
+                eval_potential_ext(c, *ext.tree, ext, theta, dxt, dyt, m, dout, cnt, false);
+            else
+                eval_potential(c, theta, dxt, dyt, m, dout, cnt, false);
         } else {
             record(c, 0);
             record(c, 1);
-            eval_potential(c, theta, dxt, dyt, m, dout, cnt, true);
+            if (ext.tree)
+                eval_potential_ext(c, *ext.tree, ext, theta, dxt, dyt, m, dout, cnt, true);
+            else
+                eval_potential(c, theta, dxt, dyt, m, dout, cnt, true);
         }
         c->ev_valid = c->profile;
         c->eval_has_tsort = true;
@@ -502,7 +560,7 @@ int run_potential_noalloc(tc_context *ctx, double theta, const double *xs, const
 
 int run_force(tc_context *ctx, double theta, const double *xs, const double *ys, const double *vs, int n,
               const double *x0s, const double *y0s, const double *hs, int nb, double *xr, double *yr,
-              int device_ptrs, bool rebuild)
+              int device_ptrs, bool rebuild, const ExtTree &ext = ExtTree())
 {
     return guarded(ctx, [&] {
         tc_context *c = ctx;
@@ -521,10 +579,12 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
             c->tree_kind = TC_KIND_NONE;
             return;
         }
-        if (!rebuild) {
+        if (!rebuild && !ext.tree) {
             check_resident(c, TC_KIND_FORCE);
             c->bev_valid = false;
         }
+        if (!rebuild && ext.tree && ext.ready)
+            TC_HIP(hipStreamWaitEvent(c->stream, ext.ready, 0));
         const double *dxs = xs, *dys = ys, *dvs = vs, *dx0 = x0s, *dy0 = y0s, *dh = hs;
         double *dxr = xr, *dyr = yr;
         if (!device_ptrs) {
@@ -547,9 +607,10 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
         unsigned long long *cnt = prepare_counters(c);
         record(c, 0);
         if (rebuild)
-            build(c, dxs, dys, dvs, n, TC_KIND_FORCE);
+            build(c, dxs, dys, dvs, n, TC_KIND_FORCE, ext);
         record(c, 1);
-        tc_eval_force(c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws, c->force_chunks,
+        tc_eval_force(ext.tree ? *ext.tree : c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws,
+                      c->force_chunks,
                       c->stream,
                       c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
@@ -592,7 +653,7 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
         }
         unsigned long long *cnt = prepare_counters(c);
         record(c, 0);
-        build(c, dxs, dys, dvs, n, TC_KIND_FORCE);
+        build(c, dxs, dys, dvs, n, TC_KIND_FORCE, ExtTree());
         record(c, 1);
         tc_eval_force_targets(c->tree, theta, dxt, dyt, m, dxr, dyr, c->precision, cnt, c->ws, c->stream,
                               c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
@@ -689,6 +750,58 @@ void dropin_force_f32(const char *fn, const float theta, const float *const xsou
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------ request-server internals
+void tc_set_last_error(const std::string &m) { set_error(m); }
+
+int tc_srv_solve(tc_context *c, const tc_request &r, TcTree &t, bool build, hipEvent_t ready,
+                 const std::function<void(const TcPotBatchItem &, hipStream_t)> *eval)
+{
+    ExtTree ext;
+    ext.tree = &t;
+    ext.build = build;
+    ext.ready = ready;
+    ext.eval = eval;
+    if (r.kind == TC_REQ_POTENTIAL)
+        return run_potential(c, r.theta, r.xs, r.ys, r.vs, build ? r.n : 0, r.xt, r.yt, r.m, r.out0, 0, build, ext);
+    return run_force(c, r.theta, r.xs, r.ys, r.vs, build ? r.n : 0, r.x0s, r.y0s, r.hs, r.nblocks, r.out0, r.out1,
+                     0, build, ext);
+}
+
+size_t tc_tree_bytes(const TcTree &t)
+{
+    size_t b = 0;
+    for (const DevBuf *d : {&t.keys, &t.perm, &t.src, &t.src32, &t.nodes, &t.exps, &t.exps32, &t.nodew, &t.selfid,
+                            &t.box_d})
+        b += d->cap;
+    return b;
+}
+
+size_t tc_srv_context_bytes(tc_context *c)
+{
+    size_t b = tc_tree_bytes(c->tree) + c->io.cap + c->cnt.cap;
+    for (const TcScratch *w : {&c->ws, &c->tws})
+        for (const DevBuf *d : {&w->k2, &w->v2, &w->hist, &w->part, &w->scan_tmp, &w->istart, &w->ilevel, &w->info,
+                                &w->tkeys, &w->tperm, &w->tsorted, &w->bbox_part, &w->ints, &w->packed, &w->work,
+                                &w->flist})
+            b += d->cap;
+    return b;
+}
+
+void tc_srv_context_trim(tc_context *c)
+{
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->tstream);
+    (void)hipStreamSynchronize(c->stream);
+    c->tree.release();
+    c->tree_kind = TC_KIND_NONE;
+    c->io.release();
+    for (TcScratch *w : {&c->ws, &c->tws})
+        for (DevBuf *d : {&w->k2, &w->v2, &w->hist, &w->part, &w->scan_tmp, &w->istart, &w->ilevel, &w->info,
+                          &w->tkeys, &w->tperm, &w->tsorted, &w->bbox_part, &w->ints, &w->packed, &w->work,
+                          &w->flist})
+            d->release();
+}
 
 extern "C" {
 

@@ -16,6 +16,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -299,6 +300,23 @@ struct TcScratch {
     }
 };
 
+// Request-server internals (tc_api.cpp, used by tc_serve.cpp).  tc_srv_solve
+// runs request r (host arrays) on context c against the shared tree t: built
+// here when build is set (then `ready` is recorded on c's stream after it), or
+// built earlier by another context whose stream recorded `ready`.  eval, when
+// set, takes the potential traversal (the server's batcher).
+struct tc_context;
+struct tc_request;
+struct TcPotBatchItem;
+int tc_srv_solve(tc_context *c, const tc_request &r, TcTree &t, bool build, hipEvent_t ready,
+                 const std::function<void(const TcPotBatchItem &, hipStream_t)> *eval);
+// Device bytes held by a context's scratch and staging buffers; release them.
+size_t tc_srv_context_bytes(tc_context *c);
+void tc_srv_context_trim(tc_context *c);
+size_t tc_tree_bytes(const TcTree &t);
+// Sets the calling thread's treecode_last_error message.
+void tc_set_last_error(const std::string &m);
+
 // Static device scratch of the no-allocation entry point (tc_noalloc.hip):
 // its size, and its address on the current device (looked up once per device
 // with hipGetSymbolAddress, then cached).
@@ -329,6 +347,39 @@ void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, h
 const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m);
 // sort_targets false: tc_sort_targets already ran on ws for these targets
 // (e.g. on another stream, ordered before st).
+// One request of a batched potential evaluation: a resident tree and targets
+// already sorted (tperm), results to out (all device memory).
+struct TcPotBatchItem {
+    const TcTree *tree;
+    double theta;
+    const double *xt, *yt;
+    const uint32_t *tperm;
+    int m;
+    double *out;
+};
+
+// Workspace of batched potential evaluations on one stream: device
+// descriptors, a ring of pinned host copies of them (a ring slot is reused
+// once the copy made from it has run), and the replayable graph of each
+// (order, precision, ring slot).
+struct TcBatchWs {
+    // size buckets: 2^6 .. 2^(6 + NBUCKET - 1) waves (64 targets each)
+    static constexpr int MAXREQ = 64, RING = 4, NBUCKET = 20;
+    DevBuf dev;
+    void *host = nullptr;
+    hipEvent_t copied[RING] = {nullptr, nullptr, nullptr, nullptr};
+    bool copy_pending[RING] = {false, false, false, false};
+    int next = 0;
+    // [(2 order + precision) * NBUCKET + bucket][ring slot]
+    hipGraphExec_t graph[2 * (TC_MAXORDER + 1) * NBUCKET][RING] = {};
+    void release();
+};
+// Evaluates k <= TcBatchWs::MAXREQ requests (same order and precision) in one
+// persistent launch (k_potential_batch) on st; with graph, through the
+// graph built for this (order, precision, size bucket).  Asynchronous.
+void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int precision, TcBatchWs &ws,
+                        unsigned long long *counters, hipStream_t st, bool graph);
+
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
                        hipEvent_t ev0, hipEvent_t ev1, bool sort_targets = true);

@@ -263,35 +263,67 @@ TC_DEV void m2l_wave_pass(int nq, const float2 *z, const float2 *co, float rw, f
 // FP32: nodes that qualify (above) go through the wave's local expansion.
 // CNT: count interactions (a separate instance, so uncounted solves carry no
 // counter arithmetic)
-template <int P, bool F32, bool CNT>
-__global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
-                                                   const float2 *__restrict__ exps32,
-                                                   const double4 *__restrict__ src,
-                                                   const float4 *__restrict__ src32, const double *__restrict__ xt,
-                                                   const double *__restrict__ yt, const uint32_t *__restrict__ tperm,
-                                                   int m, double th2, double *__restrict__ out,
-                                                   unsigned long long *counters)
+// Per-request arguments of a potential traversal (one request of a batch:
+// its waves are wave0 .. wave0 + ceil(m / 64) - 1 of the batch).
+struct TcPotReq {
+    const TcNode *nodes;
+    const double *exps;
+    const float2 *exps32;
+    const double4 *src;
+    const float4 *src32;
+    const double *xt, *yt;
+    const uint32_t *tperm;
+    double *out;
+    double th2;
+    int m, wave0;
+};
+
+// LDS of one workgroup (four waves) of the potential traversal.
+template <int P, bool F32> struct TcPotShared {
+    int node[4][TC_STACK];
+    unsigned long long mask[4][TC_STACK];
+    double4 src[4][64];
+    // FP32 wave-level local expansions: beta[l P + k - 1] = (-1)^k C(l + k - 1, l),
+    // queued node ids, their z0 = com - centre, and each wave's {centre, radius, 1 / theta'}
+    float beta[F32 ? M2L_G * P : 1];
+    int q[4][F32 ? M2L_Q : 1];
+    double4 wc[4];
+    float2 qz[4][F32 ? M2L_Q : 1];
+};
+
+template <int P, bool F32> TC_DEV void potential_shared_init(TcPotShared<P, F32> &sh)
 {
-    __shared__ int s_node[4][TC_STACK];
-    __shared__ unsigned long long s_mask[4][TC_STACK];
-    __shared__ double4 s_src[4][64];
-    // FP32 wave-level local expansions: s_beta[l P + k - 1] = (-1)^k C(l + k - 1, l),
-    // queued node ids and their staged coefficients per wave
-    __shared__ float s_beta[F32 ? M2L_G * P : 1];
-    __shared__ int s_q[4][F32 ? M2L_Q : 1];
-    __shared__ double4 s_wc[4];  // FP32: {centre x, centre y, radius} of each wave's targets, 1 / theta'
-    __shared__ float2 s_qz[4][F32 ? M2L_Q : 1];  // FP32: z0 = com - centre of each queued node
-    static_assert(!F32 || 4 * (P + 1) * sizeof(float2) <= sizeof(s_src[0]), "staged coefficients fit a source tile");
     if constexpr (F32) {
         for (int i = threadIdx.x; i < M2L_G * P; i += 256) {
             const int l = i / P, kk = i - l * P + 1;
             const double c = c_binomT[l * 64 + l + kk - 1];
-            s_beta[i] = (float)((kk & 1) ? -c : c);
+            sh.beta[i] = (float)((kk & 1) ? -c : c);
         }
         __syncthreads();
     }
+}
+
+// One wave's traversal: targets tperm[k0 .. k0 + 63] of request rq, k0 = 64 x
+// (the wave's index within the request); shared by the one-request kernel
+// and the batched one, so a request's results do not depend on which runs it.
+template <int P, bool F32, bool CNT>
+TC_DEV void potential_wave(TcPotShared<P, F32> &sh, const TcNode *__restrict__ nodes,
+                           const double *__restrict__ exps, const float2 *__restrict__ exps32,
+                           const double4 *__restrict__ src, const float4 *__restrict__ src32,
+                           const double *__restrict__ xt, const double *__restrict__ yt,
+                           const uint32_t *__restrict__ tperm, double *__restrict__ out, const double th2,
+                           const int m, int k0, unsigned long long *counters)
+{
+    int(&s_node)[4][TC_STACK] = sh.node;
+    unsigned long long(&s_mask)[4][TC_STACK] = sh.mask;
+    double4(&s_src)[4][64] = sh.src;
+    const float *s_beta = sh.beta;
+    int(&s_q)[4][F32 ? M2L_Q : 1] = sh.q;
+    double4 *s_wc = sh.wc;
+    float2(&s_qz)[4][F32 ? M2L_Q : 1] = sh.qz;
+    static_assert(!F32 || 4 * (P + 1) * sizeof(float2) <= sizeof(s_src[0]), "staged coefficients fit a source tile");
     const int wid = threadIdx.x >> 6, lane = tc_lane();
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int k = k0 + lane;
     const bool valid = k < m;
     const uint32_t ti = valid ? tperm[k] : 0u;
     const double x = valid ? xt[ti] : 0.0, y = valid ? yt[ti] : 0.0;
@@ -655,6 +687,67 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
         atomicAdd(&counters[TC_CNT_CYC_L2P], cl2p);
         atomicAdd(&counters[TC_CNT_CYC_TOTAL], tc_clock() - cyc0);
     }
+}
+
+// (separate kernel arguments rather than a TcPotReq: a by-value struct
+// argument costs this kernel registers and occupancy)
+template <int P, bool F32, bool CNT>
+__global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+                                                   const float2 *__restrict__ exps32,
+                                                   const double4 *__restrict__ src,
+                                                   const float4 *__restrict__ src32, const double *__restrict__ xt,
+                                                   const double *__restrict__ yt, const uint32_t *__restrict__ tperm,
+                                                   int m, double th2, double *__restrict__ out,
+                                                   unsigned long long *counters)
+{
+    __shared__ TcPotShared<P, F32> sh;
+    potential_shared_init<P, F32>(sh);
+    potential_wave<P, F32, CNT>(sh, nodes, exps, exps32, src, src32, xt, yt, tperm, out, th2, m,
+                                (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64, counters);
+}
+
+// Cross-request traversal: one launch evaluates the waves of all meta[0]
+// requests of a batch (meta[1] waves in total); wave g finds its request by
+// binary search over the requests' first waves and does the same per-wave
+// work as k_potential, so each result equals the one-request launch's.  The
+// descriptors and the batch size live in device memory and waves beyond
+// meta[1] exit at once, so a launch with a grid rounded up to a size bucket
+// has fixed arguments: one graph per (order, precision, size bucket) replays it.
+template <int P, bool F32, bool CNT>
+__global__ __launch_bounds__(256) void k_potential_batch(const TcPotReq *__restrict__ reqs,
+                                                         const int *__restrict__ meta,
+                                                         unsigned long long *counters)
+{
+    __shared__ TcPotShared<P, F32> sh;
+    potential_shared_init<P, F32>(sh);
+    const int nreq = meta[0], nwaves = meta[1];
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= nwaves)
+        return;
+    int lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].wave0 <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    // the descriptor is wave-uniform (scalar registers) and its pointers point
+    // to global memory: telling the compiler so lets node records and
+    // coefficients travel through scalar loads as in k_potential
+    const TcPotReq &rq = reqs[lo];
+    auto up = [](const void *p) {
+        typedef const __attribute__((address_space(1))) char *gptr;
+        const gptr q = (gptr)(tc_uniform64((unsigned long long)p));
+        return (const void *)q;
+    };
+    potential_wave<P, F32, CNT>(
+        sh, static_cast<const TcNode *>(up(rq.nodes)), static_cast<const double *>(up(rq.exps)),
+        static_cast<const float2 *>(up(rq.exps32)), static_cast<const double4 *>(up(rq.src)),
+        static_cast<const float4 *>(up(rq.src32)), static_cast<const double *>(up(rq.xt)),
+        static_cast<const double *>(up(rq.yt)), static_cast<const uint32_t *>(up(rq.tperm)),
+        const_cast<double *>(static_cast<const double *>(up(rq.out))), tc_uniform(rq.th2), tc_uniform(rq.m),
+        (g - tc_uniform(rq.wave0)) * 64, counters);
 }
 
 // ------------------------------------------------------------------ force kernels
@@ -1661,25 +1754,43 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
 
 int tiles_of(int n, int t) { return (n + t - 1) / t; }
 
+TcPotReq potential_req(const TcTree &t, double th2, const double *xt, const double *yt, const uint32_t *tperm,
+                       int m, double *out, int precision, int wave0)
+{
+    TcPotReq rq;
+    rq.nodes = t.nodes.as<TcNode>();
+    rq.exps = t.exps.as<double>();
+    rq.exps32 = precision == 1 ? t.exps32.as<float2>() : nullptr;
+    rq.src = t.src.as<double4>();
+    rq.src32 = precision == 1 ? t.src32.as<float4>() : nullptr;
+    rq.xt = xt;
+    rq.yt = yt;
+    rq.tperm = tperm;
+    rq.out = out;
+    rq.th2 = th2;
+    rq.m = m;
+    rq.wave0 = wave0;
+    return rq;
+}
+
 template <int P>
 void launch_potential(const TcTree &t, double th2, const double *xt, const double *yt, const uint32_t *tperm,
                       int m, double *out, int precision, unsigned long long *counters, hipStream_t st)
 {
     const int grid = tiles_of(m, 256);
-    auto go = [&](auto kern, const float2 *e32, const float4 *s32) {
-        kern<<<grid, 256, 0, st>>>(t.nodes.as<TcNode>(), t.exps.as<double>(), e32, t.src.as<double4>(), s32, xt, yt,
-                                   tperm, m, th2, out, counters);
+    const TcPotReq q = potential_req(t, th2, xt, yt, tperm, m, out, precision, 0);
+    auto go = [&](auto kern) {
+        kern<<<grid, 256, 0, st>>>(q.nodes, q.exps, q.exps32, q.src, q.src32, q.xt, q.yt, q.tperm, q.m, q.th2, q.out,
+                                   counters);
     };
-    const float2 *e32 = t.exps32.as<float2>();
-    const float4 *s32 = t.src32.as<float4>();
     if (precision == 1 && counters)
-        go(k_potential<P, true, true>, e32, s32);
+        go(k_potential<P, true, true>);
     else if (precision == 1)
-        go(k_potential<P, true, false>, e32, s32);
+        go(k_potential<P, true, false>);
     else if (counters)
-        go(k_potential<P, false, true>, nullptr, nullptr);
+        go(k_potential<P, false, true>);
     else
-        go(k_potential<P, false, false>, nullptr, nullptr);
+        go(k_potential<P, false, false>);
     TC_HIP(hipGetLastError());
 }
 
@@ -1834,6 +1945,164 @@ void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n,
         return;
     k_direct<<<nidx, 256, 0, st>>>(xs, ys, vs, n, xt, yt, idx, force, out0, out1);
     TC_HIP(hipGetLastError());
+}
+
+// Device layout of a batch: TcPotReq[MAXREQ], meta {k, waves}, work counter.
+constexpr size_t BATCH_DESC = sizeof(TcPotReq) * TcBatchWs::MAXREQ;
+constexpr size_t BATCH_HOST = BATCH_DESC + 64;  // descriptors + meta per ring slot
+
+template <int P, bool F32, bool CNT>
+void batch_kernel(TcBatchWs &ws, unsigned long long *counters, int grid, hipStream_t st)
+{
+    TcPotReq *desc = ws.dev.as<TcPotReq>();
+    int *meta = reinterpret_cast<int *>(ws.dev.as<char>() + BATCH_DESC);
+    k_potential_batch<P, F32, CNT><<<grid, 256, 0, st>>>(desc, meta, counters);
+    TC_HIP(hipGetLastError());
+}
+
+// The graph of one (order, precision, size bucket, ring slot): the descriptor
+// copy from that ring slot's pinned buffer, then the launch over the bucket's
+// grid.  Built node by node rather than by stream capture, so building it
+// never restricts what other threads may call meanwhile.
+template <int P, bool F32>
+hipGraphExec_t batch_graph(TcBatchWs &ws, const void *host, int grid)
+{
+    TcPotReq *desc = ws.dev.as<TcPotReq>();
+    int *meta = reinterpret_cast<int *>(ws.dev.as<char>() + BATCH_DESC);
+    unsigned long long *counters = nullptr;
+    hipGraph_t g = nullptr;
+    TC_HIP(hipGraphCreate(&g, 0));
+    hipGraphExec_t ex = nullptr;
+    try {
+        hipGraphNode_t cp = nullptr, kn = nullptr;
+        TC_HIP(hipGraphAddMemcpyNode1D(&cp, g, nullptr, 0, ws.dev.ptr, host, BATCH_HOST, hipMemcpyHostToDevice));
+        void *args[] = {&desc, &meta, &counters};
+        hipKernelNodeParams kp{};
+        kp.func = reinterpret_cast<void *>(k_potential_batch<P, F32, false>);
+        kp.gridDim = dim3(grid);
+        kp.blockDim = dim3(256);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        TC_HIP(hipGraphAddKernelNode(&kn, g, &cp, 1, &kp));
+        TC_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    } catch (...) {
+        (void)hipGraphDestroy(g);
+        throw;
+    }
+    (void)hipGraphDestroy(g);
+    return ex;
+}
+
+// size bucket of a batch: its waves rounded up to a power of two (at least 64 waves)
+int batch_bucket(int waves)
+{
+    int b = 6;
+    while ((1 << b) < waves)
+        ++b;
+    return b;
+}
+
+template <int P>
+void batch_launch(TcBatchWs &ws, int slot, int precision, int waves, unsigned long long *counters, hipStream_t st,
+                  bool graph)
+{
+    char *host = static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST;
+    const bool f32 = precision == 1;
+    const int bucket = batch_bucket(waves);
+    const int grid = (1 << bucket) / 4;
+    auto copy = [&] {
+        TC_HIP(hipMemcpyAsync(ws.dev.ptr, host, BATCH_HOST, hipMemcpyHostToDevice, st));
+    };
+    if (counters || !graph) {
+        copy();
+        TC_HIP(hipEventRecord(ws.copied[slot], st));
+        if (f32 && counters)
+            batch_kernel<P, true, true>(ws, counters, grid, st);
+        else if (f32)
+            batch_kernel<P, true, false>(ws, nullptr, grid, st);
+        else if (counters)
+            batch_kernel<P, false, true>(ws, counters, grid, st);
+        else
+            batch_kernel<P, false, false>(ws, nullptr, grid, st);
+        return;
+    }
+    hipGraphExec_t &ex = ws.graph[(2 * P + (f32 ? 1 : 0)) * TcBatchWs::NBUCKET + bucket - 6][slot];
+    if (!ex)
+        ex = f32 ? batch_graph<P, true>(ws, host, grid) : batch_graph<P, false>(ws, host, grid);
+    TC_HIP(hipGraphLaunch(ex, st));
+    TC_HIP(hipEventRecord(ws.copied[slot], st));
+}
+
+void TcBatchWs::release()
+{
+    for (auto &row : graph)
+        for (auto &g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    for (auto &e : copied)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (host)
+        (void)hipHostFree(host);
+    host = nullptr;
+    dev.release();
+}
+
+void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int precision, TcBatchWs &ws,
+                        unsigned long long *counters, hipStream_t st, bool graph)
+{
+    if (k <= 0)
+        return;
+    if (k > TcBatchWs::MAXREQ)
+        throw TcError(-3, "tc_potential_batch: at most 64 requests per batch");
+    if (!tc_order_supported(order))
+        throw TcError(-3, "tc_potential_batch: unsupported order");
+    for (int i = 0; i < k; ++i) {
+        const TcTree &t = *items[i].tree;
+        if (t.order != order || (precision == 1 && !t.has_fp32))
+            throw TcError(-3, "tc_potential_batch: a tree of another order or without FP32 mirrors");
+    }
+    if (!ws.host) {
+        TC_HIP(hipHostMalloc(&ws.host, BATCH_HOST * TcBatchWs::RING, hipHostMallocDefault));
+        for (auto &e : ws.copied)
+            TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ws.dev.reserve(BATCH_HOST + 64);
+    }
+    // a ring slot is written again only after the copy made from it has run
+    const int slot = ws.next;
+    ws.next = (ws.next + 1) % TcBatchWs::RING;
+    if (ws.copy_pending[slot])
+        TC_HIP(hipEventSynchronize(ws.copied[slot]));
+    char *host = static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST;
+    TcPotReq *desc = reinterpret_cast<TcPotReq *>(host);
+    // the largest size bucket bounds a batch's waves (2^25 waves, 2^31 targets)
+    long long total = 0;
+    for (int i = 0; i < k; ++i)
+        total += (items[i].m + 63) / 64;
+    if (total > (1ll << (TcBatchWs::NBUCKET + 5)))
+        throw TcError(-3, "tc_potential_batch: more targets than one launch takes");
+    int waves = 0;
+    for (int i = 0; i < k; ++i) {
+        const TcPotBatchItem &it = items[i];
+        desc[i] = potential_req(*it.tree, it.theta * it.theta, it.xt, it.yt, it.tperm, it.m, it.out, precision,
+                                waves);
+        waves += (it.m + 63) / 64;
+    }
+    int *meta = reinterpret_cast<int *>(host + BATCH_DESC);
+    meta[0] = k;
+    meta[1] = waves;
+    switch (order) {
+#define TC_CASE(P) \
+    case P: batch_launch<P>(ws, slot, precision, waves, counters, st, graph); break;
+        TC_ORDERS(TC_CASE)
+#undef TC_CASE
+    }
+    ws.copy_pending[slot] = true;
 }
 
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,

@@ -2,20 +2,38 @@
 // the reference, whose two functions are one-shot and non-reentrant,
 // upward.cpp:51-59).
 //
-// A server owns `nslots` slots; each slot is a tc_context (own HIP stream,
-// own grow-only HBM buffers, own resident tree) driven by its own host
-// thread.  Requests wait in one queue; whenever a slot is free a worker
-// takes the first queued request whose HBM footprint fits in the budget
-// still free, preferring a request whose sources are already resident in
-// that slot (same source_id, kind and precision), in which case the tree is
-// reused and only the new targets are evaluated.  Requests running in
-// different slots overlap on the device: copies of one request run beside
-// the kernels of others.  Kernels never communicate across requests and use
-// no cross-request atomics, so a result is bit-identical alone and in a mix.
+// A server owns `nslots` slots; each slot is a tc_context (own HIP streams,
+// own scratch) driven by its own host thread.  Requests wait in one queue;
+// whenever a slot is free a worker takes the first queued request that fits
+// the HBM budget, preferring one whose tree is already cached.
+//
+// Trees live in a cache shared by all slots, keyed by the request's
+// source_id (and kind, precision, order, leaf capacity): a request whose
+// sources were seen before evaluates its new targets against the cached tree,
+// whichever slot built it (the builder's stream records an event after the
+// build; readers wait on it on the GPU).  Trees are read-only once built.
+//
+// HBM admission counts everything the server holds: the cached trees and each
+// slot's grow-only buffers, plus an upper estimate of what each running
+// request may add.  A request that does not fit first evicts idle trees
+// (least recently used), then releases idle slots' buffers; if it still does
+// not fit it waits for running requests to finish (a request alone always
+// runs: submission rejects one larger than the whole budget).
+//
+// With TC_SERVER_BATCH, potential traversals of requests in different slots
+// are batched: each slot prepares its request (copies, tree, target order) on
+// its own stream, then hands the traversal to a leader -- the first slot
+// that finds none running -- which launches the traversals of all waiting
+// requests of the same order and precision as one kernel (k_potential_batch,
+// through a graph built once per size bucket and replayed) on the server's batch stream.
+//
+// Kernels never communicate across requests and use no cross-request atomics,
+// so a result is bit-identical alone and in a mix, batched or not.
 #include "tc_common.h"
 
 #include "../../include/treecodes2d.h"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -33,46 +51,89 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// The request's expansion order and leaf capacity (0 selects the reference's).
+int req_order(const tc_request &r) { return r.order > 0 ? r.order : (r.kind == TC_REQ_POTENTIAL ? 12 : 24); }
+int req_leaf(const tc_request &r) { return r.leaf > 0 ? r.leaf : (r.kind == TC_REQ_POTENTIAL ? 512 : 192); }
+
+size_t pos(long long v) { return v > 0 ? (size_t)v : 0; }
+
+// Upper estimate of a request's tree in HBM: sorted sources and mirrors, and
+// nodes (at most 15 n / leaf internal cells over the 15 levels, four children
+// each) with their expansions.
+size_t tree_estimate(const tc_request &r)
+{
+    const size_t n = pos(r.n);
+    const size_t p = (size_t)req_order(r);
+    const size_t nodes = 4 * (15 * n / (size_t)req_leaf(r) + 1) + 1;
+    const size_t per_node = 64 + 16 * p + 32 + 4 + (r.precision ? 8 * (p + 1) : 0);
+    return n * (4 + 4 + 32 + (r.precision ? 16 : 0)) + nodes * per_node + (64u << 10);
+}
+
+// Upper estimate of the buffers a slot's context needs for a request besides
+// the tree: host-array staging (io), sort scratch of sources and targets, and
+// for the force its near-field lists and outputs.
+size_t context_estimate(const tc_request &r, bool build)
+{
+    const size_t n = build ? pos(r.n) : 0;
+    const size_t nodes = build ? 4 * (15 * n / (size_t)req_leaf(r) + 1) + 1 : 0;
+    size_t b = n * (24 + 4 + 4 + 32 + 8) + nodes * 64 + (2u << 20);
+    if (r.kind == TC_REQ_POTENTIAL) {
+        const size_t m = pos(r.m);
+        b += m * (24 + 4 + 4 + 16 + 8 + 8);
+    } else {
+        const size_t nb = pos(r.nblocks), bricks = 16 * nb;
+        const size_t maxc = std::max<size_t>(4096, std::min<size_t>(bricks, 1u << 20) / 4);
+        b += nb * (24 + 16 * 1024) + maxc * (64 * 16 + 68 * 4) + bricks * 16;
+    }
+    return b;
+}
+
+struct TreeEntry {
+    long long source_id = 0;
+    int kind = 0, precision = 0, order = 0, leaf = 0;
+    TcTree tree;
+    hipEvent_t ready = nullptr;
+    int users = 0;        // requests reading or building it
+    bool built = false;   // the build is enqueued and `ready` recorded after it
+    unsigned long long last_use = 0;
+    size_t bytes = 0;     // exact size once built (the builder's reservation covers it before)
+};
+
 struct Job {
     long long ticket = 0;
     tc_request r{};
-    size_t footprint = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
     int status = 0;
     int slot = -1;
     bool reused = false;
     bool done = false;
+    size_t reserve = 0;  // admitted growth
+    TreeEntry *entry = nullptr;
     std::string err;
 };
 
-// The request's expansion order and leaf capacity (0 selects the reference's).
-int req_order(const tc_request &r) { return r.order > 0 ? r.order : (r.kind == TC_REQ_POTENTIAL ? 12 : 24); }
-int req_leaf(const tc_request &r) { return r.leaf > 0 ? r.leaf : (r.kind == TC_REQ_POTENTIAL ? 512 : 192); }
-
-// Upper estimate of the HBM a request occupies while it runs.
-size_t footprint(const tc_request &r)
-{
-    const size_t n = (size_t)(r.n > 0 ? r.n : 0);
-    const int p = req_order(r);
-    const int leaf = req_leaf(r);
-    const size_t nodes = 8 * n / (size_t)leaf + 64;
-    size_t b = n * (24 + 32 + 8 + 8 + 8 + (r.precision ? 16 : 0));
-    b += nodes * (64 + 32 + 24 + 16 * (size_t)p + (r.precision ? 8 * (size_t)p : 0));
-    if (r.kind == TC_REQ_POTENTIAL)
-        b += (size_t)(r.m > 0 ? r.m : 0) * (16 + 8 + 8 + 8 + 8 + 8);
-    else
-        b += (size_t)(r.nblocks > 0 ? r.nblocks : 0) * (24 + 2 * 8 * 1024);
-    return b + (1u << 20);
-}
+// A potential traversal waiting for the batch leader.
+struct BatchReq {
+    TcPotBatchItem item;
+    int order = 0, precision = 0;
+    hipEvent_t ready = nullptr, done = nullptr;  // the owning slot's events
+    bool launched = false;
+    int status = 0;
+    std::string err;
+};
 
 }  // namespace
 
 struct tc_server {
     int device = 0;
-    size_t budget = 0, in_use = 0, peak = 0;
+    int flags = 0;
+    size_t budget = 0, reserved = 0, peak = 0;
     std::vector<tc_context *> ctx;
-    std::vector<long long> res_src;
-    std::vector<int> res_kind, res_prec, res_order, res_leaf;
+    std::vector<size_t> slot_bytes;
+    std::vector<bool> slot_busy;
+    std::vector<hipEvent_t> slot_ready, slot_done;
+    std::vector<TreeEntry *> cache;
+    unsigned long long tick = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Job *> queue;
@@ -80,19 +141,188 @@ struct tc_server {
     std::vector<std::thread> workers;
     bool stop = false;
     long long next_ticket = 1;
-    long long completed = 0, reused = 0, failed = 0;
+    long long completed = 0, reused = 0, failed = 0, evictions = 0, trims = 0;
+    long long est_miss = 0;  // requests that took more than their estimate
+    size_t est_over = 0;     // the largest excess
     int inflight = 0, max_inflight = 0;
+    // batching
+    std::mutex bmu;
+    std::condition_variable cv_batch;
+    std::deque<BatchReq *> bq;
+    bool leader = false;
+    hipStream_t bstream = nullptr;
+    TcBatchWs bws;
+    long long batches = 0, batched = 0;
 };
 
 namespace {
 
-bool resident(const tc_server *s, int slot, const tc_request &r)
+size_t held(const tc_server *s)
 {
-    return r.source_id != 0 && s->res_src[slot] == r.source_id && s->res_kind[slot] == r.kind &&
-           s->res_prec[slot] == r.precision && s->res_order[slot] == req_order(r) && s->res_leaf[slot] == req_leaf(r);
+    size_t b = 0;
+    for (size_t x : s->slot_bytes)
+        b += x;
+    for (const TreeEntry *e : s->cache)
+        b += e->bytes;
+    return b;
 }
 
-void run_job(tc_server *s, int slot, Job *j)
+bool matches(const TreeEntry *e, const tc_request &r)
+{
+    return r.source_id != 0 && e->source_id == r.source_id && e->kind == r.kind && e->precision == r.precision &&
+           e->order == req_order(r) && e->leaf == req_leaf(r);
+}
+
+TreeEntry *lookup(tc_server *s, const tc_request &r)
+{
+    for (TreeEntry *e : s->cache)
+        if (e->built && matches(e, r))
+            return e;
+    return nullptr;
+}
+
+// Another slot is building this request's tree: the request waits for it.
+bool building(const tc_server *s, const tc_request &r)
+{
+    for (const TreeEntry *e : s->cache)
+        if (!e->built && matches(e, r))
+            return true;
+    return false;
+}
+
+void free_entry(tc_server *s, TreeEntry *e)
+{
+    (void)hipSetDevice(s->device);
+    if (e->ready)
+        (void)hipEventSynchronize(e->ready);
+    e->tree.release();
+    if (e->ready)
+        (void)hipEventDestroy(e->ready);
+    delete e;
+}
+
+// Growth a job may add to what the server holds.  A slot's buffers grow one
+// by one (each to the largest size any of its requests needed), so a slot
+// already holding large buffers of one kind may still grow others: the bound
+// is the request's whole estimate, not its excess over the slot's total.
+size_t job_need(const tc_request &r, bool hit)
+{
+    return context_estimate(r, !hit) + (hit ? 0 : tree_estimate(r));
+}
+
+// Makes room for `need` more bytes: idle trees first (least recently used),
+// then the buffers of idle slots; frees nothing when even all of that would
+// not be enough.  Caller holds s->mu.
+bool make_room(tc_server *s, size_t need)
+{
+    auto fits = [&] { return held(s) + s->reserved + need <= s->budget; };
+    if (fits())
+        return true;
+    size_t idle = 0;
+    for (const TreeEntry *e : s->cache)
+        if (e->users == 0 && e->built)
+            idle += e->bytes;
+    for (size_t i = 0; i < s->ctx.size(); ++i)
+        if (!s->slot_busy[i])
+            idle += s->slot_bytes[i];
+    if (held(s) - idle + s->reserved + need > s->budget)
+        return false;
+    while (!fits()) {
+        TreeEntry *victim = nullptr;
+        for (TreeEntry *e : s->cache)
+            if (e->users == 0 && e->built && (!victim || e->last_use < victim->last_use))
+                victim = e;
+        if (!victim)
+            break;
+        s->cache.erase(std::find(s->cache.begin(), s->cache.end(), victim));
+        free_entry(s, victim);
+        s->evictions++;
+    }
+    for (size_t i = 0; i < s->ctx.size() && !fits(); ++i)
+        if (!s->slot_busy[i] && s->slot_bytes[i]) {
+            tc_srv_context_trim(s->ctx[i]);
+            s->slot_bytes[i] = 0;
+            s->trims++;
+        }
+    return fits();
+}
+
+// Batch leader: launches the traversals of all waiting requests of the same
+// order and precision as the first one (up to TcBatchWs::MAXREQ) as one kernel
+// on the batch stream; each request's slot waits on its own `done` event.
+void lead_batches(tc_server *s, std::unique_lock<std::mutex> &lk)
+{
+    while (!s->bq.empty()) {
+        const int order = s->bq.front()->order, prec = s->bq.front()->precision;
+        std::vector<BatchReq *> take;
+        for (auto it = s->bq.begin(); it != s->bq.end() && (int)take.size() < TcBatchWs::MAXREQ;)
+            if ((*it)->order == order && (*it)->precision == prec) {
+                take.push_back(*it);
+                it = s->bq.erase(it);
+            } else {
+                ++it;
+            }
+        lk.unlock();
+        int status = 0;
+        std::string err;
+        try {
+            TC_HIP(hipSetDevice(s->device));
+            std::vector<TcPotBatchItem> items;
+            for (BatchReq *b : take) {
+                TC_HIP(hipStreamWaitEvent(s->bstream, b->ready, 0));
+                items.push_back(b->item);
+            }
+            tc_potential_batch(items.data(), (int)items.size(), order, prec, s->bws, nullptr, s->bstream, true);
+            for (BatchReq *b : take)
+                TC_HIP(hipEventRecord(b->done, s->bstream));
+        } catch (const TcError &e) {
+            status = e.code;
+            err = e.what();
+        } catch (const std::exception &e) {
+            status = -2;
+            err = e.what();
+        }
+        lk.lock();
+        s->batches++;
+        s->batched += (long long)take.size();
+        for (BatchReq *b : take) {
+            b->status = status;
+            b->err = err;
+            b->launched = true;
+        }
+        s->cv_batch.notify_all();
+    }
+}
+
+// The traversal hand-off of one request (a slot thread).
+void batch_eval(tc_server *s, int slot, int order, int prec, const TcPotBatchItem &it, hipStream_t st)
+{
+    BatchReq br;
+    br.item = it;
+    br.order = order;
+    br.precision = prec;
+    br.ready = s->slot_ready[slot];
+    br.done = s->slot_done[slot];
+    TC_HIP(hipEventRecord(br.ready, st));
+    std::unique_lock<std::mutex> lk(s->bmu);
+    s->bq.push_back(&br);
+    while (!br.launched) {
+        if (!s->leader) {
+            s->leader = true;
+            lead_batches(s, lk);
+            s->leader = false;
+            s->cv_batch.notify_all();
+        } else {
+            s->cv_batch.wait(lk);
+        }
+    }
+    lk.unlock();
+    if (br.status)
+        throw TcError(br.status, "treecode server: batched traversal failed: " + br.err);
+    TC_HIP(hipStreamWaitEvent(st, br.done, 0));
+}
+
+void run_job(tc_server *s, int slot, Job *j, bool build)
 {
     tc_context *c = s->ctx[slot];
     const tc_request &r = j->r;
@@ -103,13 +333,11 @@ void run_job(tc_server *s, int slot, Job *j)
     if (rc == 0)
         rc = tc_set_option(c, pot ? TC_OPT_POT_LEAF : TC_OPT_FORCE_LEAF, req_leaf(r));
     if (rc == 0) {
-        if (r.kind == TC_REQ_POTENTIAL)
-            rc = j->reused ? tc_potential_resident(c, r.theta, r.xt, r.yt, r.m, r.out0, 0)
-                           : tc_potential(c, r.theta, r.xs, r.ys, r.vs, r.n, r.xt, r.yt, r.m, r.out0, 0);
-        else
-            rc = j->reused ? tc_force_mrag_resident(c, r.theta, r.x0s, r.y0s, r.hs, r.nblocks, r.out0, r.out1, 0)
-                           : tc_force_mrag(c, r.theta, r.xs, r.ys, r.vs, r.n, r.x0s, r.y0s, r.hs, r.nblocks, r.out0,
-                                           r.out1, 0);
+        std::function<void(const TcPotBatchItem &, hipStream_t)> ev = [&](const TcPotBatchItem &it, hipStream_t st) {
+            batch_eval(s, slot, req_order(r), r.precision, it, st);
+        };
+        const bool batch = pot && (s->flags & TC_SERVER_BATCH);
+        rc = tc_srv_solve(c, r, j->entry->tree, build, j->entry->ready, batch ? &ev : nullptr);
     }
     j->status = rc;
     if (rc)
@@ -121,48 +349,95 @@ void worker(tc_server *s, int slot)
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         Job *pick = nullptr;
-        auto fits = [&](const Job *j) { return s->in_use + j->footprint <= s->budget || s->inflight == 0; };
-        for (auto it = s->queue.begin(); it != s->queue.end() && !pick; ++it)
-            if (resident(s, slot, (*it)->r) && fits(*it))
-                pick = *it;
-        for (auto it = s->queue.begin(); it != s->queue.end() && !pick; ++it)
-            if (fits(*it))
-                pick = *it;
+        TreeEntry *hit = nullptr;
+        size_t need = 0;
+        // cached trees first, then arrival order; each must fit the budget
+        for (int pass = 0; pass < 2 && !pick; ++pass)
+            for (auto it = s->queue.begin(); it != s->queue.end() && !pick; ++it) {
+                TreeEntry *e = lookup(s, (*it)->r);
+                if (pass == 0 && !e)
+                    continue;
+                if (!e && building(s, (*it)->r))
+                    continue;
+                const size_t nd = job_need((*it)->r, e != nullptr);
+                if (make_room(s, nd) || s->inflight == 0) {
+                    pick = *it;
+                    hit = e;
+                    need = nd;
+                }
+            }
         if (!pick) {
             if (s->stop)
                 return;
             s->cv_work.wait(lk);
             continue;
         }
-        for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
-            if (*it == pick) {
-                s->queue.erase(it);
-                break;
-            }
-        pick->reused = resident(s, slot, pick->r);
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), pick));
+        const bool build = hit == nullptr;
+        if (build) {
+            hit = new TreeEntry();
+            hit->source_id = pick->r.source_id;
+            hit->kind = pick->r.kind;
+            hit->precision = pick->r.precision;
+            hit->order = req_order(pick->r);
+            hit->leaf = req_leaf(pick->r);
+            (void)hipSetDevice(s->device);
+            if (hipEventCreateWithFlags(&hit->ready, hipEventDisableTiming) != hipSuccess)
+                hit->ready = nullptr;
+            s->cache.push_back(hit);
+        }
+        hit->users++;
+        hit->last_use = ++s->tick;
+        pick->entry = hit;
+        pick->reused = !build;
         pick->slot = slot;
+        pick->reserve = need;
         pick->t_start = now_ms();
-        s->in_use += pick->footprint;
-        s->peak = std::max(s->peak, s->in_use);
+        s->reserved += need;
+        s->slot_busy[slot] = true;
+        s->peak = std::max(s->peak, held(s) + s->reserved);
         s->inflight++;
         s->max_inflight = std::max(s->max_inflight, s->inflight);
         lk.unlock();
-        run_job(s, slot, pick);
+        if (!hit->ready) {
+            pick->status = -2;
+            pick->err = "treecode server: event creation failed";
+        } else {
+            run_job(s, slot, pick, build);
+        }
+        const size_t cbytes = tc_srv_context_bytes(s->ctx[slot]);
+        const size_t tbytes = build ? tc_tree_bytes(hit->tree) : 0;
         lk.lock();
-        s->in_use -= pick->footprint;
+        // the estimates admission relies on must bound what the request took
+        const size_t grew = cbytes > s->slot_bytes[slot] ? cbytes - s->slot_bytes[slot] : 0;
+        const size_t cest = context_estimate(pick->r, build), test = build ? tree_estimate(pick->r) : 0;
+        if (grew > cest || tbytes > test) {
+            s->est_miss++;
+            s->est_over = std::max(s->est_over, std::max(grew > cest ? grew - cest : 0, tbytes > test ? tbytes - test : 0));
+        }
+        s->reserved -= pick->reserve;
+        s->slot_busy[slot] = false;
+        s->slot_bytes[slot] = cbytes;
+        hit->users--;
+        if (build) {
+            // a request without sources leaves no tree worth keeping
+            if (pick->status == 0 && hit->tree.n > 0 && pick->r.source_id != 0) {
+                hit->built = true;
+                hit->bytes = tbytes;
+            } else {
+                s->cache.erase(std::find(s->cache.begin(), s->cache.end(), hit));
+                free_entry(s, hit);
+            }
+        }
+        s->peak = std::max(s->peak, held(s) + s->reserved);
         s->inflight--;
         if (pick->status == 0) {
-            s->res_src[slot] = pick->r.source_id;
-            s->res_kind[slot] = pick->r.kind;
-            s->res_prec[slot] = pick->r.precision;
-            s->res_order[slot] = req_order(pick->r);
-            s->res_leaf[slot] = req_leaf(pick->r);
             s->completed++;
             s->reused += pick->reused;
         } else {
-            s->res_src[slot] = 0;
             s->failed++;
         }
+        pick->entry = nullptr;
         pick->t_done = now_ms();
         pick->done = true;
         s->cv_done.notify_all();
@@ -174,33 +449,52 @@ void worker(tc_server *s, int slot)
 
 extern "C" {
 
-TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib)
+TC_API tc_server *tc_server_create_ex(int device, int nslots, double hbm_budget_gib, int flags)
 {
-    if (nslots < 1 || nslots > 64 || !(hbm_budget_gib > 0)) {
-        tc_context_destroy(nullptr);
+    if (nslots < 1 || nslots > 64 || !(hbm_budget_gib > 0))
         return nullptr;
-    }
     tc_server *s = new tc_server();
     s->device = device;
+    s->flags = flags;
     s->budget = (size_t)(hbm_budget_gib * 1073741824.0);
-    for (int i = 0; i < nslots; ++i) {
+    bool ok = true;
+    for (int i = 0; i < nslots && ok; ++i) {
         tc_context *c = tc_context_create(device);
         if (!c) {
-            for (tc_context *o : s->ctx)
-                tc_context_destroy(o);
-            delete s;
-            return nullptr;
+            ok = false;
+            break;
         }
         s->ctx.push_back(c);
-        s->res_src.push_back(0);
-        s->res_kind.push_back(-1);
-        s->res_order.push_back(0);
-        s->res_leaf.push_back(0);
-        s->res_prec.push_back(-1);
+        s->slot_bytes.push_back(0);
+        s->slot_busy.push_back(false);
+        hipEvent_t a = nullptr, b = nullptr;
+        ok = hipEventCreateWithFlags(&a, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b, hipEventDisableTiming) == hipSuccess;
+        s->slot_ready.push_back(a);
+        s->slot_done.push_back(b);
+    }
+    if (ok)
+        ok = hipStreamCreateWithFlags(&s->bstream, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        for (tc_context *o : s->ctx)
+            tc_context_destroy(o);
+        for (hipEvent_t e : s->slot_ready)
+            if (e)
+                (void)hipEventDestroy(e);
+        for (hipEvent_t e : s->slot_done)
+            if (e)
+                (void)hipEventDestroy(e);
+        delete s;
+        return nullptr;
     }
     for (int i = 0; i < nslots; ++i)
         s->workers.emplace_back(worker, s, i);
     return s;
+}
+
+TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib)
+{
+    return tc_server_create_ex(device, nslots, hbm_budget_gib, 0);
 }
 
 TC_API void tc_server_destroy(tc_server *s)
@@ -216,8 +510,20 @@ TC_API void tc_server_destroy(tc_server *s)
         t.join();
     for (auto &kv : s->jobs)
         delete kv.second;
+    (void)hipSetDevice(s->device);
+    if (s->bstream)
+        (void)hipStreamSynchronize(s->bstream);
+    for (TreeEntry *e : s->cache)
+        free_entry(s, e);
     for (tc_context *c : s->ctx)
         tc_context_destroy(c);
+    s->bws.release();
+    for (hipEvent_t e : s->slot_ready)
+        (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->slot_done)
+        (void)hipEventDestroy(e);
+    if (s->bstream)
+        (void)hipStreamDestroy(s->bstream);
     delete s;
 }
 
@@ -227,10 +533,10 @@ TC_API long long tc_server_submit(tc_server *s, const tc_request *r)
         return -1;
     Job *j = new Job();
     j->r = *r;
-    j->footprint = footprint(*r);
     j->t_submit = now_ms();
     std::lock_guard<std::mutex> lk(s->mu);
-    if (j->footprint > s->budget) {
+    // a request that alone exceeds the budget can never run
+    if (context_estimate(*r, true) + tree_estimate(*r) > s->budget) {
         delete j;
         return -2;
     }
@@ -259,24 +565,36 @@ TC_API int tc_server_wait(tc_server *s, long long ticket, double *times_ms)
         times_ms[4] = (double)j->slot;
     }
     const int st = j->status;
+    if (st)
+        tc_set_last_error(j->err);
     s->jobs.erase(it);
     delete j;
     return st;
 }
 
-TC_API int tc_server_stats(tc_server *s, long long *out)
+TC_API int tc_server_stats_ex(tc_server *s, long long *out, int nout)
 {
-    if (!s)
+    if (!s || !out)
         return -1;
     std::lock_guard<std::mutex> lk(s->mu);
-    out[0] = s->completed;
-    out[1] = s->reused;
-    out[2] = s->failed;
-    out[3] = s->max_inflight;
-    out[4] = (long long)s->peak;
-    out[5] = (long long)s->budget;
-    out[6] = (long long)s->queue.size();
-    return 0;
+    long long batches = 0, batched = 0;
+    {
+        std::lock_guard<std::mutex> bl(s->bmu);
+        batches = s->batches;
+        batched = s->batched;
+    }
+    const long long v[] = {s->completed, s->reused, s->failed, s->max_inflight, (long long)s->peak,
+                           (long long)s->budget, (long long)s->queue.size(), batches, batched, s->evictions, s->trims,
+                           (long long)s->cache.size(), (long long)held(s), s->est_miss, (long long)s->est_over};
+    const int n = std::min<int>(nout, (int)(sizeof(v) / sizeof(v[0])));
+    for (int i = 0; i < n; ++i)
+        out[i] = v[i];
+    return n;
+}
+
+TC_API int tc_server_stats(tc_server *s, long long *out)
+{
+    return tc_server_stats_ex(s, out, 7) == 7 ? 0 : -1;
 }
 
 }  // extern "C"

@@ -307,12 +307,13 @@ def tune_serving(cache_path):
     return done
 
 
-def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned=True):
+def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned=True, batch=False):
     """Replay the served mixed-size stream: Poisson arrivals, 70% potential / 30% force,
     25% of requests reuse the previous request's sources (the server keeps the tree).
     With `tuned`, each request carries the order and leaf capacity of the nearest
-    autotuned size bucket (FP32); others use the reference's.  Latency is measured
-    by the server from submission to completion."""
+    autotuned size bucket (FP32); others use the reference's.  With `batch`, the
+    potential traversals of concurrent requests are launched together (graph replay
+    per size bucket).  Latency is measured by the server from submission to completion."""
     import threading
     reqs = workloads.request_stream(nreq, rate_hz=rate_hz, seed=seed)
     pools = workloads.request_pools(seed=seed)
@@ -340,7 +341,7 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
             last[(r["kind"], r["src_seed"])] = ol or (0, 0)
         ntuned += ol is not None
         prepared.append((r, xs, ys, vs, tg, ol or (0, 0)))
-    srv = pkg.Server(slots=slots, hbm_budget_gib=budget_gib)
+    srv = pkg.Server(slots=slots, hbm_budget_gib=budget_gib, batch=batch)
     info = [None] * nreq
     kept = {}  # outputs of the requests whose accuracy is checked (every 40th)
     tickets = [None] * nreq
@@ -404,7 +405,11 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_service_ms": float(svc.mean()), "makespan_s": makespan,
             "targets_per_s": targets / makespan, "reused_trees": int(st["reused"]),
-            "max_inflight": int(st["max_inflight"]),
+            "reusing_requests": int(sum(p[0]["reuses_sources"] for p in prepared)),
+            "max_inflight": int(st["max_inflight"]), "batched": bool(batch),
+            "batch_launches": int(st["batches"]), "batched_requests": int(st["batched"]),
+            "budget_gib": budget_gib, "peak_admitted_gib": st["peak_bytes"] / 2 ** 30,
+            "evicted_trees": int(st["evicted"]), "trimmed_slots": int(st["trimmed"]),
             "rel_err_sampled": {"requests": len(errs), "max": max(errs), "median": float(np.median(errs))},
             "params": ("order / leaf of the nearest autotuned size bucket (2d-treecodes_amd/tuned.json) for %d of %d "
                        "requests, the reference's for the rest" % (ntuned, nreq)) if tuner else
@@ -556,6 +561,9 @@ def main():
                     help="requests of the replayed served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate of the stream (1/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget", type=float, default=16.0, help="the server's HBM budget (GiB)")
+    ap.add_argument("--no-serve-batch", action="store_true",
+                    help="replay the stream only without cross-request batching")
     ap.add_argument("--tune", action="store_true",
                     help="re-run the autotuner for this workload (grid of order x leaf) before timing")
     ap.add_argument("--err-target", type=float, default=5e-7, help="autotuner error target (default 5e-7)")
@@ -607,9 +615,17 @@ def main():
     serving = None
     if args.serve_requests > 0:
         log("served stream: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
-        serving = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, 64.0,
+        serving = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_budget,
                                tuned=not args.serve_reference_params)
-        log("  p50 %.2f ms p99 %.2f ms" % (serving["p50_ms"], serving["p99_ms"]))
+        log("  p50 %.2f ms p99 %.2f ms, %d trees reused" % (serving["p50_ms"], serving["p99_ms"],
+                                                             serving["reused_trees"]))
+        if not args.no_serve_batch:
+            log("served stream with batched traversals")
+            b = serve_stream(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_budget,
+                             tuned=not args.serve_reference_params, batch=True)
+            log("  p50 %.2f ms p99 %.2f ms, %d launches for %d traversals" % (
+                b["p50_ms"], b["p99_ms"], b["batch_launches"], b["batched_requests"]))
+            serving = {"unbatched": serving, "batched": b}
 
     res = {
         "metric": "Mtargets/s at rel err <=1e-6, FP32-bulk vs FP64 plain",

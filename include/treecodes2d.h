@@ -90,7 +90,7 @@ typedef struct tc_context tc_context;
 /* Options (tc_set_option keys) */
 enum {
     TC_OPT_PRECISION = 0,      /* 0 = plain FP64 (default), 1 = FP32 bulk P2P/E2P */
-    TC_OPT_POT_ORDER = 1,      /* potential expansion order (6,8,10,12,14,16,20,24,32), default 12 */
+    TC_OPT_POT_ORDER = 1,      /* potential expansion order (6,8,9,10,12,14,16,20,24,32), default 12 */
     TC_OPT_POT_LEAF = 2,       /* potential leaf capacity, default 512 */
     TC_OPT_FORCE_ORDER = 3,    /* force expansion order, default 24 */
     TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
@@ -204,10 +204,13 @@ TC_API int tc_morton_sort_device(tc_context *ctx, const double *x, const double 
                                  unsigned int *keys, unsigned int *perm);
 
 /* ---- request server: continuous batching over slots (no counterpart in the reference) ----
- * Each slot is a context with its own stream and host thread; queued requests are
- * admitted to free slots while their estimated HBM footprint fits the budget, and
- * requests in different slots overlap on the GPU.  All request pointers are host
- * memory and must stay valid until tc_server_wait returns for that ticket. */
+ * Each slot is a context with its own stream and host thread; requests in different
+ * slots overlap on the GPU.  Trees are cached by source_id in one cache that every slot
+ * reads.  Admission counts what the server holds (cached trees, each slot's buffers)
+ * plus an estimate of each running request's growth against the HBM budget, evicting
+ * idle trees (least recently used) and idle slots' buffers to make room.  All request
+ * pointers are host memory and must stay valid until tc_server_wait returns for that
+ * ticket. */
 typedef struct tc_server tc_server;
 
 enum { TC_REQ_POTENTIAL = 0, TC_REQ_FORCE = 1 };
@@ -230,6 +233,11 @@ typedef struct tc_request {
 } tc_request;
 
 TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib);
+/* TC_SERVER_BATCH: the potential traversals of requests running in different slots at
+ * the same time are launched together, as one kernel over all their targets (a graph
+ * built per size bucket and replayed); results are bit-identical to unbatched. */
+enum { TC_SERVER_BATCH = 1 };
+TC_API tc_server *tc_server_create_ex(int device, int nslots, double hbm_budget_gib, int flags);
 TC_API void tc_server_destroy(tc_server *srv);
 /* Returns a ticket > 0, or < 0 if the request can never fit the HBM budget. */
 TC_API long long tc_server_submit(tc_server *srv, const tc_request *req);
@@ -238,6 +246,11 @@ TC_API long long tc_server_submit(tc_server *srv, const tc_request *req);
 TC_API int tc_server_wait(tc_server *srv, long long ticket, double *times_ms);
 /* out[7] = {completed, reused trees, failed, max in flight, peak admitted bytes, budget bytes, queued} */
 TC_API int tc_server_stats(tc_server *srv, long long *out);
+/* The first nout of {completed, reused trees, failed, max in flight, peak admitted bytes,
+ * budget bytes, queued, batch launches, batched requests, evicted trees, trimmed slots,
+ * cached trees, bytes held, requests that took more HBM than admission estimated, the
+ * largest such excess in bytes}; returns how many were written. */
+TC_API int tc_server_stats_ex(tc_server *srv, long long *out, int nout);
 
 #ifdef __cplusplus
 }

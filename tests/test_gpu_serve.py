@@ -29,10 +29,10 @@ def _alone(pkg, r, xs, ys, vs, tg, precision):
         c.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_mix_equals_alone(pkg, wl, precision):
+@pytest.mark.parametrize("precision,batch", [(0, False), (1, False), (0, True), (1, True)])
+def test_mix_equals_alone(pkg, wl, precision, batch):
     items = _mix(wl)
-    srv = pkg.Server(slots=4, hbm_budget_gib=8.0)
+    srv = pkg.Server(slots=4, hbm_budget_gib=8.0, batch=batch)
     try:
         tickets = []
         for r, xs, ys, vs, tg in items:
@@ -48,6 +48,12 @@ def test_mix_equals_alone(pkg, wl, precision):
         srv.close()
     assert st["completed"] == len(items) and st["failed"] == 0
     assert st["max_inflight"] >= 2  # requests overlapped
+    npot = sum(r["kind"] == "potential" for r, *_ in items)
+    if batch:
+        # every potential traversal went through the batcher, in at most as many launches
+        assert st["batched"] == npot and 1 <= st["batches"] <= npot
+    else:
+        assert st["batched"] == 0
     for (r, xs, ys, vs, tg), (res, info) in zip(items, outs):
         ref = _alone(pkg, r, xs, ys, vs, tg, precision)
         if r["kind"] == "potential":
@@ -111,3 +117,144 @@ def test_request_order_and_leaf(pkg, wl, oracle):
     assert np.array_equal(b[::3], c)
     ref8, _ = oracle.potential(0.5, xs, ys, vs, xt, yt, order=8, leafcap=64)
     assert rel_l2(a, ref8) <= 1e-12
+
+
+def test_tree_cache_shared_by_slots(pkg, wl):
+    # a tree built by one slot serves requests in every slot
+    xs, ys, vs, xt, yt = wl.uniform_potential(60000, seed=45)
+    srv = pkg.Server(slots=4, hbm_budget_gib=4.0)
+    try:
+        first, info0 = srv.wait(srv.submit_potential(0.5, xs, ys, vs, xt, yt, source_id=11))
+        tickets = [srv.submit_potential(0.5, xs, ys, vs, xt[k::8], yt[k::8], source_id=11) for k in range(8)]
+        outs = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert not info0["reused"]
+    assert all(info["reused"] for _, info in outs)
+    assert len({info["slot"] for _, info in outs}) >= 2
+    assert st["reused"] == 8 and st["cached_trees"] == 1
+    for k, (res, _) in enumerate(outs):
+        assert np.array_equal(res, first[k::8])
+
+
+def test_waits_for_tree_being_built(pkg, wl):
+    # requests submitted together with the same sources: one builds, the others
+    # wait for its tree rather than building their own
+    xs, ys, vs, xt, yt = wl.uniform_potential(80000, seed=46)
+    srv = pkg.Server(slots=4, hbm_budget_gib=4.0)
+    try:
+        tickets = [srv.submit_potential(0.5, xs, ys, vs, xt[k::4], yt[k::4], source_id=12) for k in range(4)]
+        outs = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["reused"] == 3
+    ctx = pkg.Context()
+    ref = ctx.potential(0.5, xs, ys, vs, xt, yt)
+    ctx.close()
+    for k, (res, _) in enumerate(outs):
+        assert np.array_equal(res, ref[k::4])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_graph_replays(pkg, wl, precision):
+    # the batcher replays graphs captured per size bucket: rounds of concurrent
+    # requests of varied sizes, with eager solves in between, each result equal
+    # to the request solved alone
+    rng = np.random.default_rng(47)
+    rounds = []
+    for _ in range(4):
+        items = []
+        for j in range(10):
+            n = int(rng.integers(2000, 40000))
+            m = int(rng.integers(500, 50000))
+            xs, ys, vs, _, _ = wl.uniform_potential(n, seed=int(rng.integers(1 << 30)))
+            xt, yt = rng.uniform(-0.1, 1.1, m), rng.uniform(-0.1, 1.1, m)
+            items.append((float(rng.choice([0.3, 0.5, 0.7])), xs, ys, vs, xt, yt))
+        rounds.append(items)
+    ctx = pkg.Context(precision=precision)
+    srv = pkg.Server(slots=4, hbm_budget_gib=4.0, batch=True)
+    try:
+        for items in rounds:
+            tickets = [srv.submit_potential(*it, precision=precision) for it in items]
+            outs = [srv.wait(t)[0] for t in tickets]
+            for it, res in zip(items, outs):
+                assert np.array_equal(res, ctx.potential(*it))
+        st = srv.stats()
+    finally:
+        srv.close()
+        ctx.close()
+    assert st["batched"] == 40 and st["failed"] == 0
+
+
+def _hip():
+    import ctypes
+    import glob
+    import os
+    paths = [p for p in glob.glob("/opt/rocm/lib/libamdhip64.so*") if os.path.isfile(p)]
+    L = ctypes.CDLL(sorted(paths)[0])
+    L.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    return L
+
+
+def test_peak_device_memory_within_budget(pkg, wl):
+    # requests whose trees together exceed the budget: the server evicts idle trees
+    # and trims idle slots, and the device memory it holds (sampled while the
+    # requests run) never exceeds the budget
+    import ctypes
+    import threading
+    hip = _hip()
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    budget = 1.0
+    rng = np.random.default_rng(48)
+    items = []
+    for j in range(24):
+        n = int(rng.integers(200000, 1000000))
+        xs, ys, vs, xt, yt = wl.uniform_potential(n, seed=1000 + j)
+        items.append(("potential", j, xs, ys, vs, xt, yt))
+        if j % 3 == 0:
+            x0s, y0s, hs = wl.mrag_grid_blocks(8, 8, 1.0 / 256)
+            items.append(("force", j, xs, ys, vs, x0s, y0s, hs))
+    srv = pkg.Server(slots=4, hbm_budget_gib=budget, batch=True)
+    low = [None]
+    stop = threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            f = free_bytes()
+            low[0] = f if low[0] is None else min(low[0], f)
+            stop.wait(0.0005)
+
+    try:
+        base = free_bytes()
+        th = threading.Thread(target=sample, daemon=True)
+        th.start()
+        tickets = []
+        for kind, j, *a in items:
+            if kind == "potential":
+                tickets.append(srv.submit_potential(0.5, *a, precision=j % 2, source_id=1 + j))
+            else:
+                tickets.append(srv.submit_force(0.5, *a, precision=j % 2, source_id=1 + j))
+        for t in tickets:
+            srv.wait(t)
+        stop.set()
+        th.join()
+        st = srv.stats()
+    finally:
+        stop.set()
+        srv.close()
+    used = base - low[0]
+    msg = "sampled %.1f MiB, admitted peak %.1f MiB, budget %.0f MiB, stats %s" % (
+        used / 2 ** 20, st["peak_bytes"] / 2 ** 20, budget * 1024, st)
+    print(msg)
+    assert st["completed"] == len(items) and st["failed"] == 0
+    assert st["evicted"] > 0
+    assert st["estimate_misses"] == 0, msg
+    assert st["peak_bytes"] <= budget * 2 ** 30, msg
+    assert used <= budget * 2 ** 30, msg
