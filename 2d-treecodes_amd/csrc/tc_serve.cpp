@@ -12,13 +12,19 @@
 // sources were seen before evaluates its new targets against the cached tree,
 // whichever slot built it (the builder's stream records an event after the
 // build; readers wait on it on the GPU).  Trees are read-only once built.
+// At most 4 trees per slot stay cached: beyond, the least recently used idle
+// tree's buffers are kept for the next build instead of being freed (hipFree
+// waits for the whole device), so a steady stream allocates and frees
+// nothing once its buffers have grown.
 //
-// HBM admission counts everything the server holds: the cached trees and each
-// slot's grow-only buffers, plus an upper estimate of what each running
-// request may add.  A request that does not fit first evicts idle trees
-// (least recently used), then releases idle slots' buffers; if it still does
-// not fit it waits for running requests to finish (a request alone always
-// runs: submission rejects one larger than the whole budget).
+// HBM admission counts everything the server holds: the cached and spare
+// trees and each slot's grow-only buffers, plus an upper estimate of what each
+// running request may add.  A request that does not fit first frees spare
+// trees, then evicts idle trees (least recently used), then releases idle
+// slots' buffers (these frees, under memory pressure only, hold the server
+// lock); if it still does not fit it waits for running requests to finish (a
+// request alone always runs: submission rejects one larger than the whole
+// budget).
 //
 // With TC_SERVER_BATCH, potential traversals of requests in different slots
 // are batched: each slot prepares its request (copies, tree, target order) on
@@ -91,12 +97,19 @@ size_t context_estimate(const tc_request &r, bool build)
 struct TreeEntry {
     long long source_id = 0;
     int kind = 0, precision = 0, order = 0, leaf = 0;
-    TcTree tree;
+    TcTree *tree = nullptr;
     hipEvent_t ready = nullptr;
     int users = 0;        // requests reading or building it
     bool built = false;   // the build is enqueued and `ready` recorded after it
     unsigned long long last_use = 0;
-    size_t bytes = 0;     // exact size once built (the builder's reservation covers it before)
+    size_t bytes = 0;     // device bytes of its buffers (growth while building is reserved)
+};
+
+// Buffers of an evicted tree kept for the next build (no free, no allocation
+// unless the next tree is larger).
+struct SpareTree {
+    TcTree *tree;
+    size_t bytes;
 };
 
 struct Job {
@@ -133,6 +146,9 @@ struct tc_server {
     std::vector<bool> slot_busy;
     std::vector<hipEvent_t> slot_ready, slot_done;
     std::vector<TreeEntry *> cache;
+    std::vector<SpareTree> spare;
+    size_t max_trees = 0;  // cached trees kept; beyond, the least recently used is recycled
+    size_t freeing = 0;    // bytes evicted but not yet freed (still held)
     unsigned long long tick = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -164,7 +180,9 @@ size_t held(const tc_server *s)
         b += x;
     for (const TreeEntry *e : s->cache)
         b += e->bytes;
-    return b;
+    for (const SpareTree &t : s->spare)
+        b += t.bytes;
+    return b + s->freeing;
 }
 
 bool matches(const TreeEntry *e, const tc_request &r)
@@ -190,15 +208,36 @@ bool building(const tc_server *s, const tc_request &r)
     return false;
 }
 
-void free_entry(tc_server *s, TreeEntry *e)
+// Frees a tree's buffers (hipFree waits for the whole device: never called
+// with s->mu held while requests run).
+void free_tree(tc_server *s, TcTree *t)
 {
     (void)hipSetDevice(s->device);
-    if (e->ready)
-        (void)hipEventSynchronize(e->ready);
-    e->tree.release();
+    t->release();
+    delete t;
+}
+
+// Removes an idle entry from the cache; its tree goes to the caller.  No
+// request uses the entry, and each finished request synchronized its streams,
+// so no kernel still reads the tree.
+TcTree *drop_entry(tc_server *s, TreeEntry *e)
+{
+    s->cache.erase(std::find(s->cache.begin(), s->cache.end(), e));
+    TcTree *t = e->tree;
     if (e->ready)
         (void)hipEventDestroy(e->ready);
     delete e;
+    return t;
+}
+
+// The least recently used idle cached tree, or nullptr.
+TreeEntry *lru_idle(tc_server *s)
+{
+    TreeEntry *victim = nullptr;
+    for (TreeEntry *e : s->cache)
+        if (e->users == 0 && e->built && (!victim || e->last_use < victim->last_use))
+            victim = e;
+    return victim;
 }
 
 // Growth a job may add to what the server holds.  A slot's buffers grow one
@@ -210,32 +249,41 @@ size_t job_need(const tc_request &r, bool hit)
     return context_estimate(r, !hit) + (hit ? 0 : tree_estimate(r));
 }
 
-// Makes room for `need` more bytes: idle trees first (least recently used),
-// then the buffers of idle slots; frees nothing when even all of that would
-// not be enough.  Caller holds s->mu.
-bool make_room(tc_server *s, size_t need)
+// Makes room for `need` more bytes: spare tree buffers first, then idle
+// cached trees (least recently used), then the buffers of idle slots; takes
+// nothing when even all of that would not be enough.  Evicted trees are
+// appended to `victims`, their bytes counted in s->freeing until the caller
+// frees them.  Caller holds s->mu.
+bool make_room(tc_server *s, size_t need, std::vector<TcTree *> &victims, size_t &mine)
 {
-    auto fits = [&] { return held(s) + s->reserved + need <= s->budget; };
+    auto fits = [&] { return held(s) - mine + s->reserved + need <= s->budget; };
     if (fits())
         return true;
     size_t idle = 0;
     for (const TreeEntry *e : s->cache)
         if (e->users == 0 && e->built)
             idle += e->bytes;
+    for (const SpareTree &t : s->spare)
+        idle += t.bytes;
     for (size_t i = 0; i < s->ctx.size(); ++i)
         if (!s->slot_busy[i])
             idle += s->slot_bytes[i];
-    if (held(s) - idle + s->reserved + need > s->budget)
+    if (held(s) - mine - idle + s->reserved + need > s->budget)
         return false;
+    while (!fits() && !s->spare.empty()) {
+        victims.push_back(s->spare.back().tree);
+        s->freeing += s->spare.back().bytes;
+        mine += s->spare.back().bytes;
+        s->spare.pop_back();
+    }
     while (!fits()) {
-        TreeEntry *victim = nullptr;
-        for (TreeEntry *e : s->cache)
-            if (e->users == 0 && e->built && (!victim || e->last_use < victim->last_use))
-                victim = e;
+        TreeEntry *victim = lru_idle(s);
         if (!victim)
             break;
-        s->cache.erase(std::find(s->cache.begin(), s->cache.end(), victim));
-        free_entry(s, victim);
+        const size_t b = victim->bytes;
+        victims.push_back(drop_entry(s, victim));
+        s->freeing += b;
+        mine += b;
         s->evictions++;
     }
     for (size_t i = 0; i < s->ctx.size() && !fits(); ++i)
@@ -245,6 +293,48 @@ bool make_room(tc_server *s, size_t need)
             s->trims++;
         }
     return fits();
+}
+
+// Keeps at most s->max_trees cached trees: the least recently used idle ones
+// beyond go to the spare buffers (at most one per slot; the rest are freed by
+// the caller after releasing the lock).  Caller holds s->mu.
+void enforce_cap(tc_server *s, std::vector<TcTree *> &victims, size_t &mine)
+{
+    size_t built = 0;
+    for (const TreeEntry *e : s->cache)
+        built += e->built;
+    while (built > s->max_trees) {
+        TreeEntry *victim = lru_idle(s);
+        if (!victim)
+            break;
+        const size_t b = victim->bytes;
+        TcTree *t = drop_entry(s, victim);
+        s->evictions++;
+        --built;
+        if (s->spare.size() < s->ctx.size()) {
+            s->spare.push_back({t, b});
+        } else {
+            victims.push_back(t);
+            s->freeing += b;
+            mine += b;
+        }
+    }
+}
+
+// Frees trees evicted by the cache limit (s->mu released meanwhile; no
+// reservation counts on their bytes), then stops counting them.
+void free_victims(tc_server *s, std::unique_lock<std::mutex> &lk, std::vector<TcTree *> &victims, size_t &mine)
+{
+    if (victims.empty())
+        return;
+    lk.unlock();
+    for (TcTree *t : victims)
+        free_tree(s, t);
+    lk.lock();
+    s->freeing -= mine;
+    victims.clear();
+    mine = 0;
+    s->cv_work.notify_all();
 }
 
 // Batch leader: launches the traversals of all waiting requests of the same
@@ -337,7 +427,7 @@ void run_job(tc_server *s, int slot, Job *j, bool build)
             batch_eval(s, slot, req_order(r), r.precision, it, st);
         };
         const bool batch = pot && (s->flags & TC_SERVER_BATCH);
-        rc = tc_srv_solve(c, r, j->entry->tree, build, j->entry->ready, batch ? &ev : nullptr);
+        rc = tc_srv_solve(c, r, *j->entry->tree, build, j->entry->ready, batch ? &ev : nullptr);
     }
     j->status = rc;
     if (rc)
@@ -350,7 +440,8 @@ void worker(tc_server *s, int slot)
     while (true) {
         Job *pick = nullptr;
         TreeEntry *hit = nullptr;
-        size_t need = 0;
+        size_t need = 0, mine = 0;
+        std::vector<TcTree *> victims;
         // cached trees first, then arrival order; each must fit the budget
         for (int pass = 0; pass < 2 && !pick; ++pass)
             for (auto it = s->queue.begin(); it != s->queue.end() && !pick; ++it) {
@@ -360,13 +451,20 @@ void worker(tc_server *s, int slot)
                 if (!e && building(s, (*it)->r))
                     continue;
                 const size_t nd = job_need((*it)->r, e != nullptr);
-                if (make_room(s, nd) || s->inflight == 0) {
+                // the candidate's own tree is not evicted to make room for it
+                if (e)
+                    e->users++;
+                // alone, a request always fits once other threads' frees are done
+                if (make_room(s, nd, victims, mine) || (s->inflight == 0 && s->freeing == mine)) {
                     pick = *it;
                     hit = e;
                     need = nd;
+                } else if (e) {
+                    e->users--;
                 }
             }
         if (!pick) {
+            free_victims(s, lk, victims, mine);
             if (s->stop)
                 return;
             s->cv_work.wait(lk);
@@ -381,12 +479,22 @@ void worker(tc_server *s, int slot)
             hit->precision = pick->r.precision;
             hit->order = req_order(pick->r);
             hit->leaf = req_leaf(pick->r);
+            // the largest spare buffers, else new ones
+            if (!s->spare.empty()) {
+                auto big = std::max_element(s->spare.begin(), s->spare.end(),
+                                            [](const SpareTree &a, const SpareTree &b) { return a.bytes < b.bytes; });
+                hit->tree = big->tree;
+                hit->bytes = big->bytes;
+                s->spare.erase(big);
+            } else {
+                hit->tree = new TcTree();
+            }
             (void)hipSetDevice(s->device);
             if (hipEventCreateWithFlags(&hit->ready, hipEventDisableTiming) != hipSuccess)
                 hit->ready = nullptr;
             s->cache.push_back(hit);
+            hit->users++;
         }
-        hit->users++;
         hit->last_use = ++s->tick;
         pick->entry = hit;
         pick->reused = !build;
@@ -395,9 +503,18 @@ void worker(tc_server *s, int slot)
         pick->t_start = now_ms();
         s->reserved += need;
         s->slot_busy[slot] = true;
-        s->peak = std::max(s->peak, held(s) + s->reserved);
         s->inflight++;
         s->max_inflight = std::max(s->max_inflight, s->inflight);
+        const size_t tree_before = hit->bytes;
+        // this request's evictions (memory pressure only) are freed before it
+        // allocates, under the lock: freed outside it, their bytes would count
+        // both as held and as room this request's reservation already took
+        for (TcTree *t : victims)
+            free_tree(s, t);
+        victims.clear();
+        s->freeing -= mine;
+        mine = 0;
+        s->peak = std::max(s->peak, held(s) + s->reserved);
         lk.unlock();
         if (!hit->ready) {
             pick->status = -2;
@@ -406,27 +523,37 @@ void worker(tc_server *s, int slot)
             run_job(s, slot, pick, build);
         }
         const size_t cbytes = tc_srv_context_bytes(s->ctx[slot]);
-        const size_t tbytes = build ? tc_tree_bytes(hit->tree) : 0;
+        if (build && pick->status != 0)
+            (void)tc_synchronize(s->ctx[slot]);  // a failed build's kernels end before its buffers are reused
+        const size_t tbytes = build ? tc_tree_bytes(*hit->tree) : 0;
         lk.lock();
         // the estimates admission relies on must bound what the request took
         const size_t grew = cbytes > s->slot_bytes[slot] ? cbytes - s->slot_bytes[slot] : 0;
+        const size_t tgrew = tbytes > tree_before ? tbytes - tree_before : 0;
         const size_t cest = context_estimate(pick->r, build), test = build ? tree_estimate(pick->r) : 0;
-        if (grew > cest || tbytes > test) {
+        if (grew > cest || tgrew > test) {
             s->est_miss++;
-            s->est_over = std::max(s->est_over, std::max(grew > cest ? grew - cest : 0, tbytes > test ? tbytes - test : 0));
+            s->est_over = std::max(s->est_over, std::max(grew > cest ? grew - cest : 0, tgrew > test ? tgrew - test : 0));
         }
         s->reserved -= pick->reserve;
         s->slot_busy[slot] = false;
         s->slot_bytes[slot] = cbytes;
         hit->users--;
         if (build) {
+            hit->bytes = tbytes;
             // a request without sources leaves no tree worth keeping
-            if (pick->status == 0 && hit->tree.n > 0 && pick->r.source_id != 0) {
+            if (pick->status == 0 && hit->tree->n > 0 && pick->r.source_id != 0) {
                 hit->built = true;
-                hit->bytes = tbytes;
+                enforce_cap(s, victims, mine);
             } else {
-                s->cache.erase(std::find(s->cache.begin(), s->cache.end(), hit));
-                free_entry(s, hit);
+                TcTree *t = drop_entry(s, hit);
+                s->spare.push_back({t, tbytes});
+                if (s->spare.size() > s->ctx.size()) {
+                    victims.push_back(s->spare.front().tree);
+                    s->freeing += s->spare.front().bytes;
+                    mine += s->spare.front().bytes;
+                    s->spare.erase(s->spare.begin());
+                }
             }
         }
         s->peak = std::max(s->peak, held(s) + s->reserved);
@@ -442,6 +569,7 @@ void worker(tc_server *s, int slot)
         pick->done = true;
         s->cv_done.notify_all();
         s->cv_work.notify_all();
+        free_victims(s, lk, victims, mine);
     }
 }
 
@@ -457,6 +585,7 @@ TC_API tc_server *tc_server_create_ex(int device, int nslots, double hbm_budget_
     s->device = device;
     s->flags = flags;
     s->budget = (size_t)(hbm_budget_gib * 1073741824.0);
+    s->max_trees = 4 * (size_t)nslots;
     bool ok = true;
     for (int i = 0; i < nslots && ok; ++i) {
         tc_context *c = tc_context_create(device);
@@ -513,8 +642,14 @@ TC_API void tc_server_destroy(tc_server *s)
     (void)hipSetDevice(s->device);
     if (s->bstream)
         (void)hipStreamSynchronize(s->bstream);
-    for (TreeEntry *e : s->cache)
-        free_entry(s, e);
+    for (TreeEntry *e : s->cache) {
+        if (e->ready)
+            (void)hipEventDestroy(e->ready);
+        free_tree(s, e->tree);
+        delete e;
+    }
+    for (SpareTree &t : s->spare)
+        free_tree(s, t.tree);
     for (tc_context *c : s->ctx)
         tc_context_destroy(c);
     s->bws.release();

@@ -206,7 +206,8 @@ TC_API int tc_morton_sort_device(tc_context *ctx, const double *x, const double 
 /* ---- request server: continuous batching over slots (no counterpart in the reference) ----
  * Each slot is a context with its own stream and host thread; requests in different
  * slots overlap on the GPU.  Trees are cached by source_id in one cache that every slot
- * reads.  Admission counts what the server holds (cached trees, each slot's buffers)
+ * reads (at most 4 per slot; the least recently used idle tree's buffers are recycled
+ * into the next build).  Admission counts what the server holds (trees, each slot's buffers)
  * plus an estimate of each running request's growth against the HBM budget, evicting
  * idle trees (least recently used) and idle slots' buffers to make room.  All request
  * pointers are host memory and must stay valid until tc_server_wait returns for that
