@@ -37,6 +37,7 @@ EXPORTED = [
     "tc_get_profile", "tc_server_create_ex", "tc_server_stats_ex",
 ]
 SERVER_BATCH = 1  # tc_server_create_ex flag: batch concurrent potential traversals
+SERVER_BATCH_ALWAYS = 2  # also a lone traversal goes through the batch kernel (tests its graphs)
 NOALLOC_MAX_POINTS = 1 << 18  # sources and targets per tc_potential_noalloc call
 NOALLOC_SCRATCH_BYTES = 64 << 20  # its static device scratch
 
@@ -467,10 +468,12 @@ class Server:
     batch=True the potential traversals of concurrent requests run as one kernel launch."""
 
     STATS = ["completed", "reused", "failed", "max_inflight", "peak_bytes", "budget_bytes", "queued", "batches",
-             "batched", "evicted", "trimmed", "cached_trees", "held_bytes", "estimate_misses", "estimate_excess"]
+             "batched", "evicted", "trimmed", "cached_trees", "held_bytes", "estimate_misses", "estimate_excess", "merged"]
 
     def __init__(self, device=0, slots=4, hbm_budget_gib=64.0, batch=False):
-        self.h = lib().tc_server_create_ex(device, slots, float(hbm_budget_gib), SERVER_BATCH if batch else 0)
+        """batch: False, True, or "always" (a lone traversal also goes through the batch kernel)."""
+        flags = SERVER_BATCH_ALWAYS if batch == "always" else SERVER_BATCH if batch else 0
+        self.h = lib().tc_server_create_ex(device, slots, float(hbm_budget_gib), flags)
         if not self.h:
             raise TreecodeError("tc_server_create failed: " + last_error())
         self._keep = {}

@@ -364,21 +364,24 @@ struct TcPotBatchItem {
 // (order, precision, ring slot).
 struct TcBatchWs {
     // size buckets: 2^6 .. 2^(6 + NBUCKET - 1) waves (64 targets each)
-    static constexpr int MAXREQ = 64, RING = 4, NBUCKET = 20;
-    DevBuf dev;
-    void *host = nullptr;
-    hipEvent_t copied[RING] = {nullptr, nullptr, nullptr, nullptr};
-    bool copy_pending[RING] = {false, false, false, false};
+    static constexpr int MAXREQ = 64, RING = 16, NBUCKET = 20;
+    DevBuf dev;              // RING descriptor blocks
+    void *host = nullptr;    // RING pinned descriptor blocks
+    hipEvent_t done[RING] = {};  // recorded after the launch that read the block
+    bool pending[RING] = {};
     int next = 0;
     // [(2 order + precision) * NBUCKET + bucket][ring slot]
     hipGraphExec_t graph[2 * (TC_MAXORDER + 1) * NBUCKET][RING] = {};
     void release();
 };
 // Evaluates k <= TcBatchWs::MAXREQ requests (same order and precision) in one
-// persistent launch (k_potential_batch) on st; with graph, through the
-// graph built for this (order, precision, size bucket).  Asynchronous.
+// launch of k_potential_batch on st; with graph, through the graph built for
+// this (order, precision, size bucket, descriptor block).  Asynchronous; one
+// caller at a time per workspace (launches may go to different streams).
 void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int precision, TcBatchWs &ws,
                         unsigned long long *counters, hipStream_t st, bool graph);
+// One request's traversal by the one-request kernel (k_potential) on st.
+void tc_potential_item(const TcPotBatchItem &it, int precision, unsigned long long *counters, hipStream_t st);
 
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,

@@ -1947,35 +1947,37 @@ void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n,
     TC_HIP(hipGetLastError());
 }
 
-// Device layout of a batch: TcPotReq[MAXREQ], meta {k, waves}, work counter.
+// Device layout of a batch (one descriptor block): TcPotReq[MAXREQ], meta {k, waves}.
 constexpr size_t BATCH_DESC = sizeof(TcPotReq) * TcBatchWs::MAXREQ;
-constexpr size_t BATCH_HOST = BATCH_DESC + 64;  // descriptors + meta per ring slot
+constexpr size_t BATCH_HOST = BATCH_DESC + 256;  // one block, a multiple of 256 bytes
+
+TcPotReq *batch_desc(TcBatchWs &ws, int slot) { return reinterpret_cast<TcPotReq *>(ws.dev.as<char>() + (size_t)slot * BATCH_HOST); }
+int *batch_meta(TcBatchWs &ws, int slot) { return reinterpret_cast<int *>(ws.dev.as<char>() + (size_t)slot * BATCH_HOST + BATCH_DESC); }
 
 template <int P, bool F32, bool CNT>
-void batch_kernel(TcBatchWs &ws, unsigned long long *counters, int grid, hipStream_t st)
+void batch_kernel(TcBatchWs &ws, int slot, unsigned long long *counters, int grid, hipStream_t st)
 {
-    TcPotReq *desc = ws.dev.as<TcPotReq>();
-    int *meta = reinterpret_cast<int *>(ws.dev.as<char>() + BATCH_DESC);
-    k_potential_batch<P, F32, CNT><<<grid, 256, 0, st>>>(desc, meta, counters);
+    k_potential_batch<P, F32, CNT><<<grid, 256, 0, st>>>(batch_desc(ws, slot), batch_meta(ws, slot), counters);
     TC_HIP(hipGetLastError());
 }
 
-// The graph of one (order, precision, size bucket, ring slot): the descriptor
-// copy from that ring slot's pinned buffer, then the launch over the bucket's
-// grid.  Built node by node rather than by stream capture, so building it
-// never restricts what other threads may call meanwhile.
+// The graph of one (order, precision, size bucket, descriptor block): the
+// copy of that block from pinned host memory, then the launch over the
+// bucket's grid.  Built node by node rather than by stream capture, so
+// building it never restricts what other threads may call meanwhile.
 template <int P, bool F32>
-hipGraphExec_t batch_graph(TcBatchWs &ws, const void *host, int grid)
+hipGraphExec_t batch_graph(TcBatchWs &ws, int slot, int grid)
 {
-    TcPotReq *desc = ws.dev.as<TcPotReq>();
-    int *meta = reinterpret_cast<int *>(ws.dev.as<char>() + BATCH_DESC);
+    TcPotReq *desc = batch_desc(ws, slot);
+    int *meta = batch_meta(ws, slot);
     unsigned long long *counters = nullptr;
+    const char *host = static_cast<const char *>(ws.host) + (size_t)slot * BATCH_HOST;
     hipGraph_t g = nullptr;
     TC_HIP(hipGraphCreate(&g, 0));
     hipGraphExec_t ex = nullptr;
     try {
         hipGraphNode_t cp = nullptr, kn = nullptr;
-        TC_HIP(hipGraphAddMemcpyNode1D(&cp, g, nullptr, 0, ws.dev.ptr, host, BATCH_HOST, hipMemcpyHostToDevice));
+        TC_HIP(hipGraphAddMemcpyNode1D(&cp, g, nullptr, 0, desc, host, BATCH_HOST, hipMemcpyHostToDevice));
         void *args[] = {&desc, &meta, &counters};
         hipKernelNodeParams kp{};
         kp.func = reinterpret_cast<void *>(k_potential_batch<P, F32, false>);
@@ -2007,31 +2009,26 @@ template <int P>
 void batch_launch(TcBatchWs &ws, int slot, int precision, int waves, unsigned long long *counters, hipStream_t st,
                   bool graph)
 {
-    char *host = static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST;
     const bool f32 = precision == 1;
     const int bucket = batch_bucket(waves);
     const int grid = (1 << bucket) / 4;
-    auto copy = [&] {
-        TC_HIP(hipMemcpyAsync(ws.dev.ptr, host, BATCH_HOST, hipMemcpyHostToDevice, st));
-    };
     if (counters || !graph) {
-        copy();
-        TC_HIP(hipEventRecord(ws.copied[slot], st));
+        TC_HIP(hipMemcpyAsync(batch_desc(ws, slot), static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST,
+                              BATCH_HOST, hipMemcpyHostToDevice, st));
         if (f32 && counters)
-            batch_kernel<P, true, true>(ws, counters, grid, st);
+            batch_kernel<P, true, true>(ws, slot, counters, grid, st);
         else if (f32)
-            batch_kernel<P, true, false>(ws, nullptr, grid, st);
+            batch_kernel<P, true, false>(ws, slot, nullptr, grid, st);
         else if (counters)
-            batch_kernel<P, false, true>(ws, counters, grid, st);
+            batch_kernel<P, false, true>(ws, slot, counters, grid, st);
         else
-            batch_kernel<P, false, false>(ws, nullptr, grid, st);
+            batch_kernel<P, false, false>(ws, slot, nullptr, grid, st);
         return;
     }
     hipGraphExec_t &ex = ws.graph[(2 * P + (f32 ? 1 : 0)) * TcBatchWs::NBUCKET + bucket - 6][slot];
     if (!ex)
-        ex = f32 ? batch_graph<P, true>(ws, host, grid) : batch_graph<P, false>(ws, host, grid);
+        ex = f32 ? batch_graph<P, true>(ws, slot, grid) : batch_graph<P, false>(ws, slot, grid);
     TC_HIP(hipGraphLaunch(ex, st));
-    TC_HIP(hipEventRecord(ws.copied[slot], st));
 }
 
 void TcBatchWs::release()
@@ -2042,11 +2039,15 @@ void TcBatchWs::release()
                 (void)hipGraphExecDestroy(g);
                 g = nullptr;
             }
-    for (auto &e : copied)
-        if (e) {
-            (void)hipEventDestroy(e);
-            e = nullptr;
+    for (int i = 0; i < RING; ++i) {
+        if (done[i]) {
+            if (pending[i])
+                (void)hipEventSynchronize(done[i]);
+            (void)hipEventDestroy(done[i]);
+            done[i] = nullptr;
         }
+        pending[i] = false;
+    }
     if (host)
         (void)hipHostFree(host);
     host = nullptr;
@@ -2067,25 +2068,28 @@ void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int preci
         if (t.order != order || (precision == 1 && !t.has_fp32))
             throw TcError(-3, "tc_potential_batch: a tree of another order or without FP32 mirrors");
     }
-    if (!ws.host) {
-        TC_HIP(hipHostMalloc(&ws.host, BATCH_HOST * TcBatchWs::RING, hipHostMallocDefault));
-        for (auto &e : ws.copied)
-            TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        ws.dev.reserve(BATCH_HOST + 64);
-    }
-    // a ring slot is written again only after the copy made from it has run
-    const int slot = ws.next;
-    ws.next = (ws.next + 1) % TcBatchWs::RING;
-    if (ws.copy_pending[slot])
-        TC_HIP(hipEventSynchronize(ws.copied[slot]));
-    char *host = static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST;
-    TcPotReq *desc = reinterpret_cast<TcPotReq *>(host);
     // the largest size bucket bounds a batch's waves (2^25 waves, 2^31 targets)
     long long total = 0;
     for (int i = 0; i < k; ++i)
         total += (items[i].m + 63) / 64;
     if (total > (1ll << (TcBatchWs::NBUCKET + 5)))
         throw TcError(-3, "tc_potential_batch: more targets than one launch takes");
+    if (!ws.host) {
+        TC_HIP(hipHostMalloc(&ws.host, BATCH_HOST * TcBatchWs::RING, hipHostMallocDefault));
+        for (auto &e : ws.done)
+            TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ws.dev.reserve(BATCH_HOST * TcBatchWs::RING);
+    }
+    // a descriptor block is written again only after the launch that read it
+    // has finished (launches may run on different streams)
+    const int slot = ws.next;
+    ws.next = (ws.next + 1) % TcBatchWs::RING;
+    if (ws.pending[slot]) {
+        TC_HIP(hipEventSynchronize(ws.done[slot]));
+        ws.pending[slot] = false;
+    }
+    char *host = static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST;
+    TcPotReq *desc = reinterpret_cast<TcPotReq *>(host);
     int waves = 0;
     for (int i = 0; i < k; ++i) {
         const TcPotBatchItem &it = items[i];
@@ -2102,7 +2106,24 @@ void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int preci
         TC_ORDERS(TC_CASE)
 #undef TC_CASE
     }
-    ws.copy_pending[slot] = true;
+    TC_HIP(hipEventRecord(ws.done[slot], st));
+    ws.pending[slot] = true;
+}
+
+void tc_potential_item(const TcPotBatchItem &it, int precision, unsigned long long *counters, hipStream_t st)
+{
+    if (it.m <= 0)
+        return;
+    const TcTree &t = *it.tree;
+    if (precision == 1 && !t.has_fp32)
+        throw TcError(-3, "tc_potential_item: tree was built without FP32 mirrors");
+    switch (t.order) {
+#define TC_CASE(P) \
+    case P: launch_potential<P>(t, it.theta * it.theta, it.xt, it.yt, it.tperm, it.m, it.out, precision, counters, st); break;
+        TC_ORDERS(TC_CASE)
+#undef TC_CASE
+    default: throw TcError(-3, "tc_potential_item: unsupported order");
+    }
 }
 
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,

@@ -31,7 +31,8 @@
 // its own stream, then hands the traversal to a leader -- the first slot
 // that finds none running -- which launches the traversals of all waiting
 // requests of the same order and precision as one kernel (k_potential_batch,
-// through a graph built once per size bucket and replayed) on the server's batch stream.
+// through a graph built once per size bucket and replayed) on one of their
+// streams; a traversal that finds no other waiting runs unbatched.
 //
 // Kernels never communicate across requests and use no cross-request atomics,
 // so a result is bit-identical alone and in a mix, batched or not.
@@ -129,8 +130,10 @@ struct Job {
 struct BatchReq {
     TcPotBatchItem item;
     int order = 0, precision = 0;
+    hipStream_t stream = nullptr;                // the owning slot's stream
     hipEvent_t ready = nullptr, done = nullptr;  // the owning slot's events
     bool launched = false;
+    bool wait_done = false;  // launched on another slot's stream: wait for `done`
     int status = 0;
     std::string err;
 };
@@ -166,9 +169,8 @@ struct tc_server {
     std::condition_variable cv_batch;
     std::deque<BatchReq *> bq;
     bool leader = false;
-    hipStream_t bstream = nullptr;
     TcBatchWs bws;
-    long long batches = 0, batched = 0;
+    long long batches = 0, batched = 0, merged = 0;
 };
 
 namespace {
@@ -337,9 +339,13 @@ void free_victims(tc_server *s, std::unique_lock<std::mutex> &lk, std::vector<Tc
     s->cv_work.notify_all();
 }
 
-// Batch leader: launches the traversals of all waiting requests of the same
-// order and precision as the first one (up to TcBatchWs::MAXREQ) as one kernel
-// on the batch stream; each request's slot waits on its own `done` event.
+// Batch leader: takes all waiting traversals of the same order and precision
+// as the first one (up to TcBatchWs::MAXREQ).  One alone is launched as the
+// one-request kernel on its owner's stream (idle: its thread is waiting
+// here), so it costs nothing over an unbatched request.  Several are launched
+// as one k_potential_batch on the first one's stream, after the others'
+// `ready` events; the others then wait on their `done` events.  Launches of
+// successive batches go to different streams and overlap.
 void lead_batches(tc_server *s, std::unique_lock<std::mutex> &lk)
 {
     while (!s->bq.empty()) {
@@ -357,14 +363,23 @@ void lead_batches(tc_server *s, std::unique_lock<std::mutex> &lk)
         std::string err;
         try {
             TC_HIP(hipSetDevice(s->device));
-            std::vector<TcPotBatchItem> items;
-            for (BatchReq *b : take) {
-                TC_HIP(hipStreamWaitEvent(s->bstream, b->ready, 0));
-                items.push_back(b->item);
+            if (take.size() == 1 && !(s->flags & TC_SERVER_BATCH_ALWAYS)) {
+                tc_potential_item(take[0]->item, prec, nullptr, take[0]->stream);
+            } else {
+                hipStream_t st = take[0]->stream;
+                std::vector<TcPotBatchItem> items;
+                for (BatchReq *b : take) {
+                    if (b != take[0])
+                        TC_HIP(hipStreamWaitEvent(st, b->ready, 0));
+                    items.push_back(b->item);
+                }
+                tc_potential_batch(items.data(), (int)items.size(), order, prec, s->bws, nullptr, st, true);
+                for (BatchReq *b : take)
+                    if (b != take[0]) {
+                        TC_HIP(hipEventRecord(b->done, st));
+                        b->wait_done = true;
+                    }
             }
-            tc_potential_batch(items.data(), (int)items.size(), order, prec, s->bws, nullptr, s->bstream, true);
-            for (BatchReq *b : take)
-                TC_HIP(hipEventRecord(b->done, s->bstream));
         } catch (const TcError &e) {
             status = e.code;
             err = e.what();
@@ -375,6 +390,7 @@ void lead_batches(tc_server *s, std::unique_lock<std::mutex> &lk)
         lk.lock();
         s->batches++;
         s->batched += (long long)take.size();
+        s->merged += take.size() > 1 ? (long long)take.size() : 0;
         for (BatchReq *b : take) {
             b->status = status;
             b->err = err;
@@ -391,6 +407,7 @@ void batch_eval(tc_server *s, int slot, int order, int prec, const TcPotBatchIte
     br.item = it;
     br.order = order;
     br.precision = prec;
+    br.stream = st;
     br.ready = s->slot_ready[slot];
     br.done = s->slot_done[slot];
     TC_HIP(hipEventRecord(br.ready, st));
@@ -409,7 +426,8 @@ void batch_eval(tc_server *s, int slot, int order, int prec, const TcPotBatchIte
     lk.unlock();
     if (br.status)
         throw TcError(br.status, "treecode server: batched traversal failed: " + br.err);
-    TC_HIP(hipStreamWaitEvent(st, br.done, 0));
+    if (br.wait_done)
+        TC_HIP(hipStreamWaitEvent(st, br.done, 0));
 }
 
 void run_job(tc_server *s, int slot, Job *j, bool build)
@@ -426,7 +444,7 @@ void run_job(tc_server *s, int slot, Job *j, bool build)
         std::function<void(const TcPotBatchItem &, hipStream_t)> ev = [&](const TcPotBatchItem &it, hipStream_t st) {
             batch_eval(s, slot, req_order(r), r.precision, it, st);
         };
-        const bool batch = pot && (s->flags & TC_SERVER_BATCH);
+        const bool batch = pot && (s->flags & (TC_SERVER_BATCH | TC_SERVER_BATCH_ALWAYS));
         rc = tc_srv_solve(c, r, *j->entry->tree, build, j->entry->ready, batch ? &ev : nullptr);
     }
     j->status = rc;
@@ -602,8 +620,6 @@ TC_API tc_server *tc_server_create_ex(int device, int nslots, double hbm_budget_
         s->slot_ready.push_back(a);
         s->slot_done.push_back(b);
     }
-    if (ok)
-        ok = hipStreamCreateWithFlags(&s->bstream, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
         for (tc_context *o : s->ctx)
             tc_context_destroy(o);
@@ -640,8 +656,7 @@ TC_API void tc_server_destroy(tc_server *s)
     for (auto &kv : s->jobs)
         delete kv.second;
     (void)hipSetDevice(s->device);
-    if (s->bstream)
-        (void)hipStreamSynchronize(s->bstream);
+    s->bws.release();
     for (TreeEntry *e : s->cache) {
         if (e->ready)
             (void)hipEventDestroy(e->ready);
@@ -652,13 +667,10 @@ TC_API void tc_server_destroy(tc_server *s)
         free_tree(s, t.tree);
     for (tc_context *c : s->ctx)
         tc_context_destroy(c);
-    s->bws.release();
     for (hipEvent_t e : s->slot_ready)
         (void)hipEventDestroy(e);
     for (hipEvent_t e : s->slot_done)
         (void)hipEventDestroy(e);
-    if (s->bstream)
-        (void)hipStreamDestroy(s->bstream);
     delete s;
 }
 
@@ -712,15 +724,16 @@ TC_API int tc_server_stats_ex(tc_server *s, long long *out, int nout)
     if (!s || !out)
         return -1;
     std::lock_guard<std::mutex> lk(s->mu);
-    long long batches = 0, batched = 0;
+    long long batches = 0, batched = 0, merged = 0;
     {
         std::lock_guard<std::mutex> bl(s->bmu);
         batches = s->batches;
         batched = s->batched;
+        merged = s->merged;
     }
     const long long v[] = {s->completed, s->reused, s->failed, s->max_inflight, (long long)s->peak,
                            (long long)s->budget, (long long)s->queue.size(), batches, batched, s->evictions, s->trims,
-                           (long long)s->cache.size(), (long long)held(s), s->est_miss, (long long)s->est_over};
+                           (long long)s->cache.size(), (long long)held(s), s->est_miss, (long long)s->est_over, merged};
     const int n = std::min<int>(nout, (int)(sizeof(v) / sizeof(v[0])));
     for (int i = 0; i < n; ++i)
         out[i] = v[i];

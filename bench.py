@@ -307,7 +307,7 @@ def tune_serving(cache_path):
     return done
 
 
-def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned=True, batch=False):
+def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned=True, batch=False, **mix):
     """Replay the served mixed-size stream: Poisson arrivals, 70% potential / 30% force,
     25% of requests reuse the previous request's sources (the server keeps the tree).
     With `tuned`, each request carries the order and leaf capacity of the nearest
@@ -315,7 +315,7 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
     potential traversals of concurrent requests are launched together (graph replay
     per size bucket).  Latency is measured by the server from submission to completion."""
     import threading
-    reqs = workloads.request_stream(nreq, rate_hz=rate_hz, seed=seed)
+    reqs = workloads.request_stream(nreq, rate_hz=rate_hz, seed=seed, **mix)
     pools = workloads.request_pools(seed=seed)
     tuner = None
     if tuned and precision == 1:
@@ -408,6 +408,7 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
             "reusing_requests": int(sum(p[0]["reuses_sources"] for p in prepared)),
             "max_inflight": int(st["max_inflight"]), "batched": bool(batch),
             "batch_launches": int(st["batches"]), "batched_requests": int(st["batched"]),
+            "merged_requests": int(st["merged"]),
             "budget_gib": budget_gib, "peak_admitted_gib": st["peak_bytes"] / 2 ** 30,
             "evicted_trees": int(st["evicted"]), "trimmed_slots": int(st["trimmed"]),
             "rel_err_sampled": {"requests": len(errs), "max": max(errs), "median": float(np.median(errs))},
@@ -625,7 +626,18 @@ def main():
                              tuned=not args.serve_reference_params, batch=True)
             log("  p50 %.2f ms p99 %.2f ms, %d launches for %d traversals" % (
                 b["p50_ms"], b["p99_ms"], b["batch_launches"], b["batched_requests"]))
-            serving = {"unbatched": serving, "batched": b}
+            # a burst: 3000 small potential requests at once, where traversals of
+            # different slots do meet
+            burst = {}
+            for mode in (False, True):
+                r = serve_stream(3000, 1e6, args.serve_slots, args.serve_budget, batch=mode, pot_frac=1.0,
+                                 pot_n=(1e3, 3e4))
+                burst["batched" if mode else "unbatched"] = {k: r[k] for k in (
+                    "p50_ms", "p99_ms", "makespan_s", "mean_service_ms", "batch_launches", "batched_requests",
+                    "merged_requests")}
+            log("  burst of 3000 small requests: makespan %.3f s unbatched, %.3f s batched" % (
+                burst["unbatched"]["makespan_s"], burst["batched"]["makespan_s"]))
+            serving = {"unbatched": serving, "batched": b, "burst_3000_small": burst}
 
     res = {
         "metric": "Mtargets/s at rel err <=1e-6, FP32-bulk vs FP64 plain",

@@ -236,8 +236,13 @@ typedef struct tc_request {
 TC_API tc_server *tc_server_create(int device, int nslots, double hbm_budget_gib);
 /* TC_SERVER_BATCH: the potential traversals of requests running in different slots at
  * the same time are launched together, as one kernel over all their targets (a graph
- * built per size bucket and replayed); results are bit-identical to unbatched. */
-enum { TC_SERVER_BATCH = 1 };
+ * built per size bucket and replayed); a traversal with no other waiting is launched
+ * alone.  Results are bit-identical to unbatched. */
+enum {
+    TC_SERVER_BATCH = 1,
+    TC_SERVER_BATCH_ALWAYS = 2  /* also a lone traversal goes through the batch kernel and its graphs
+                                 * (for testing the graphs; slower) */
+};
 TC_API tc_server *tc_server_create_ex(int device, int nslots, double hbm_budget_gib, int flags);
 TC_API void tc_server_destroy(tc_server *srv);
 /* Returns a ticket > 0, or < 0 if the request can never fit the HBM budget. */
@@ -250,7 +255,8 @@ TC_API int tc_server_stats(tc_server *srv, long long *out);
 /* The first nout of {completed, reused trees, failed, max in flight, peak admitted bytes,
  * budget bytes, queued, batch launches, batched requests, evicted trees, trimmed slots,
  * cached trees, bytes held, requests that took more HBM than admission estimated, the
- * largest such excess in bytes}; returns how many were written. */
+ * largest such excess in bytes, requests launched in batches of two or more}; returns how
+ * many were written. */
 TC_API int tc_server_stats_ex(tc_server *srv, long long *out, int nout);
 
 #ifdef __cplusplus

@@ -174,7 +174,7 @@ def test_batched_graph_replays(pkg, wl, precision):
             items.append((float(rng.choice([0.3, 0.5, 0.7])), xs, ys, vs, xt, yt))
         rounds.append(items)
     ctx = pkg.Context(precision=precision)
-    srv = pkg.Server(slots=4, hbm_budget_gib=4.0, batch=True)
+    srv = pkg.Server(slots=4, hbm_budget_gib=4.0, batch="always")
     try:
         for items in rounds:
             tickets = [srv.submit_potential(*it, precision=precision) for it in items]
@@ -186,6 +186,7 @@ def test_batched_graph_replays(pkg, wl, precision):
         srv.close()
         ctx.close()
     assert st["batched"] == 40 and st["failed"] == 0
+    assert st["batches"] <= 40
 
 
 def _hip():
