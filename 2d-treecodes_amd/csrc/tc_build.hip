@@ -941,7 +941,7 @@ __global__ __launch_bounds__(256) void k_leaf_upward(const uint32_t *child, cons
         nd.level = l;
         nd.cx = cc.x;
         nd.cy = cc.y;
-        nd.pad0 = nd.pad1 = 0;
+        nd.rr = rad * rad;
         nodes[id] = nd;
         nodew[id] = make_double4(w, wx, wy, 0.0);
     }
@@ -1108,7 +1108,7 @@ TC_DEV void e2e_node(int rank, const int *selfid, const int2 *se, const int *lev
         nd.level = l;
         nd.cx = cc.x;
         nd.cy = cc.y;
-        nd.pad0 = nd.pad1 = 0;
+        nd.rr = rad * rad;
         nodes[id] = nd;
         nodew[id] = make_double4(wsum, wxsum, wysum, 0.0);
     }

@@ -38,7 +38,8 @@ inline bool tc_order_supported(int p)
 struct alignas(16) TcNode {
     double xcom, ycom, r, mass;
     int s, e, child, level;  // child: first child id, 0 for a leaf
-    int cx, cy, pad0, pad1;  // integer cell coordinates at `level`
+    int cx, cy;              // integer cell coordinates at `level`
+    double rr;               // r * r (FP64, rounded as the product is): the opening test's left side
 };
 static_assert(sizeof(TcNode) == 64, "TcNode must stay 64 B (one s_load_dwordx16)");
 
@@ -85,6 +86,19 @@ TC_DEV int tc_lane() { return __lane_id(); }
 
 // Shader clock for the per-category cycle counters (one scalar instruction).
 TC_DEV unsigned long long tc_clock() { return __builtin_readcyclecounter(); }
+
+// Workgroups go to the 8 XCDs (each with its own L2) round-robin: workgroup i
+// to XCD i mod 8.  This renumbers them so that each XCD takes C consecutive
+// logical blocks at a time, chunks rotating over the XCDs: neighbouring
+// blocks (nearby targets, which visit the same tree nodes) share an L2, and
+// every XCD still gets work from the whole domain (contiguous eighths would
+// put the dense regions on a few XCDs).  The tail beyond whole rounds of 8 C
+// keeps its numbering.  A bijection of 0 .. gridDim.x - 1.
+template <int C> TC_DEV int tc_xcd_chunk_block()
+{
+    const int bi = blockIdx.x, full = (int)(gridDim.x / (8 * C)) * (8 * C);
+    return bi < full ? (bi / (8 * C)) * (8 * C) + (bi & 7) * C + ((bi >> 3) % C) : bi;
+}
 
 // Orders LDS traffic between lanes of one wavefront (no instruction, only a
 // code-motion and memory-model fence at wavefront scope).

@@ -278,10 +278,16 @@ struct TcPotReq {
     int m, wave0;
 };
 
+// A DFS stack entry: the node and the lanes that reach it (one 16-byte LDS
+// access per push or pop).
+struct alignas(16) TcStackEntry {
+    unsigned long long mask;
+    int node, pad;
+};
+
 // LDS of one workgroup (four waves) of the potential traversal.
 template <int P, bool F32> struct TcPotShared {
-    int node[4][TC_STACK];
-    unsigned long long mask[4][TC_STACK];
+    TcStackEntry stk[4][TC_STACK];
     double4 src[4][64];
     // FP32 wave-level local expansions: beta[l P + k - 1] = (-1)^k C(l + k - 1, l),
     // queued node ids, their z0 = com - centre, and each wave's {centre, radius, 1 / theta'}
@@ -314,8 +320,7 @@ TC_DEV void potential_wave(TcPotShared<P, F32> &sh, const TcNode *__restrict__ n
                            const uint32_t *__restrict__ tperm, double *__restrict__ out, const double th2,
                            const int m, int k0, unsigned long long *counters)
 {
-    int(&s_node)[4][TC_STACK] = sh.node;
-    unsigned long long(&s_mask)[4][TC_STACK] = sh.mask;
+    TcStackEntry(&s_stk)[4][TC_STACK] = sh.stk;
     double4(&s_src)[4][64] = sh.src;
     const float *s_beta = sh.beta;
     int(&s_q)[4][F32 ? M2L_Q : 1] = sh.q;
@@ -365,24 +370,30 @@ TC_DEV void potential_wave(TcPotShared<P, F32> &sh, const TcNode *__restrict__ n
     if constexpr (CNT)
         cyc0 = tc_clock();
     int sp = 0;
-    s_node[wid][0] = 0;
-    s_mask[wid][0] = live;
+    if (lane == 0)
+        s_stk[wid][0] = TcStackEntry{live, 0, 0};
     sp = 1;
     while (sp > 0) {
         --sp;
         tc_wave_sync();
-        const int id = tc_uniform(s_node[wid][sp]);
-        const unsigned long long mask = tc_uniform64(s_mask[wid][sp]);
+        // the entry in one LDS read, made scalar before the node record's loads
+        // are issued (LDS and scalar loads share a wait counter: a wait for
+        // the LDS read behind the record's loads would split them in two rounds)
+        const uint4 top = *reinterpret_cast<const uint4 *>(&s_stk[wid][sp]);
+        const int id = tc_uniform((int)top.z);
+        const unsigned long long mask =
+            ((unsigned long long)(unsigned)tc_uniform((int)top.y) << 32) | (unsigned)tc_uniform((int)top.x);
         // no early exit for empty nodes: all fields of the record are needed
         // before any branch, so they arrive in one round of scalar loads (an
         // empty node leaves no lane accepting or opening it)
         const TcNode nd = nodes[id];
         const bool nonempty = nd.s != nd.e;
-        const bool in = ((mask >> lane) & 1ull) && nonempty;
         const double dx = x - nd.xcom, dy = y - nd.ycom;
         const double r2 = dx * dx + dy * dy;
-        const bool acc_ok = in & (nd.r * nd.r < th2 * r2);  // no short circuit: r arrives with the rest
-        const unsigned long long amask = tc_ballot(acc_ok);
+        // the lanes that accept, as a wave mask: the compare's own lane mask
+        // combined on the scalar unit; the lane predicate is that mask itself
+        const unsigned long long amask = tc_ballot(nd.rr < th2 * r2) & mask & (nonempty ? ~0ull : 0ull);
+        const bool acc_ok = __builtin_amdgcn_inverse_ballot_w64(amask);
         if constexpr (CNT)
             tc0 = tc_clock();
         if (acc_ok) {
@@ -408,7 +419,7 @@ TC_DEV void potential_wave(TcPotShared<P, F32> &sh, const TcNode *__restrict__ n
         if (nd.child == 0) {
             if constexpr (CNT)
                 tc0 = tc_clock();
-            const bool act = (omask >> lane) & 1ull;
+            const bool act = __builtin_amdgcn_inverse_ballot_w64(omask);
             const int nact = __popcll(omask), nsrc = nd.e - nd.s, nchunk = (nsrc + 63) >> 6;
             if constexpr (CNT)
                 np2p += (unsigned long long)nact * (unsigned long long)nsrc;
@@ -586,8 +597,7 @@ TC_DEV void potential_wave(TcPotShared<P, F32> &sh, const TcNode *__restrict__ n
                     }
                     if (push) {
                         const int pos = sp + __popcll(pm & tc_lanemask_lt());
-                        s_node[wid][pos] = nd.child + lane;
-                        s_mask[wid][pos] = omask;
+                        s_stk[wid][pos] = TcStackEntry{omask, nd.child + lane, 0};
                     }
                     sp += __popcll(pm);
                     const int nadd = __popcll(mm);
@@ -601,10 +611,8 @@ TC_DEV void potential_wave(TcPotShared<P, F32> &sh, const TcNode *__restrict__ n
                 }
             }
             // children pushed in order 0..3, popped 3..0 (treecode-potential.cpp:76-79)
-            if (lane < 4) {
-                s_node[wid][sp + lane] = nd.child + lane;
-                s_mask[wid][sp + lane] = omask;
-            }
+            if (lane < 4)
+                s_stk[wid][sp + lane] = TcStackEntry{omask, nd.child + lane, 0};
             sp += 4;
         }
     }
@@ -703,7 +711,7 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
     __shared__ TcPotShared<P, F32> sh;
     potential_shared_init<P, F32>(sh);
     potential_wave<P, F32, CNT>(sh, nodes, exps, exps32, src, src32, xt, yt, tperm, out, th2, m,
-                                (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64, counters);
+                                (tc_xcd_chunk_block<16>() * 4 + (threadIdx.x >> 6)) * 64, counters);
 }
 
 // Cross-request traversal: one launch evaluates the waves of all meta[0]
