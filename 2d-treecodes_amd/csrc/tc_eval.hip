@@ -761,12 +761,12 @@ __global__ __launch_bounds__(256) void k_potential_batch(const TcPotReq *__restr
 // ------------------------------------------------------------------ force kernels
 // Multipole-to-local (downward_e2l, force-kernels.m4:102-155) for up to
 // E2L_NQ nodes at once, from coefficients and node records staged in LDS: the
-// wave splits into groups of E2L_G lanes (16 for p <= 16, else 32), group g
-// takes node g and lane l of a group owns local coefficient l+1.
+// wave splits into groups of E2L_G lanes (8 for p <= 8, 16 for p <= 16, else
+// 32), group g takes node g and lane l of a group owns local coefficient l+1.
 // z0 = com - brick centre, u = 1/z0; each lane forms u^(l+1) by binary
 // powering, publishes (-1)^k a_k u^k, then sums C(l+k, k-1) (-1)^k a_k u^k
 // over k and multiplies by u^(l+1).
-template <int P> constexpr int E2L_G = P <= 16 ? 16 : 32;
+template <int P> constexpr int E2L_G = P <= 8 ? 8 : (P <= 16 ? 16 : 32);
 template <int P> constexpr int E2L_NQ = 64 / E2L_G<P>;
 
 // rec[g] = {xcom, ycom, r, mass}; co[g * 2P + j] = Re a_(j+1), co[g * 2P + P + j] = Im a_(j+1)
@@ -788,9 +788,9 @@ TC_DEV void e2l_group(const double4 *rec, const double *co, int nq, double x0b, 
         ur = zx * inv;
         ui = -zy * inv;
         double br = ur, bi = ui;
-        int e = l + 1;  // <= G: 5 bits for G = 16, 6 for G = 32
+        int e = l + 1;  // <= G: 4 bits for G = 8, 5 for G = 16, 6 for G = 32
 #pragma unroll
-        for (int bit = 0; bit < (G == 16 ? 5 : 6); ++bit) {
+        for (int bit = 0; bit < (G == 8 ? 4 : (G == 16 ? 5 : 6)); ++bit) {
             if (e & 1) {
                 const double tr = pr * br - pi * bi;
                 pi = pr * bi + pi * br;
@@ -1266,7 +1266,7 @@ TC_DEV void force_skip_p2p(const int *list, int n, int &skip, const TcNode *__re
 // repeat the same traversal and add all their P2P sums.
 // CNT: the instance launched with counters, which also times the categories
 template <int P, bool F32, bool NEAR, bool CNT>
-__global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
+__global__ __launch_bounds__(256, 4) void k_force(const TcNode *__restrict__ nodes, const double *__restrict__ exps,
                                                const float2 *__restrict__ exps32, const double4 *__restrict__ src,
                                                const float4 *__restrict__ src32, const double *__restrict__ x0s,
                                                const double *__restrict__ y0s, const double *__restrict__ hs,
@@ -1278,7 +1278,7 @@ __global__ __launch_bounds__(256, F32 ? 4 : 5) void k_force(const TcNode *__rest
     __shared__ int s_stack[4][FS_CAP];
     __shared__ int s_list[4][3][FQ_CAP];  // E2L, E2P, P2P node ids
     __shared__ double4 s_buf[4][FBUF_BYTES / 32];
-    __shared__ double2 s_coef[4][68];  // E2L_NQ groups of E2L_G + 1 entries; L2P uses [1..P]
+    __shared__ double2 s_coef[4][72];  // E2L_NQ groups of E2L_G + 1 entries (at most 8 x 9); L2P uses [1..P]
     __shared__ double s_binom[P * P];  // [k - 1][L - 1] = C(L + k - 1, k - 1)
     __shared__ double s_ninv[TC_MAXORDER + 1];  // -1/L, nearest double
 
