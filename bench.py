@@ -292,18 +292,21 @@ def tune_serving(cache_path):
             cells = [dict(a, ms=max(a["ms"], b["ms"]), err=max(a["err"], b["err"]),
                           ok=a["err"] <= targets[0] and b["err"] <= targets[1]) for a, b in zip(*tables)]
             base = [c for c in cells if c["order"] == p0 and c["leaf"] == l0][0]
+            # the plain FP64 error at the reference's parameters, for comparison
+            ref64 = max(tuner.tune(kind, d, theta=theta, precision=0, err_target=1.0, reps=1, save=False,
+                                   orders=(p0,), leaves=(l0,))["err"] for d in datas)
             target = max(targets)
             best = min((c for c in cells if c["ok"]), key=lambda c: c["ms"])
             tuner.cache[autotune.bucket_key(kind, 1, theta, n, m)] = {
                 "order": best["order"], "leaf": best["leaf"], "ms": best["ms"], "err": best["err"],
                 "err_target": target}
             tuner._save()
-            log("  %s N=%d blocks=%d theta=%.1f: order %d leaf %d (%.3f ms, err %.1e; reference %d/%d: %.3f ms, "
-                "err %.1e)" % (kind, n, nb, theta, best["order"], best["leaf"], best["ms"], best["err"], p0, l0,
-                               base["ms"], base["err"]))
+            log("  %s N=%d blocks=%d theta=%.1f: order %d leaf %d (%.3f ms, err %.1e; FP32 at the reference's %d/%d: "
+                "%.3f ms, err %.1e; FP64 there: err %.1e)" % (kind, n, nb, theta, best["order"], best["leaf"], best["ms"],
+                                                              best["err"], p0, l0, base["ms"], base["err"], ref64))
             done.append({"kind": kind, "n": n, "blocks": nb, "theta": theta, "order": best["order"],
                          "leaf": best["leaf"], "ms": best["ms"], "err": best["err"], "ref_ms": base["ms"],
-                         "ref_err": base["err"]})
+                         "ref_err": base["err"], "ref_err_fp64": ref64})
     return done
 
 
@@ -376,8 +379,9 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
     srv.close()
     # accuracy of served results: every 40th request against the FP64 direct sum
     # on 256 of its targets (GPU direct-sum kernels)
-    errs = []
+    errs, errs64 = [], []
     ectx = pkg.Context()
+    rctx = pkg.Context()  # plain FP64 at the reference's order / leaf, on the same sampled targets
     rng = np.random.default_rng(5)
     dev = ectx.to_device
     for i in sorted(kept):
@@ -388,6 +392,8 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
             idx = np.sort(rng.choice(len(tg[0]), min(256, len(tg[0])), replace=False)).astype(np.int32)
             ref = ectx.direct_potential(*src, dev(tg[0]), dev(tg[1]), dev(idx, np.int32))
             errs.append(float(np.linalg.norm(res[idx] - ref) / np.linalg.norm(ref)))
+            r64 = rctx.potential(r["theta"], xs, ys, vs, *tg)
+            errs64.append(float(np.linalg.norm(r64[idx] - ref) / np.linalg.norm(ref)))
         else:
             m = len(tg[0]) * 1024
             idx = np.sort(rng.choice(m, min(256, m), replace=False)).astype(np.int32)
@@ -397,7 +403,11 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
             rx, ry = ectx.direct_force(*src, dev(tx), dev(ty), dev(np.arange(len(idx), dtype=np.int32), np.int32))
             num = np.sum((res[0][idx] - rx) ** 2) + np.sum((res[1][idx] - ry) ** 2)
             errs.append(float(np.sqrt(num / (np.sum(rx ** 2) + np.sum(ry ** 2)))))
+            f64 = rctx.force(r["theta"], xs, ys, vs, *tg)
+            num = np.sum((f64[0][idx] - rx) ** 2) + np.sum((f64[1][idx] - ry) ** 2)
+            errs64.append(float(np.sqrt(num / (np.sum(rx ** 2) + np.sum(ry ** 2)))))
     ectx.close()
+    rctx.close()
     lat = np.array([x["latency_ms"] for x in info])
     svc = np.array([x["service_ms"] for x in info])
     targets = sum(len(p[4][0]) * (1 if p[0]["kind"] == "potential" else 1024) for p in prepared)
@@ -412,6 +422,10 @@ def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned
             "budget_gib": budget_gib, "peak_admitted_gib": st["peak_bytes"] / 2 ** 30,
             "evicted_trees": int(st["evicted"]), "trimmed_slots": int(st["trimmed"]),
             "rel_err_sampled": {"requests": len(errs), "max": max(errs), "median": float(np.median(errs))},
+            "rel_err_sampled_fp64_reference_params": {"requests": len(errs64), "max": max(errs64),
+                                                      "median": float(np.median(errs64)),
+                                                      "note": "the same sampled requests solved in plain FP64 at the "
+                                                              "reference's order / leaf (12 / 512, 24 / 192)"},
             "params": ("order / leaf of the nearest autotuned size bucket (2d-treecodes_amd/tuned.json) for %d of %d "
                        "requests, the reference's for the rest" % (ntuned, nreq)) if tuner else
                       "the reference's order / leaf (12 / 512 potential, 24 / 192 force)"}
