@@ -516,6 +516,7 @@ def measure(cfg, theta, args, cpu_budget, host=False, n=0):
         log("host-array C ABI timing")
         hdt = cases[head].host_timed(data)
         hostr = {"ms": hdt * 1e3, "mtargets_per_s": m / hdt / 1e6, "dtype": head,
+                 "headline_form": "end-to-end: the metric as a drop-in caller sees it",
                  "path": "end-to-end C ABI with host arrays in and out (pageable memory): H2D copies, build, "
                          "solve, D2H copy; PCIe-inclusive (value is the HBM-resident rate)"}
         log("  %.3f ms -> %.1f Mtargets/s" % (hdt * 1e3, hostr["mtargets_per_s"]))
