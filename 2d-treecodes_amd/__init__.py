@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtreecodes2d.so")
 
 (OPT_PRECISION, OPT_POT_ORDER, OPT_POT_LEAF, OPT_FORCE_ORDER, OPT_FORCE_LEAF, OPT_PROFILE, OPT_COUNTERS,
- OPT_FORCE_CHUNKS) = range(8)
+ OPT_FORCE_CHUNKS, OPT_POT_XCD_CHUNK, OPT_FORCE_INLINE) = range(10)
 PH_BUILD, PH_TSORT, PH_EVAL, PH_TOTAL = range(4)
 CNT_P2P, CNT_E2P, CNT_E2L, CNT_L2P = range(4)
 FP64, FP32 = 0, 1

@@ -1,4 +1,4 @@
-"""Autotuner for leaf capacity and expansion order.
+"""Autotuner for leaf capacity, expansion order and tiling.
 
 For one workload (sources, targets or blocks, theta, precision) it measures,
 on the GPU, every (leaf capacity, order) pair of a grid: the whole-solve time
@@ -8,7 +8,9 @@ error is within the target (default: no worse than the reference defaults,
 order 12 / leaf 512 for the potential and 24 / 192 for the force, and at
 most 1e-6), and caches the choice per (kind, precision, theta, log2 N,
 log2 targets) bucket in a JSON file so a server can look it up per request
-size.
+size.  With `tilings`, it then times the chosen pair under each tiling value
+(TILING: the potential traversal's workgroups per XCD chunk, or the force
+traversal's inline P2P leaves per brick) and records the fastest as "tile".
 """
 import json
 import math
@@ -20,6 +22,8 @@ import numpy as np
 DEFAULT = {"potential": (12, 512), "force": (24, 192)}
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 ORDERS = (6, 8, 9, 10, 12, 14, 16, 20, 24, 32)
+# tiling option per kind and the values a sweep tries (0 is the kernel's default)
+TILING = {"potential": ("OPT_POT_XCD_CHUNK", (1, 4, 16, 64)), "force": ("OPT_FORCE_INLINE", (16, 32, 64, 128))}
 LEAVES = (64, 128, 192, 256, 384, 512, 768, 1024)
 
 
@@ -42,6 +46,13 @@ class Tuner:
         """Tuned (order, leaf) for this size bucket, or None."""
         e = self.cache.get(bucket_key(kind, precision, theta, n, m))
         return (e["order"], e["leaf"]) if e else None
+
+    def lookup_tile(self, kind, precision, theta, n, m):
+        """(option key, value) of the tuned tiling for this size bucket, or None."""
+        e = self.cache.get(bucket_key(kind, precision, theta, n, m))
+        if not e or not e.get("tile"):
+            return None
+        return getattr(self.pkg, TILING[kind][0]), e["tile"]
 
     def nearest(self, kind, precision, theta, n, m, max_dist=3):
         """(order, leaf) of the tuned bucket of the same kind, precision and theta
@@ -67,9 +78,10 @@ class Tuner:
             os.replace(tmp, self.cache_path)
 
     def tune(self, kind, data, theta=0.5, precision=1, orders=ORDERS, leaves=LEAVES, err_target=None,
-             nsample=1024, reps=3, seed=0, log=None, save=True):
+             nsample=1024, reps=3, seed=0, log=None, save=True, tilings=None):
         """data: potential {xs, ys, vs, xt, yt}; force {xs, ys, vs, x0s, y0s, hs}.
-        Returns {"order", "leaf", "ms", "err", "table": [...]} and records it in the cache."""
+        Returns {"order", "leaf", "ms", "err", "table": [...]} (with tilings also "tile" and
+        "tile_table") and records it in the cache."""
         pkg = self.pkg
         ctx = pkg.Context(device=self.device, precision=precision)
         d = {k: ctx.to_device(v) for k, v in data.items()}
@@ -137,10 +149,34 @@ class Tuner:
         best = min(ok, key=lambda t: t["ms"])
         res = {"order": best["order"], "leaf": best["leaf"], "ms": best["ms"], "err": best["err"],
                "err_target": err_target, "met_target": met, "table": table}
+        if tilings:
+            # the chosen pair under each tiling value; the tiling moves results only
+            # by rounding (summation order) or not at all, so the error is rechecked
+            # but the choice is by time
+            tkey = getattr(pkg, TILING[kind][0])
+            ctx.set(okey, best["order"])
+            ctx.set(lkey, best["leaf"])
+            ttable = []
+            for tv in tilings:
+                ctx.set(tkey, tv)
+                run()
+                err = error()
+                ts = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    run()
+                    ts.append(time.perf_counter() - t0)
+                ttable.append({"tile": tv, "ms": 1e3 * min(ts), "err": err})
+                if log:
+                    log("tune %s tile %s=%d: %.3f ms err %.2e" % (kind, TILING[kind][0], tv, ttable[-1]["ms"], err))
+            ctx.set(tkey, 0)
+            okt = [t for t in ttable if t["err"] <= max(err_target, best["err"] * 1.01)] or ttable
+            bt = min(okt, key=lambda t: t["ms"])
+            res.update(tile=bt["tile"], ms=bt["ms"], err=bt["err"], tile_table=ttable)
         n = len(data["xs"])
         if save:
-            self.cache[bucket_key(kind, precision, theta, n, m)] = {k: res[k] for k in ("order", "leaf", "ms", "err",
-                                                                                         "err_target", "met_target")}
+            self.cache[bucket_key(kind, precision, theta, n, m)] = {
+                k: res[k] for k in ("order", "leaf", "ms", "err", "err_target", "met_target", "tile") if k in res}
             self._save()
         ctx.close()
         return res

@@ -35,6 +35,7 @@ struct tc_context {
     int device = 0;
     hipStream_t stream = nullptr;
     int precision = 0, pot_order = 12, pot_leaf = 32 * 16, force_order = 24, force_leaf = 192, force_chunks = 0;
+    int pot_xcd_log2 = TC_XCD_CHUNK_LOG2, force_inline = 0;  // tiling options (0: the kernel's default)
     bool profile = false, count = false;
     int tree_kind = TC_KIND_NONE;
     int tree_precision = 0;
@@ -363,7 +364,8 @@ void eval_potential(tc_context *c, double theta, const double *xt, const double 
                     unsigned long long *cnt, bool sort_targets)
 {
     tc_eval_potential(c->tree, theta, xt, yt, m, out, c->precision, cnt, c->tws, c->stream,
-                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets);
+                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets,
+                      c->pot_xcd_log2);
 }
 
 // An external tree's evaluation: the target sort (unless done beside the
@@ -373,7 +375,8 @@ void eval_potential_ext(tc_context *c, const TcTree &t, const ExtTree &ext, doub
 {
     if (!ext.eval) {
         tc_eval_potential(t, theta, xt, yt, m, out, c->precision, cnt, c->tws, c->stream,
-                          c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets);
+                          c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, sort_targets,
+                          c->pot_xcd_log2);
         return;
     }
     if (sort_targets)
@@ -535,7 +538,8 @@ int run_potential_noalloc(tc_context *ctx, double theta, const double *xs, const
             record(c, 1);
             TC_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
             tc_eval_potential(c->na_tree, theta, xt, yt, m, out, c->precision, cnt, c->na_tws, c->stream,
-                              c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, false);
+                              c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr, false,
+                              c->pot_xcd_log2);
             TC_HIP(hipStreamSynchronize(c->stream));
         } catch (...) {
             if (forked)
@@ -610,7 +614,7 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
             build(c, dxs, dys, dvs, n, TC_KIND_FORCE, ext);
         record(c, 1);
         tc_eval_force(ext.tree ? *ext.tree : c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws,
-                      c->force_chunks,
+                      c->force_chunks, c->force_inline,
                       c->stream,
                       c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
         c->ev_valid = c->profile;
@@ -919,6 +923,18 @@ TC_API int tc_set_option(tc_context *c, int key, int value)
     case TC_OPT_FORCE_CHUNKS:
         if (value < 0) break;
         c->force_chunks = value;
+        return 0;
+    case TC_OPT_POT_XCD_CHUNK:
+        if (value == 0) {
+            c->pot_xcd_log2 = TC_XCD_CHUNK_LOG2;
+            return 0;
+        }
+        if (value < 1 || value > 256 || (value & (value - 1)) != 0) break;
+        c->pot_xcd_log2 = __builtin_ctz((unsigned)value);
+        return 0;
+    case TC_OPT_FORCE_INLINE:
+        if (value < 0 || value > 4096) break;
+        c->force_inline = value;
         return 0;
     case TC_OPT_COUNTERS:
         c->count = value != 0;

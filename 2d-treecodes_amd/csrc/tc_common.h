@@ -93,11 +93,13 @@ TC_DEV unsigned long long tc_clock() { return __builtin_readcyclecounter(); }
 // blocks (nearby targets, which visit the same tree nodes) share an L2, and
 // every XCD still gets work from the whole domain (contiguous eighths would
 // put the dense regions on a few XCDs).  The tail beyond whole rounds of 8 C
-// keeps its numbering.  A bijection of 0 .. gridDim.x - 1.
-template <int C> TC_DEV int tc_xcd_chunk_block()
+// keeps its numbering.  A bijection of 0 .. gridDim.x - 1; C = 2^lc (C = 1 is
+// the hardware's own round-robin order).
+constexpr int TC_XCD_CHUNK_LOG2 = 4;  // 16 blocks per chunk (measured best at 2^24 targets)
+TC_DEV int tc_xcd_chunk_block(int lc)
 {
-    const int bi = blockIdx.x, full = (int)(gridDim.x / (8 * C)) * (8 * C);
-    return bi < full ? (bi / (8 * C)) * (8 * C) + (bi & 7) * C + ((bi >> 3) % C) : bi;
+    const int bi = blockIdx.x, span = 8 << lc, full = (int)(gridDim.x >> (3 + lc)) << (3 + lc);
+    return bi < full ? (bi & ~(span - 1)) + ((bi & 7) << lc) + ((bi >> 3) & ((1 << lc) - 1)) : bi;
 }
 
 // Orders LDS traffic between lanes of one wavefront (no instruction, only a
@@ -397,12 +399,18 @@ void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int preci
 // One request's traversal by the one-request kernel (k_potential) on st.
 void tc_potential_item(const TcPotBatchItem &it, int precision, unsigned long long *counters, hipStream_t st);
 
+// xcd_chunk_log2: the traversal's workgroups go to the XCDs in chunks of
+// 2^xcd_chunk_log2 consecutive blocks (a tiling option; results do not depend on it).
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
-                       hipEvent_t ev0, hipEvent_t ev1, bool sort_targets = true);
+                       hipEvent_t ev0, hipEvent_t ev1, bool sort_targets = true,
+                       int xcd_chunk_log2 = TC_XCD_CHUNK_LOG2);
+// inline_leaves: direct-sum leaves a brick sums inside the traversal before the
+// rest go to chunks summed by other waves (a tiling option; results differ only
+// in summation order).
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
-                   TcScratch &ws, int max_chunks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+                   TcScratch &ws, int max_chunks, int inline_leaves, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 void tc_eval_force_targets(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *xr,
                            double *yr, int precision, unsigned long long *counters, TcScratch &ws,
                            hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);

@@ -706,12 +706,12 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                                                    const float4 *__restrict__ src32, const double *__restrict__ xt,
                                                    const double *__restrict__ yt, const uint32_t *__restrict__ tperm,
                                                    int m, double th2, double *__restrict__ out,
-                                                   unsigned long long *counters)
+                                                   unsigned long long *counters, int xcd_lc)
 {
     __shared__ TcPotShared<P, F32> sh;
     potential_shared_init<P, F32>(sh);
     potential_wave<P, F32, CNT>(sh, nodes, exps, exps32, src, src32, xt, yt, tperm, out, th2, m,
-                                (tc_xcd_chunk_block<16>() * 4 + (threadIdx.x >> 6)) * 64, counters);
+                                (tc_xcd_chunk_block(xcd_lc) * 4 + (threadIdx.x >> 6)) * 64, counters);
 }
 
 // Cross-request traversal: one launch evaluates the waves of all meta[0]
@@ -909,7 +909,7 @@ template <bool F32> constexpr int P2P_TILE_N = F32 ? 64 : 32;
 // leave one wave running long after the rest.  A brick whose chunks do not fit
 // in the pool is marked in pmark and listed in ovf; the NEAR instantiation of
 // k_force then repeats its traversal and sums all its P2P leaves.
-constexpr int FP2P_INLINE = 64;
+constexpr int FP2P_INLINE = 64;  // default of the inline count (TcNearLists::inl)
 enum { FN_OVF = 0, FN_CHUNKS = 1, FN_OBRICKS = 2, FN_NCTR = 4 };
 struct TcNearLists {
     uint32_t *pmark;    // [nb] 1: the brick is listed in ovf
@@ -921,6 +921,7 @@ struct TcNearLists {
     double2 *cpart;     // [maxc * 64] chunk sums per target
     uint32_t *ctr;      // [FN_NCTR] counters
     int maxc;
+    int inl;            // P2P leaves a brick sums in the traversal before chunks take the rest
 };
 
 typedef __attribute__((address_space(3))) void tc_lds_void;
@@ -1211,7 +1212,7 @@ TC_DEV void force_near_p2p(const int *list, int n, const TcNearLists &nl, int wa
     if (ovfl)
         return;
     const int lane = tc_lane();
-    const int take = min(n, FP2P_INLINE - ninl);
+    const int take = min(n, nl.inl - ninl);
     if (take > 0)
         force_flush_p2p<F32>(list, take, nodes, src, src32, buf, xt, yt, fx, fy, np2p);
     ninl += take;
@@ -1318,7 +1319,7 @@ __global__ __launch_bounds__(256, 4) void k_force(const TcNode *__restrict__ nod
         bool ovfl = false;
         // NEAR: the far-field pass summed the brick's first FP2P_INLINE P2P
         // leaves; the repeated traversal lists the leaves in the same order
-        int skip = FP2P_INLINE;
+        int skip = nlist.inl;
         // targets as in force_p2p_8x8 (force-kernels.m4:28-32); the brick-wide
         // values are moved to scalar registers
         const double xb = tc_uniform(x0 + (bx + 0) * h), yb = tc_uniform(y0 + (by + 0) * h);
@@ -1783,13 +1784,14 @@ TcPotReq potential_req(const TcTree &t, double th2, const double *xt, const doub
 
 template <int P>
 void launch_potential(const TcTree &t, double th2, const double *xt, const double *yt, const uint32_t *tperm,
-                      int m, double *out, int precision, unsigned long long *counters, hipStream_t st)
+                      int m, double *out, int precision, unsigned long long *counters, hipStream_t st,
+                      int xcd_lc = TC_XCD_CHUNK_LOG2)
 {
     const int grid = tiles_of(m, 256);
     const TcPotReq q = potential_req(t, th2, xt, yt, tperm, m, out, precision, 0);
     auto go = [&](auto kern) {
         kern<<<grid, 256, 0, st>>>(q.nodes, q.exps, q.exps32, q.src, q.src32, q.xt, q.yt, q.tperm, q.m, q.th2, q.out,
-                                   counters);
+                                   counters, xcd_lc);
     };
     if (precision == 1 && counters)
         go(k_potential<P, true, true>);
@@ -1876,7 +1878,7 @@ void launch_force_pass(const TcTree &t, double theta, const double *x0s, const d
 template <int P>
 void launch_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                   int nblocks, double *xr, double *yr, int precision, unsigned long long *counters, TcScratch &ws,
-                  int max_chunks, hipStream_t st)
+                  int max_chunks, int inline_leaves, hipStream_t st)
 {
     if (nblocks > (1 << 26))
         throw TcError(-2, "tc_eval_force: too many blocks");
@@ -1908,6 +1910,7 @@ void launch_force(const TcTree &t, double theta, const double *x0s, const double
     u += chunk;
     nl.ctr = u;
     nl.maxc = maxc;
+    nl.inl = inline_leaves > 0 ? inline_leaves : FP2P_INLINE;
     uint32_t *work = ws.work.as<uint32_t>();
     for (int first = 0; first < nbricks; first += chunk) {
         const int nb = std::min(chunk, nbricks - first);
@@ -2136,12 +2139,14 @@ void tc_potential_item(const TcPotBatchItem &it, int precision, unsigned long lo
 
 void tc_eval_potential(const TcTree &t, double theta, const double *xt, const double *yt, int m, double *out,
                        int precision, unsigned long long *counters, TcScratch &ws, hipStream_t st,
-                       hipEvent_t ev0, hipEvent_t ev1, bool sort_targets)
+                       hipEvent_t ev0, hipEvent_t ev1, bool sort_targets, int xcd_chunk_log2)
 {
     if (m <= 0)
         return;
     if (precision == 1 && !t.has_fp32)
         throw TcError(-3, "tc_eval_potential: tree was built without FP32 mirrors");
+    if (xcd_chunk_log2 < 0 || xcd_chunk_log2 > 8)
+        throw TcError(-3, "tc_eval_potential: XCD chunk of 1 .. 256 workgroups");
     if (sort_targets)
         tc_sort_targets(xt, yt, m, ws, st);
     const double th2 = theta * theta;
@@ -2150,7 +2155,7 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
         TC_HIP(hipEventRecord(ev0, st));
     switch (t.order) {
 #define TC_CASE(P) \
-    case P: launch_potential<P>(t, th2, xt, yt, tperm, m, out, precision, counters, st); break;
+    case P: launch_potential<P>(t, th2, xt, yt, tperm, m, out, precision, counters, st, xcd_chunk_log2); break;
         TC_ORDERS(TC_CASE)
 #undef TC_CASE
     default: throw TcError(-3, "tc_eval_potential: unsupported order");
@@ -2161,7 +2166,7 @@ void tc_eval_potential(const TcTree &t, double theta, const double *xt, const do
 
 void tc_eval_force(const TcTree &t, double theta, const double *x0s, const double *y0s, const double *hs,
                    int nblocks, double *xr, double *yr, int precision, unsigned long long *counters,
-                   TcScratch &ws, int max_chunks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+                   TcScratch &ws, int max_chunks, int inline_leaves, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (nblocks <= 0)
         return;
@@ -2172,7 +2177,8 @@ void tc_eval_force(const TcTree &t, double theta, const double *x0s, const doubl
     switch (t.order) {
 #define TC_CASE(P)                                                                                         \
     case P:                                                                                                \
-        launch_force<P>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, ws, max_chunks, st); \
+        launch_force<P>(t, theta, x0s, y0s, hs, nblocks, xr, yr, precision, counters, ws, max_chunks, \
+                        inline_leaves, st);                                                                \
         break;
         TC_ORDERS(TC_CASE)
 #undef TC_CASE

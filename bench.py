@@ -3,8 +3,9 @@
 
 Default workload (the precision form of treecode_potential): N = M = 2^24,
 half uniform in [0,1]^2 and half Plummer (a = 0.02) about (0.5, 0.5)
-truncated to [0,1]^2, values uniform in [-1, 1), theta = 0.5, order 12,
-leaf capacity 512.  One step = one whole solve (bbox, Morton sort, tree
+truncated to [0,1]^2, values uniform in [-1, 1), theta = 0.5, each precision
+at the order, leaf capacity and tiling autotuned for it
+(2d-treecodes_amd/tuned.json).  One step = one whole solve (bbox, Morton sort, tree
 build, upward pass, target sort, traversal) with inputs already resident
 in HBM.  `value` is the FP32-bulk rate in Mtargets/s; the plain FP64 GPU
 path is timed beside it, and both are checked against the FP64 direct sum
@@ -12,7 +13,9 @@ on 4096 random targets (GPU direct-sum kernel).
 
 Other workloads (--config): c1 (potential, 65,536 uniform), c2 (force_mrag,
 1,048,576 sources in 4 Gaussians, 4,096 blocks), c5 (force_mrag, 8,388,608
-sources in 16 Gaussians, 16,384 blocks of mixed spacing).
+sources in 16 Gaussians, 16,384 blocks of mixed spacing).  Unless --no-legs,
+the default run also times C3 at theta 0.3 and 0.7, C5, C2 and C1 (`legs`)
+and replays the served stream (`serving`).
 
 The JSON line also carries `roofline` (the traversal kernel's algorithmic
 FLOP/s from deterministic interaction counts over its HIP-event duration)
@@ -95,7 +98,7 @@ def make_inputs(cfg, n):
 
 
 class GpuCase:
-    def __init__(self, kind, data, precision, theta, order, leaf):
+    def __init__(self, kind, data, precision, theta, order, leaf, tile=None):
         self.kind, self.theta = kind, theta
         kw = dict(precision=precision, profile=True)
         if kind == "potential":
@@ -103,6 +106,8 @@ class GpuCase:
         else:
             kw.update(force_order=order or 24, force_leaf=leaf or 192)
         self.ctx = pkg.Context(**kw)
+        if tile:
+            self.ctx.set(*tile)  # (tiling option, value)
         self.order = order or (12 if kind == "potential" else 24)
         self.leaf = leaf or (512 if kind == "potential" else 192)
         self.d = {k: self.ctx.to_device(v) for k, v in data.items()}
@@ -449,28 +454,48 @@ def measure(cfg, theta, args, cpu_budget, host=False, n=0):
     kind, data, desc = make_inputs(cfg, n)
     cases, timings, tuned = {}, {}, {}
     for prec in (["f32", "f64"] if args.only is None else [args.only]):
-        order, leaf = args.order, args.leaf
+        order, leaf, tile = args.order, args.leaf, None
         from treecodes2d_amd import autotune
         if not (args.tune or args.reference_params or order or leaf):
             m_t = len(data["xt"]) if kind == "potential" else len(data["x0s"]) * 1024
-            hit = autotune.Tuner(pkg).lookup(kind, 1 if prec == "f32" else 0, theta, len(data["xs"]), m_t)
+            tuner = autotune.Tuner(pkg)
+            hit = tuner.lookup(kind, 1 if prec == "f32" else 0, theta, len(data["xs"]), m_t)
             if hit:
                 order, leaf = hit
+                tile = tuner.lookup_tile(kind, 1 if prec == "f32" else 0, theta, len(data["xs"]), m_t)
                 tuned[prec] = {"order": order, "leaf": leaf, "source": "2d-treecodes_amd/tuned.json"}
-                log("  tuned parameters for %s: order %d leaf %d" % (prec, order, leaf))
+                if tile:
+                    tuned[prec]["tile"] = {autotune.TILING[kind][0]: tile[1]}
+                log("  tuned parameters for %s: order %d leaf %d%s" % (
+                    prec, order, leaf, " tile %s=%d" % (autotune.TILING[kind][0], tile[1]) if tile else ""))
+        if args.tune_tiling and order and leaf:
+            # the tiling sweep alone, at this precision's tuned (or given) order and leaf
+            log("tiling sweep", prec)
+            tt = autotune.Tuner(pkg, cache_path=args.tune_cache or autotune.CACHE)
+            res = tt.tune(kind, data, theta=theta, precision=1 if prec == "f32" else 0, orders=(order,),
+                          leaves=(leaf,), err_target=args.err_target, reps=3, tilings=autotune.TILING[kind][1])
+            tile = (getattr(pkg, autotune.TILING[kind][0]), res["tile"])
+            tuned[prec] = {"order": order, "leaf": leaf, "tile": {autotune.TILING[kind][0]: res["tile"]},
+                           "tile_table": [(t["tile"], round(t["ms"], 3), t["err"]) for t in res["tile_table"]]}
+            log("  %s: %s" % (autotune.TILING[kind][0], ", ".join("%d: %.3f ms" % (t["tile"], t["ms"])
+                                                                   for t in res["tile_table"])))
         if args.tune:
             log("autotuning", prec)
             grid = dict(orders=(8, 9, 10, 12, 14, 16, 20, 24), leaves=(64, 128, 192, 256, 384, 512)) \
                 if kind == "potential" else dict(orders=(8, 10, 12, 16), leaves=(32, 64, 128, 192, 384))
             tuner = autotune.Tuner(pkg, cache_path=args.tune_cache or autotune.CACHE)
             res = tuner.tune(kind, data, theta=theta, precision=1 if prec == "f32" else 0,
-                             err_target=args.err_target, reps=2, **grid)
+                             err_target=args.err_target, reps=2, tilings=autotune.TILING[kind][1], **grid)
             order, leaf = res["order"], res["leaf"]
+            tile = (getattr(pkg, autotune.TILING[kind][0]), res["tile"])
             tuned[prec] = {"order": order, "leaf": leaf, "err_sampled": res["err"],
-                           "table": [(t["order"], t["leaf"], round(t["ms"], 3), t["err"]) for t in res["table"]]}
-            log("  chose order %d leaf %d (%.3f ms, err %.2e)" % (order, leaf, res["ms"], res["err"]))
+                           "tile": {autotune.TILING[kind][0]: res["tile"]},
+                           "table": [(t["order"], t["leaf"], round(t["ms"], 3), t["err"]) for t in res["table"]],
+                           "tile_table": [(t["tile"], round(t["ms"], 3), t["err"]) for t in res["tile_table"]]}
+            log("  chose order %d leaf %d %s=%d (%.3f ms, err %.2e)" % (order, leaf, autotune.TILING[kind][0],
+                                                                       res["tile"], res["ms"], res["err"]))
         log("timing", prec)
-        c = GpuCase(kind, data, 1 if prec == "f32" else 0, theta, order, leaf)
+        c = GpuCase(kind, data, 1 if prec == "f32" else 0, theta, order, leaf, tile)
         dt, phases = c.timed(args.steps, args.warmup)
         cases[prec] = c
         timings[prec] = (dt, phases)
@@ -585,6 +610,8 @@ def main():
     ap.add_argument("--err-target", type=float, default=5e-7, help="autotuner error target (default 5e-7)")
     ap.add_argument("--tune-cache", default="", metavar="PATH",
                     help="autotuner cache to write (default 2d-treecodes_amd/tuned.json)")
+    ap.add_argument("--tune-tiling", action="store_true",
+                    help="sweep the tiling option at the tuned order / leaf of each precision (see autotune.TILING)")
     ap.add_argument("--tune-serving", action="store_true",
                     help="autotune the served stream's size buckets and exit")
     ap.add_argument("--serve-reference-params", action="store_true",

@@ -96,10 +96,17 @@ enum {
     TC_OPT_FORCE_LEAF = 4,     /* force leaf capacity, default 192 */
     TC_OPT_PROFILE = 5,        /* 1 = record phase times with HIP events */
     TC_OPT_COUNTERS = 6,       /* 1 = count interactions (see tc_get_stats) */
-    TC_OPT_FORCE_CHUNKS = 7    /* force near-field pool, in chunks of 64 leaves; 0 = automatic
-                                * (max(4096, bricks / 4)).  A brick with more than 64 P2P leaves
-                                * spreads the rest over chunks; bricks that find the pool full
-                                * repeat their traversal on a slower path (diagnostic setting) */
+    TC_OPT_FORCE_CHUNKS = 7,   /* force near-field pool, in chunks of 64 leaves; 0 = automatic
+                                * (max(4096, bricks / 4)).  A brick with more P2P leaves than
+                                * TC_OPT_FORCE_INLINE spreads the rest over chunks; bricks that find
+                                * the pool full repeat their traversal on a slower path */
+    /* tiling (autotuned with order and leaf capacity; see autotune.py): */
+    TC_OPT_POT_XCD_CHUNK = 8,  /* potential traversal: workgroups per XCD chunk, a power of two in
+                                * 1..256 (1 = the hardware's round-robin order); 0 = 16.  Results do
+                                * not depend on it */
+    TC_OPT_FORCE_INLINE = 9    /* force traversal: P2P leaves a brick sums itself before further
+                                * ones go to chunks summed by other waves, 1..4096; 0 = 64.  Changes
+                                * only the order of summation */
 };
 
 /* Phase times (tc_get_stats times_ms[]) */

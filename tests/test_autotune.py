@@ -63,3 +63,30 @@ def test_tuner_selection_rule(pkg, wl, tmp_path):
     res = tuner.tune("potential", data, theta=0.5, precision=1, orders=(8, 12), leaves=(128, 512),
                      err_target=1e-30, reps=1, save=False)
     assert not res["met_target"] and (res["order"], res["leaf"]) == (12, 512)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["potential", "force"])
+def test_tuner_tiling_sweep(pkg, wl, tmp_path, kind):
+    # with tilings, the chosen (order, leaf) is timed under each tiling value; the
+    # fastest is recorded, looked up, and the chosen cell's error is unchanged
+    from treecodes2d_amd import autotune
+    tuner = autotune.Tuner(pkg, cache_path=str(tmp_path / "t.json"))
+    if kind == "potential":
+        xs, ys, vs, xt, yt = wl.uniform_potential(1 << 15, seed=34)
+        data, m = dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt), 1 << 15
+        grid = dict(orders=(10, 12), leaves=(128,))
+    else:
+        xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(1 << 15)
+        data, m = dict(xs=xs, ys=ys, vs=vs, x0s=x0s[:64], y0s=y0s[:64], hs=hs[:64]), 64 * 1024
+        grid = dict(orders=(8, 12), leaves=(32,))
+    values = autotune.TILING[kind][1]
+    res = tuner.tune(kind, data, theta=0.5, precision=1, err_target=1.0, reps=1, tilings=values, **grid)
+    assert [t["tile"] for t in res["tile_table"]] == list(values)
+    assert res["tile"] in values
+    assert res["ms"] == min(t["ms"] for t in res["tile_table"])
+    chosen = [t for t in res["table"] if (t["order"], t["leaf"]) == (res["order"], res["leaf"])][0]
+    for t in res["tile_table"]:
+        assert abs(t["err"] - chosen["err"]) <= 1e-3 * chosen["err"] + 1e-12
+    key, val = tuner.lookup_tile(kind, 1, 0.5, len(xs), m)
+    assert key == getattr(pkg, autotune.TILING[kind][0]) and val == res["tile"]

@@ -171,6 +171,51 @@ def test_force_many_near_leaves(pkg, oracle, wl, precision, chunks):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("inline", [1, 16, 4096])
+def test_force_inline_tiling(pkg, oracle, wl, precision, inline):
+    # the inline P2P count (a tiling option) only moves leaves between the
+    # traversal and the chunk waves: counts equal the oracle's, results agree
+    # with it and with the default tiling up to summation order
+    st = wl.Stream(32)
+    n = 30000
+    xs, ys, vs = st.uniform(n), st.uniform(n), st.uniform(n, -1, 1)
+    x0s = np.array([0.0, 0.31, 0.62, 0.0])
+    y0s = np.array([0.0, 0.27, 0.55, 0.5])
+    hs = np.array([1.0 / 32, 1.0 / 1024, 1.0 / 2048, 1.0 / 64])
+    ctx = pkg.Context(precision=precision, counters=True, force_order=12, force_leaf=8)
+    base = ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    ctx.set(pkg.OPT_FORCE_INLINE, inline)
+    fx, fy = ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    rx, ry, cnt = oracle.force_mrag(0.5, xs, ys, vs, x0s, y0s, hs, order=12, leafcap=8)
+    tol = FP64_TOL if precision == 0 else FP32_TOL
+    assert rel_l2(fx, rx) <= tol and rel_l2(fy, ry) <= tol
+    same = 1e-14 if precision == 0 else 1e-7
+    assert rel_l2(fx, base[0]) <= same and rel_l2(fy, base[1]) <= same
+    c = ctx.stats()["counters"]
+    assert (c["p2p"], c["e2p"], c["e2l"], c["l2p"]) == tuple(int(v) for v in cnt)
+    for bad in (-1, 4097):
+        with pytest.raises(pkg.TreecodeError, match="invalid option"):
+            ctx.set(pkg.OPT_FORCE_INLINE, bad)
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_potential_xcd_chunk_tiling(pkg, wl, precision):
+    # the traversal's workgroup-to-XCD chunking (a tiling option) only changes
+    # which workgroup runs which targets: results are bit-identical
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(200000, seed=33)
+    ctx = pkg.Context(precision=precision, pot_order=10, pot_leaf=128)
+    ref = ctx.potential(0.5, xs, ys, vs, xt, yt)
+    for chunk in (1, 2, 64, 256):
+        ctx.set(pkg.OPT_POT_XCD_CHUNK, chunk)
+        assert np.array_equal(ctx.potential(0.5, xs, ys, vs, xt, yt), ref), chunk
+    for bad in (-1, 3, 512):
+        with pytest.raises(pkg.TreecodeError, match="invalid option"):
+            ctx.set(pkg.OPT_POT_XCD_CHUNK, bad)
+    ctx.close()
+
+
 def test_potential_targets_elsewhere(pkg, oracle, wl, ctx64):
     # targets outside the source box, in random order, fewer than sources
     st = wl.Stream(77)
