@@ -22,8 +22,10 @@ import numpy as np
 DEFAULT = {"potential": (12, 512), "force": (24, 192)}
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 ORDERS = (6, 8, 9, 10, 12, 14, 16, 20, 24, 32)
-# tiling option per kind and the values a sweep tries (0 is the kernel's default)
+# tiling option per kind, the values a sweep tries and the kernel's default
 TILING = {"potential": ("OPT_POT_XCD_CHUNK", (1, 4, 16, 64)), "force": ("OPT_FORCE_INLINE", (16, 32, 64, 128))}
+TILING_DEFAULT = {"potential": 16, "force": 64}
+TILE_MARGIN = 0.01  # another tiling replaces the default only when at least this much faster
 LEAVES = (64, 128, 192, 256, 384, 512, 768, 1024)
 
 
@@ -172,6 +174,9 @@ class Tuner:
             ctx.set(tkey, 0)
             okt = [t for t in ttable if t["err"] <= max(err_target, best["err"] * 1.01)] or ttable
             bt = min(okt, key=lambda t: t["ms"])
+            dflt = [t for t in okt if t["tile"] == TILING_DEFAULT[kind]]
+            if dflt and bt["ms"] > dflt[0]["ms"] * (1.0 - TILE_MARGIN):
+                bt = dflt[0]  # within timing noise of the default
             res.update(tile=bt["tile"], ms=bt["ms"], err=bt["err"], tile_table=ttable)
         n = len(data["xs"])
         if save:

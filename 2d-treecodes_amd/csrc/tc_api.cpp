@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <initializer_list>
 #include <limits>
@@ -30,6 +31,46 @@
 void tc_upload_binomials_eval(hipStream_t st);
 
 enum TcKind { TC_KIND_NONE = 0, TC_KIND_POTENTIAL = 1, TC_KIND_FORCE = 2 };
+
+constexpr size_t STAGE_BYTES = size_t(16) << 20;  // per staging buffer
+constexpr size_t STAGE_MIN = size_t(4) << 20;     // smaller copies go direct
+
+// Two pinned staging buffers and their events, so the CPU copy of one chunk
+// overlaps the DMA of the other (allocated on first large copy).
+struct TcStager {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool busy[2] = {false, false};
+    void ready()
+    {
+        if (buf[0])
+            return;
+        for (int b = 0; b < 2; ++b) {
+            TC_HIP(hipHostMalloc(&buf[b], STAGE_BYTES, hipHostMallocDefault));
+            TC_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+        }
+    }
+    // Waits until buffer b is no longer read or written by the device.
+    void wait(int b)
+    {
+        if (busy[b]) {
+            TC_HIP(hipEventSynchronize(ev[b]));
+            busy[b] = false;
+        }
+    }
+    void release()
+    {
+        for (int b = 0; b < 2; ++b) {
+            if (buf[b])
+                (void)hipHostFree(buf[b]);
+            if (ev[b])
+                (void)hipEventDestroy(ev[b]);
+            buf[b] = nullptr;
+            ev[b] = nullptr;
+            busy[b] = false;
+        }
+    }
+};
 
 struct tc_context {
     int device = 0;
@@ -53,11 +94,9 @@ struct tc_context {
     bool ev_valid = false, eval_has_tsort = false, bev_valid = false;
     unsigned long long counters[TC_NCOUNTERS] = {0};
     bool counters_on_host = false;  // the last call left its counters in `counters` (no-allocation path)
-    // pinned staging for host arrays: two buffers, so the CPU copy of one
-    // chunk overlaps the DMA of the other (allocated on first large copy)
-    void *stage[2] = {nullptr, nullptr};
-    hipEvent_t stage_ev[2] = {nullptr, nullptr};
-    bool stage_busy[2] = {false, false};
+    // pinned staging for host arrays: one set on the context stream, one for
+    // target uploads on the second stream (beside the source build)
+    TcStager stg, tstg;
     // tc_potential_noalloc: tree, source and target scratch and counters carved
     // from the device's static scratch array (address looked up at creation)
     void *na_base = nullptr;
@@ -108,9 +147,6 @@ void ensure_binomials(int device, hipStream_t st)
     }
 }
 
-constexpr size_t STAGE_BYTES = size_t(16) << 20;  // per staging buffer
-constexpr size_t STAGE_MIN = size_t(4) << 20;     // smaller copies go direct
-
 // memcpy split over up to 8 threads (pageable <-> pinned host memory).
 void par_memcpy(void *dst, const void *src, size_t bytes)
 {
@@ -133,46 +169,27 @@ void par_memcpy(void *dst, const void *src, size_t bytes)
         t.join();
 }
 
-void stage_ready(tc_context *c)
-{
-    if (c->stage[0])
-        return;
-    for (int b = 0; b < 2; ++b) {
-        TC_HIP(hipHostMalloc(&c->stage[b], STAGE_BYTES, hipHostMallocDefault));
-        TC_HIP(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
-    }
-}
-
-// Waits until staging buffer b is no longer read or written by the device.
-void stage_wait(tc_context *c, int b)
-{
-    if (c->stage_busy[b]) {
-        TC_HIP(hipEventSynchronize(c->stage_ev[b]));
-        c->stage_busy[b] = false;
-    }
-}
-
-// Host array -> device, stream-ordered.  Large copies go through the pinned
-// staging buffers in 16 MiB chunks: the CPU fills one buffer while the DMA
-// engine drains the other.
-void copy_in(tc_context *c, const double *h, double *d, long long n)
+// Host array -> device, ordered on stream st.  Large copies go through the
+// pinned staging buffers of sg in 16 MiB chunks: the CPU fills one buffer
+// while the DMA engine drains the other.
+void copy_in(TcStager &sg, hipStream_t st, const double *h, double *d, long long n)
 {
     const size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 0);
     if (!bytes)
         return;
     if (bytes < STAGE_MIN) {
-        TC_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+        TC_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
         return;
     }
-    stage_ready(c);
+    sg.ready();
     int b = 0;
     for (size_t o = 0; o < bytes; o += STAGE_BYTES, b ^= 1) {
         const size_t len = std::min(STAGE_BYTES, bytes - o);
-        stage_wait(c, b);
-        par_memcpy(c->stage[b], (const char *)h + o, len);
-        TC_HIP(hipMemcpyAsync((char *)d + o, c->stage[b], len, hipMemcpyHostToDevice, c->stream));
-        TC_HIP(hipEventRecord(c->stage_ev[b], c->stream));
-        c->stage_busy[b] = true;
+        sg.wait(b);
+        par_memcpy(sg.buf[b], (const char *)h + o, len);
+        TC_HIP(hipMemcpyAsync((char *)d + o, sg.buf[b], len, hipMemcpyHostToDevice, st));
+        TC_HIP(hipEventRecord(sg.ev[b], st));
+        sg.busy[b] = true;
     }
 }
 
@@ -186,23 +203,24 @@ void copy_out(tc_context *c, const double *d, double *h, long long n)
         TC_HIP(hipStreamSynchronize(c->stream));
         return;
     }
-    stage_ready(c);
+    TcStager &sg = c->stg;
+    sg.ready();
     size_t pend_o[2] = {0, 0}, pend_len[2] = {0, 0};
     int b = 0;
     auto drain = [&](int k) {
         if (pend_len[k]) {
-            stage_wait(c, k);
-            par_memcpy((char *)h + pend_o[k], c->stage[k], pend_len[k]);
+            sg.wait(k);
+            par_memcpy((char *)h + pend_o[k], sg.buf[k], pend_len[k]);
             pend_len[k] = 0;
         }
     };
     for (size_t o = 0; o < bytes; o += STAGE_BYTES, b ^= 1) {
         const size_t len = std::min(STAGE_BYTES, bytes - o);
         drain(b);
-        stage_wait(c, b);
-        TC_HIP(hipMemcpyAsync(c->stage[b], (const char *)d + o, len, hipMemcpyDeviceToHost, c->stream));
-        TC_HIP(hipEventRecord(c->stage_ev[b], c->stream));
-        c->stage_busy[b] = true;
+        sg.wait(b);
+        TC_HIP(hipMemcpyAsync(sg.buf[b], (const char *)d + o, len, hipMemcpyDeviceToHost, c->stream));
+        TC_HIP(hipEventRecord(sg.ev[b], c->stream));
+        sg.busy[b] = true;
         pend_o[b] = o;
         pend_len[b] = len;
     }
@@ -235,21 +253,21 @@ void copy_in_all(tc_context *c, double *d, std::initializer_list<HostIn> arrs)
         return;
     if (total > STAGE_BYTES) {
         for (const HostIn &a : arrs) {
-            copy_in(c, a.h, d, a.n);
+            copy_in(c->stg, c->stream, a.h, d, a.n);
             d += a.n > 0 ? a.n : 0;
         }
         return;
     }
-    stage_ready(c);
-    stage_wait(c, 0);
-    char *s = static_cast<char *>(c->stage[0]);
+    c->stg.ready();
+    c->stg.wait(0);
+    char *s = static_cast<char *>(c->stg.buf[0]);
     for (const HostIn &a : arrs) {
         par_memcpy(s, a.h, arr_bytes(a.n));
         s += arr_bytes(a.n);
     }
-    TC_HIP(hipMemcpyAsync(d, c->stage[0], total, hipMemcpyHostToDevice, c->stream));
-    TC_HIP(hipEventRecord(c->stage_ev[0], c->stream));
-    c->stage_busy[0] = true;
+    TC_HIP(hipMemcpyAsync(d, c->stg.buf[0], total, hipMemcpyHostToDevice, c->stream));
+    TC_HIP(hipEventRecord(c->stg.ev[0], c->stream));
+    c->stg.busy[0] = true;
 }
 
 // Device block from d (arrays back to back) -> host arrays; returns with the
@@ -268,11 +286,11 @@ void copy_out_all(tc_context *c, const double *d, std::initializer_list<HostOut>
         TC_HIP(hipStreamSynchronize(c->stream));
         return;
     }
-    stage_ready(c);
-    stage_wait(c, 0);
-    TC_HIP(hipMemcpyAsync(c->stage[0], d, total, hipMemcpyDeviceToHost, c->stream));
+    c->stg.ready();
+    c->stg.wait(0);
+    TC_HIP(hipMemcpyAsync(c->stg.buf[0], d, total, hipMemcpyDeviceToHost, c->stream));
     TC_HIP(hipStreamSynchronize(c->stream));
-    const char *s = static_cast<const char *>(c->stage[0]);
+    const char *s = static_cast<const char *>(c->stg.buf[0]);
     for (const HostOut &a : arrs) {
         par_memcpy(a.h, s, arr_bytes(a.n));
         s += arr_bytes(a.n);
@@ -419,6 +437,11 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             TC_HIP(hipStreamWaitEvent(c->stream, ext.ready, 0));
         const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
         double *dout = out;
+        // host targets of at least two staging buffers are uploaded by a helper
+        // thread, through their own staging buffers on the second stream, while
+        // the source tree builds (the build waits on the host only for its
+        // census): the upload then overlaps the build instead of preceding it
+        const bool overlap = !device_ptrs && rebuild && 2 * arr_bytes(m) >= 2 * STAGE_BYTES;
         if (!device_ptrs) {
             const size_t ns = rebuild ? (size_t)n : 0;
             c->io.reserve(sizeof(double) * (3 * ns + 3 * (size_t)m));
@@ -426,7 +449,9 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             double *bxs = base, *bys = base + ns, *bvs = base + 2 * ns;
             double *bxt = base + 3 * ns, *byt = bxt + m;
             dout = byt + m;
-            if (rebuild)
+            if (overlap)
+                copy_in_all(c, bxs, {{xs, n}, {ys, n}, {vs, n}});
+            else if (rebuild)
                 copy_in_all(c, bxs, {{xs, n}, {ys, n}, {vs, n}, {xt, m}, {yt, m}});
             else
                 copy_in_all(c, bxt, {{xt, m}, {yt, m}});
@@ -438,15 +463,33 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             // source build; the evaluation waits for both
             TC_HIP(hipEventRecord(c->fork_ev, c->stream));
             TC_HIP(hipStreamWaitEvent(c->tstream, c->fork_ev, 0));
-            try {
-                tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
-                TC_HIP(hipEventRecord(c->join_ev, c->tstream));
-            } catch (...) {
-                // whatever part of the sort was queued stays ordered before
-                // later work on the context stream
-                (void)hipEventRecord(c->join_ev, c->tstream);
-                (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
-                throw;
+            std::thread up;
+            std::exception_ptr up_err;
+            if (overlap) {
+                up = std::thread([&] {
+                    try {
+                        TC_HIP(hipSetDevice(c->device));
+                        copy_in(c->tstg, c->tstream, xt, const_cast<double *>(dxt), m);
+                        copy_in(c->tstg, c->tstream, yt, const_cast<double *>(dyt), m);
+                        tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
+                    } catch (...) {
+                        up_err = std::current_exception();
+                    }
+                    // whatever was queued stays ordered before later work on
+                    // the context stream
+                    (void)hipEventRecord(c->join_ev, c->tstream);
+                });
+            } else {
+                try {
+                    tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
+                    TC_HIP(hipEventRecord(c->join_ev, c->tstream));
+                } catch (...) {
+                    // whatever part of the sort was queued stays ordered before
+                    // later work on the context stream
+                    (void)hipEventRecord(c->join_ev, c->tstream);
+                    (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
+                    throw;
+                }
             }
             record(c, 0);
             try {
@@ -454,8 +497,16 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             } catch (...) {
                 // later work on the context stream (the next call's copies
                 // into the io buffer) must not overtake the sort reading it
+                if (up.joinable())
+                    up.join();
                 (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
                 throw;
+            }
+            if (up.joinable())
+                up.join();
+            if (up_err) {
+                (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
+                std::rethrow_exception(up_err);
             }
             record(c, 1);
             TC_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
@@ -871,12 +922,8 @@ TC_API void tc_context_destroy(tc_context *c)
     for (auto &e : c->bev)
         if (e)
             (void)hipEventDestroy(e);
-    for (int b = 0; b < 2; ++b) {
-        if (c->stage[b])
-            (void)hipHostFree(c->stage[b]);
-        if (c->stage_ev[b])
-            (void)hipEventDestroy(c->stage_ev[b]);
-    }
+    c->stg.release();
+    c->tstg.release();
     for (hipEvent_t e : {c->fork_ev, c->join_ev})
         if (e)
             (void)hipEventDestroy(e);

@@ -216,6 +216,29 @@ def test_potential_xcd_chunk_tiling(pkg, wl, precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_host_targets_uploaded_beside_build(pkg, wl, precision):
+    # host target arrays of 2^21 points (two staging buffers and more) are uploaded
+    # by a helper thread on the second stream while the sources build: the results
+    # equal the device-pointer solve's, and a non-finite target is still refused
+    n = 1 << 21
+    xs, ys, vs, xt, yt = wl.mixed_plummer_potential(n, seed=35)
+    ctx = pkg.Context(precision=precision, pot_order=10, pot_leaf=192)
+    host = ctx.potential(0.5, xs, ys, vs, xt, yt)
+    d = {k: ctx.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, xt=xt, yt=yt).items()}
+    out = ctx.empty(n)
+    ctx.potential(0.5, d["xs"], d["ys"], d["vs"], d["xt"], d["yt"], out=out, device=True)
+    ctx.synchronize()
+    assert np.array_equal(host, out.numpy())
+    bad = yt.copy()
+    bad[n - 5] = np.inf
+    with pytest.raises(pkg.TreecodeError, match="non-finite"):
+        ctx.potential(0.5, xs, ys, vs, xt, bad)
+    again = ctx.potential(0.5, xs, ys, vs, xt, yt)
+    assert np.array_equal(again, host)
+    ctx.close()
+
+
 def test_potential_targets_elsewhere(pkg, oracle, wl, ctx64):
     # targets outside the source box, in random order, fewer than sources
     st = wl.Stream(77)
