@@ -19,6 +19,7 @@ constexpr int RS_ITEMS = 16;   // keys per thread per tile
 constexpr int RS_TILE = RS_NT * RS_ITEMS;
 constexpr int RS_BINS = 256;   // 8-bit digits: passes at bits 0, 8, 16, 24 (30-bit keys)
 constexpr int RS_DIRECT_TILES = 64;  // up to this many tiles, no separate scan per pass
+constexpr int RS_ROW_ITEMS = 16;     // tile counts per thread in the per-digit row scan
 constexpr int SCAN_NT = 1024;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_CHUNK = SCAN_NT * SCAN_ITEMS;
@@ -273,10 +274,11 @@ __global__ __launch_bounds__(256) void k_keys_hilbert(const double *x, const dou
 
 // ------------------------------------------------------------------ radix sort
 // Stable LSD radix sort of (key, index) pairs, 8-bit digits.  Per pass:
-// k_rs_hist (tile histograms, digit-major), one exclusive scan over the
-// digit-major table (= global output offset of each (digit, tile)), and
-// k_rs_scatter, which ranks a tile stably in LDS with wavefront ballots and
-// writes digit runs contiguously.
+// k_rs_hist (tile histograms, digit-major), k_rs_rowscan (per digit, the
+// exclusive prefix of its counts over the tiles and the digit's total), and
+// k_rs_scatter, which places digit d of tile t at (keys of smaller digits) +
+// (digit d in earlier tiles), ranks the tile stably in LDS with wavefront
+// ballots and writes digit runs contiguously.
 __global__ __launch_bounds__(RS_NT) void k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *hist, int ntiles)
 {
     __shared__ uint32_t h[RS_BINS];
@@ -291,6 +293,40 @@ __global__ __launch_bounds__(RS_NT) void k_rs_hist(const uint32_t *keys, int n, 
     }
     __syncthreads();
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block d scans row d of the digit-major tile histograms: offs[d][t] = count
+// of digit d in tiles 0..t-1, rowtot[d] = its count in all tiles.  One launch
+// of RS_BINS independent blocks (a scan of the whole table in digit-major
+// order would need three launches and a serial middle step).
+__global__ __launch_bounds__(RS_NT) void k_rs_rowscan(const uint32_t *hist, int ntiles, uint32_t *offs,
+                                                       uint32_t *rowtot)
+{
+    __shared__ uint32_t s_w[RS_NT / 64 + 1];
+    const uint32_t *row = hist + (size_t)blockIdx.x * ntiles;
+    uint32_t *orow = offs + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int base = 0; base < ntiles; base += RS_NT * RS_ROW_ITEMS) {
+        uint32_t v[RS_ROW_ITEMS];
+        uint32_t sum = 0;
+        const int i0 = base + threadIdx.x * RS_ROW_ITEMS;
+#pragma unroll
+        for (int j = 0; j < RS_ROW_ITEMS; ++j) {
+            v[j] = i0 + j < ntiles ? row[i0 + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint32_t pre = block_scan_excl<RS_NT>(sum, s_w, &tot) + carry;
+#pragma unroll
+        for (int j = 0; j < RS_ROW_ITEMS; ++j) {
+            if (i0 + j < ntiles)
+                orow[i0 + j] = pre;
+            pre += v[j];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0)
+        rowtot[blockIdx.x] = carry;
 }
 
 // Lanes of the wave holding the same 8-bit digit d as this lane (among lanes
@@ -311,12 +347,12 @@ TC_DEV unsigned long long wave_digit_peers(uint32_t d, bool ok)
     return tc_ballot(ok) & ~diff;
 }
 
-// offs: the scanned digit-major table (global start of each (digit, tile));
+// offs, rowtot: k_rs_rowscan's per-digit tile prefixes and digit totals;
 // with offs == nullptr the block derives its starts from the tile histograms
 // `hist` itself (few tiles: the whole table is a few KiB).
 __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                        uint32_t *vout, int n, int shift, const uint32_t *offs,
-                                                       const uint32_t *hist, int ntiles)
+                                                       const uint32_t *rowtot, const uint32_t *hist, int ntiles)
 {
     constexpr int NW = RS_NT / 64, WITEMS = 64 * RS_ITEMS;  // waves per tile, keys per wave
     __shared__ uint32_t s_k[RS_TILE];
@@ -332,7 +368,9 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     for (int d = lane; d < RS_BINS; d += 64)
         s_wrun[wid][d] = 0;
     if (offs) {
-        s_gbase[tid] = offs[tid * ntiles + blockIdx.x];
+        // digit tid: keys of smaller digits (scan of the digit totals) + this
+        // digit in earlier tiles
+        s_gbase[tid] = block_scan_excl<RS_NT>(rowtot[tid], s_w, nullptr) + offs[(size_t)tid * ntiles + blockIdx.x];
     } else {
         // digit tid: keys of smaller digits (all tiles) + this digit in earlier tiles
         uint32_t tot = 0, pre = 0;
@@ -1305,9 +1343,10 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
     if (ev)
         TC_HIP(hipEventRecord(ev[0], st));
     const int ntiles = tiles_of(n, RS_TILE);
-    ws.hist.reserve(sizeof(uint32_t) * (size_t)RS_BINS * ntiles * 2);
+    ws.hist.reserve(sizeof(uint32_t) * ((size_t)RS_BINS * ntiles * 2 + RS_BINS));
     uint32_t *hist = ws.hist.as<uint32_t>();
     uint32_t *offs = hist + (size_t)RS_BINS * ntiles;
+    uint32_t *rowtot = offs + (size_t)RS_BINS * ntiles;
     // ping-pong out -> scratch -> out -> scratch -> out (four 8-bit passes over 30-bit keys);
     // the first pass generates the index payload itself
     uint32_t *kin[4] = {keys_out, ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>()};
@@ -1321,9 +1360,9 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
         const int shift = 8 * pass;
         k_rs_hist<<<ntiles, RS_NT, 0, st>>>(kin[pass], n, shift, hist, ntiles);
         if (!direct)
-            tc_scan_exclusive_u32(hist, offs, RS_BINS * ntiles, nullptr, ws, st);
+            k_rs_rowscan<<<RS_BINS, RS_NT, 0, st>>>(hist, ntiles, offs, rowtot);
         k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift,
-                                                direct ? nullptr : offs, hist, ntiles);
+                                                direct ? nullptr : offs, rowtot, hist, ntiles);
         TC_HIP(hipGetLastError());
     }
 }

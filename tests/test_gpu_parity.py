@@ -29,8 +29,9 @@ def _device_sort(pkg, ctx, x, y):
     return box.numpy(), keys.numpy(), perm.numpy()
 
 
-# up to 16384 points the sort runs in one workgroup, above it in tiles of 4096
-@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 16384, 16385, 100003])
+# up to 16384 points the sort runs in one workgroup, above it in tiles of 4096;
+# above 64 tiles (262144 points) each pass scans the tile histograms per digit
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 16384, 16385, 100003, 262144, 262145, 1000003])
 def test_morton_sort_bit_exact(pkg, oracle, wl, ctx64, n):
     st = wl.Stream(100 + n)
     x, y = st.uniform(n), st.uniform(n)
@@ -56,11 +57,12 @@ def test_morton_sort_duplicates_stable(pkg, oracle, wl, ctx64):
     assert np.array_equal(keys, oks)
 
 
-def test_morton_sort_two_keys_stable(pkg, oracle, wl, ctx64):
+@pytest.mark.parametrize("n", [3 * 4096 * 16 + 77, 80 * 4096 + 5])
+def test_morton_sort_two_keys_stable(pkg, oracle, wl, ctx64, n):
     # two distinct keys over many tiles: each 1024-key run of the sort holds
     # hundreds of equal digits, so ranks within a run reach their maximum
+    # (48 tiles: offsets from the histograms directly; 81: per-digit row scans)
     st = wl.Stream(6)
-    n = 3 * 4096 * 16 + 77
     pick = st.uniform(n) < 0.5
     x = np.where(pick, 0.1, 0.9)
     y = np.where(pick, 0.2, 0.8)
