@@ -34,7 +34,7 @@ EXPORTED = [
     "tc_direct_potential", "tc_direct_force", "tc_server_create", "tc_server_destroy", "tc_server_submit",
     "tc_server_wait", "tc_server_stats", "treecode_potential_float", "treecode_force_mrag_float",
     "treecode_velocity_mrag", "tc_copy_kernel", "treecode_force", "tc_force_targets", "tc_potential_noalloc",
-    "tc_get_profile", "tc_server_create_ex", "tc_server_stats_ex",
+    "tc_get_profile", "tc_server_create_ex", "tc_server_stats_ex", "tc_potential_batch_resident",
 ]
 SERVER_BATCH = 1  # tc_server_create_ex flag: batch concurrent potential traversals
 SERVER_BATCH_ALWAYS = 2  # also a lone traversal goes through the batch kernel (tests its graphs)
@@ -111,6 +111,8 @@ def lib():
         L.tc_potential_resident.argtypes = [_P, dbl, _P, _P, it, _P, it]
     if hasattr(L, 'tc_force_mrag_resident'):
         L.tc_force_mrag_resident.argtypes = [_P, dbl, _P, _P, _P, it, _P, _P, it]
+    if hasattr(L, 'tc_potential_batch_resident'):
+        L.tc_potential_batch_resident.argtypes = [_P, it, _D, _P, _P, _I, _P]
     if hasattr(L, 'tc_synchronize'):
         L.tc_synchronize.argtypes = [_P]
     if hasattr(L, 'tc_get_stats'):
@@ -460,6 +462,25 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def potential_batch_resident(requests):
+    """tc_potential_batch_resident: requests = [(ctx, theta, xt, yt, out), ...] with DeviceArrays,
+    each context holding a resident potential tree (same order and precision for all); one
+    launch of the cross-request traversal writes every out.  Synchronous."""
+    k = len(requests)
+    if k == 0:
+        raise TreecodeError("potential_batch_resident: no request")
+    P = ctypes.c_void_p
+    ctxs = (P * k)(*[r[0].h for r in requests])
+    th = (ctypes.c_double * k)(*[float(r[1]) for r in requests])
+    xt = (P * k)(*[r[2].ptr for r in requests])
+    yt = (P * k)(*[r[3].ptr for r in requests])
+    m = (ctypes.c_int * k)(*[len(r[2]) for r in requests])
+    out = (P * k)(*[r[4].ptr for r in requests])
+    if lib().tc_potential_batch_resident(ctxs, k, th, xt, yt, m, out) != 0:
+        raise TreecodeError(last_error())
+    return [r[4] for r in requests]
 
 
 class Server:

@@ -97,6 +97,11 @@ struct tc_context {
     // pinned staging for host arrays: one set on the context stream, one for
     // target uploads on the second stream (beside the source build)
     TcStager stg, tstg;
+    // tc_potential_batch_resident: descriptors, pinned ring and graphs of its
+    // launches; the target-sort descriptors of its small requests (pinned + device)
+    TcBatchWs bws;
+    void *sdesc_host = nullptr;
+    DevBuf sdesc;
     // tc_potential_noalloc: tree, source and target scratch and counters carved
     // from the device's static scratch array (address looked up at creation)
     void *na_base = nullptr;
@@ -908,6 +913,10 @@ TC_API void tc_context_destroy(tc_context *c)
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
     c->tree.release();
+    c->bws.release();
+    c->sdesc.release();
+    if (c->sdesc_host)
+        (void)hipHostFree(c->sdesc_host);
     c->ws.release();
     c->tws.release();
     c->io.release();
@@ -1003,6 +1012,79 @@ TC_API int tc_potential_resident(tc_context *c, double theta, const double *xt, 
                                  double *out, int device_ptrs)
 {
     return run_potential(c, theta, nullptr, nullptr, nullptr, 0, xt, yt, m, out, device_ptrs, false);
+}
+
+TC_API int tc_potential_batch_resident(tc_context *const *ctxs, int k, const double *theta,
+                                       const double *const *xt, const double *const *yt, const int *m,
+                                       double *const *out)
+{
+    if (!ctxs || k <= 0 || !theta || !xt || !yt || !m || !out) {
+        set_error("tc_potential_batch_resident: null argument or no request");
+        return -1;
+    }
+    tc_context *c0 = ctxs[0];
+    return guarded(c0, [&] {
+        if (k > TcBatchWs::MAXREQ)
+            throw TcError(-3, "tc_potential_batch_resident: at most 64 requests per call");
+        for (int i = 0; i < k; ++i) {
+            tc_context *c = ctxs[i];
+            if (!c)
+                throw TcError(-1, "tc_potential_batch_resident: null context");
+            for (int j = 0; j < i; ++j)
+                if (ctxs[j] == c)
+                    throw TcError(-3, "tc_potential_batch_resident: a context appears twice");
+            if (c->device != c0->device)
+                throw TcError(-3, "tc_potential_batch_resident: contexts on different devices");
+            check_resident(c, TC_KIND_POTENTIAL);
+            if (c->tree.order != c0->tree.order || c->precision != c0->precision)
+                throw TcError(-3, "tc_potential_batch_resident: trees of different orders or precisions");
+            if (m[i] < 0 || (m[i] > 0 && (!xt[i] || !yt[i] || !out[i])))
+                throw TcError(-3, "tc_potential_batch_resident: bad target arrays");
+            c->ev_valid = false;
+        }
+        hipStream_t st = c0->stream;
+        // work queued on the other contexts (their trees' builds) before the batch
+        for (int i = 1; i < k; ++i) {
+            TC_HIP(hipEventRecord(ctxs[i]->join_ev, ctxs[i]->stream));
+            TC_HIP(hipStreamWaitEvent(st, ctxs[i]->join_ev, 0));
+        }
+        // each request's target order in its own context's scratch: requests of
+        // at most TC_SORT_SMALL_MAX targets in one launch, larger ones alone;
+        // then one launch of the traversal
+        if (!c0->sdesc_host)
+            TC_HIP(hipHostMalloc(&c0->sdesc_host, sizeof(TcSortDesc) * TcBatchWs::MAXREQ, hipHostMallocDefault));
+        c0->sdesc.reserve(sizeof(TcSortDesc) * TcBatchWs::MAXREQ);
+        TcSortDesc *sd = static_cast<TcSortDesc *>(c0->sdesc_host);
+        int nsmall = 0;
+        for (int i = 0; i < k; ++i) {
+            if (m[i] > 0 && m[i] <= TC_SORT_SMALL_MAX)
+                sd[nsmall++] = tc_sort_targets_desc(xt[i], yt[i], m[i], ctxs[i]->tws);
+            else if (m[i] > 0)
+                tc_sort_targets(xt[i], yt[i], m[i], ctxs[i]->tws, st);
+        }
+        if (nsmall) {
+            TC_HIP(hipMemcpyAsync(c0->sdesc.ptr, sd, sizeof(TcSortDesc) * nsmall, hipMemcpyHostToDevice, st));
+            tc_sort_targets_batch(c0->sdesc.as<TcSortDesc>(), nsmall, st);
+        }
+        std::vector<TcPotBatchItem> items;
+        for (int i = 0; i < k; ++i) {
+            if (m[i] == 0)
+                continue;
+            TcPotBatchItem it;
+            it.tree = &ctxs[i]->tree;
+            it.theta = theta[i];
+            it.xt = xt[i];
+            it.yt = yt[i];
+            it.tperm = ctxs[i]->tws.tperm.as<uint32_t>();
+            it.m = m[i];
+            it.out = out[i];
+            items.push_back(it);
+        }
+        if (!items.empty())
+            tc_potential_batch(items.data(), (int)items.size(), c0->tree.order, c0->precision, c0->bws, nullptr, st,
+                               true);
+        TC_HIP(hipStreamSynchronize(st));
+    });
 }
 
 #ifndef TREECODES2D_FLOAT_DROPIN

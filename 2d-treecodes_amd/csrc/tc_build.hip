@@ -455,12 +455,15 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
 // of the sorted records.  One launch instead of about sixteen, which is what a
 // small solve is made of.
 constexpr int SS_NT = 1024, SS_ITEMS = 16, SS_N = SS_NT * SS_ITEMS;
+static_assert(SS_N == TC_SORT_SMALL_MAX, "one-workgroup sort size");
 
+// The one-workgroup sort of one point set (the body of k_sort_small and of
+// k_sort_small_batch's workgroups).
 template <bool HILBERT>
-__global__ __launch_bounds__(SS_NT) void k_sort_small(const double *__restrict__ x, const double *__restrict__ y,
-                                                      const double *__restrict__ v, int n, double *__restrict__ box_out,
-                                                      uint32_t *__restrict__ bad_out, uint32_t *__restrict__ keys_out,
-                                                      uint32_t *__restrict__ perm_out, double4 *__restrict__ src_out)
+TC_DEV void sort_small(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ v,
+                       int n, double *__restrict__ box_out, uint32_t *__restrict__ bad_out,
+                       uint32_t *__restrict__ keys_out, uint32_t *__restrict__ perm_out,
+                       double4 *__restrict__ src_out)
 {
     constexpr int NW = SS_NT / 64, WITEMS = 64 * SS_ITEMS;
     __shared__ uint32_t s_k[SS_N];
@@ -613,6 +616,23 @@ __global__ __launch_bounds__(SS_NT) void k_sort_small(const double *__restrict__
         if (src_out)
             src_out[p] = make_double4(x[i], y[i], v[i], 0.0);
     }
+}
+
+template <bool HILBERT>
+__global__ __launch_bounds__(SS_NT) void k_sort_small(const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ v, int n, double *__restrict__ box_out,
+                                                      uint32_t *__restrict__ bad_out, uint32_t *__restrict__ keys_out,
+                                                      uint32_t *__restrict__ perm_out, double4 *__restrict__ src_out)
+{
+    sort_small<HILBERT>(x, y, v, n, box_out, bad_out, keys_out, perm_out, src_out);
+}
+
+// Target orders of several requests at once: workgroup b sorts desc[b]'s
+// targets exactly as k_sort_small<true> does alone.
+__global__ __launch_bounds__(SS_NT) void k_sort_small_batch(const TcSortDesc *__restrict__ desc)
+{
+    const TcSortDesc &d = desc[blockIdx.x];
+    sort_small<true>(d.x, d.y, nullptr, d.n, d.box, d.bad, d.keys, d.perm, nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_gather_src(const double4 *packed, const uint32_t *perm, int n,
@@ -1497,6 +1517,36 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
 }
 
 const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m) { return tc_bbox_bad_flag(ws, m); }
+
+TcSortDesc tc_sort_targets_desc(const double *xt, const double *yt, int m, TcScratch &ws)
+{
+    if (m < 1 || m > SS_N)
+        throw TcError(-3, "tc_sort_targets_desc: 1 to 16384 targets");
+    // the buffers and places tc_sort_targets uses for m targets
+    ws.tkeys.reserve(sizeof(uint32_t) * (size_t)m + 64);
+    ws.tperm.reserve(sizeof(uint32_t) * (size_t)m + 64);
+    ws.tsorted.reserve(sizeof(double) * 2 * (size_t)m + 64);
+    ws.info.reserve(sizeof(int) * 32 + sizeof(double) * 4);
+    ws.bbox_part.reserve(sizeof(double4) + 256);
+    TcSortDesc d;
+    d.x = xt;
+    d.y = yt;
+    d.box = reinterpret_cast<double *>(ws.info.as<char>() + 128);
+    d.bad = const_cast<uint32_t *>(tc_bbox_bad_flag(ws, m));
+    d.keys = ws.tkeys.as<uint32_t>();
+    d.perm = ws.tperm.as<uint32_t>();
+    d.n = m;
+    d.pad = 0;
+    return d;
+}
+
+void tc_sort_targets_batch(const TcSortDesc *desc_dev, int k, hipStream_t st)
+{
+    if (k <= 0)
+        return;
+    k_sort_small_batch<<<k, SS_NT, 0, st>>>(desc_dev);
+    TC_HIP(hipGetLastError());
+}
 
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st)
 {

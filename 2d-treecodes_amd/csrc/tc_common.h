@@ -358,6 +358,20 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
                    bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev = nullptr);
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);
+// Several target sets of at most TC_SORT_SMALL_MAX targets each sorted in one
+// launch: tc_sort_targets_desc reserves ws's target buffers for m targets (as
+// tc_sort_targets would) and returns where the sort of xt, yt goes; a device
+// array of k such descriptors is then sorted by tc_sort_targets_batch, each
+// with the results tc_sort_targets gives it alone.
+constexpr int TC_SORT_SMALL_MAX = 16384;
+struct TcSortDesc {
+    const double *x, *y;
+    double *box;
+    uint32_t *bad, *keys, *perm;
+    int n, pad;
+};
+TcSortDesc tc_sort_targets_desc(const double *xt, const double *yt, int m, TcScratch &ws);
+void tc_sort_targets_batch(const TcSortDesc *desc_dev, int k, hipStream_t st);
 // Device flag (uint32, nonzero = a non-finite coordinate) set by the last
 // tc_sort_targets call on m targets; valid until the next bounding-box pass.
 const uint32_t *tc_targets_bad_flag(TcScratch &ws, int m);

@@ -714,48 +714,56 @@ __global__ __launch_bounds__(256) void k_potential(const TcNode *__restrict__ no
                                 (tc_xcd_chunk_block(xcd_lc) * 4 + (threadIdx.x >> 6)) * 64, counters);
 }
 
-// Cross-request traversal: one launch evaluates the waves of all meta[0]
-// requests of a batch (meta[1] waves in total); wave g finds its request by
-// binary search over the requests' first waves and does the same per-wave
-// work as k_potential, so each result equals the one-request launch's.  The
-// descriptors and the batch size live in device memory and waves beyond
-// meta[1] exit at once, so a launch with a grid rounded up to a size bucket
-// has fixed arguments: one graph per (order, precision, size bucket) replays it.
+// Cross-request traversal: a persistent grid drains the target waves of all
+// meta[0] requests of a batch (meta[1] waves in total).  Each wave takes the
+// next wave index g from the work counter meta[2] (zero in the descriptor
+// block that is copied in before every launch), finds g's request by binary
+// search over the requests' first waves and does the same per-wave work as
+// k_potential, so each result equals the one-request launch's whichever wave
+// of the grid computes it.  Every wave leaves once the counter has passed
+// meta[1].  The descriptors, the batch size and the counter live in device
+// memory and the grid depends only on the size bucket (its waves or the
+// resident capacity, whichever is smaller), so a launch has fixed arguments:
+// one graph per (order, precision, size bucket) replays it.
 template <int P, bool F32, bool CNT>
-__global__ __launch_bounds__(256) void k_potential_batch(const TcPotReq *__restrict__ reqs,
-                                                         const int *__restrict__ meta,
+__global__ __launch_bounds__(256) void k_potential_batch(const TcPotReq *__restrict__ reqs, int *meta,
                                                          unsigned long long *counters)
 {
     __shared__ TcPotShared<P, F32> sh;
     potential_shared_init<P, F32>(sh);
     const int nreq = meta[0], nwaves = meta[1];
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= nwaves)
-        return;
-    int lo = 0, hi = nreq - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (reqs[mid].wave0 <= g)
-            lo = mid;
-        else
-            hi = mid - 1;
+    for (;;) {
+        int g = 0;
+        if (tc_lane() == 0)
+            g = atomicAdd(&meta[2], 1);
+        g = __builtin_amdgcn_readfirstlane(g);
+        if (g >= nwaves)
+            break;
+        int lo = 0, hi = nreq - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (reqs[mid].wave0 <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        // the descriptor is wave-uniform (scalar registers) and its pointers
+        // point to global memory: telling the compiler so lets node records and
+        // coefficients travel through scalar loads as in k_potential
+        const TcPotReq &rq = reqs[lo];
+        auto up = [](const void *p) {
+            typedef const __attribute__((address_space(1))) char *gptr;
+            const gptr q = (gptr)(tc_uniform64((unsigned long long)p));
+            return (const void *)q;
+        };
+        potential_wave<P, F32, CNT>(
+            sh, static_cast<const TcNode *>(up(rq.nodes)), static_cast<const double *>(up(rq.exps)),
+            static_cast<const float2 *>(up(rq.exps32)), static_cast<const double4 *>(up(rq.src)),
+            static_cast<const float4 *>(up(rq.src32)), static_cast<const double *>(up(rq.xt)),
+            static_cast<const double *>(up(rq.yt)), static_cast<const uint32_t *>(up(rq.tperm)),
+            const_cast<double *>(static_cast<const double *>(up(rq.out))), tc_uniform(rq.th2), tc_uniform(rq.m),
+            (g - tc_uniform(rq.wave0)) * 64, counters);
     }
-    // the descriptor is wave-uniform (scalar registers) and its pointers point
-    // to global memory: telling the compiler so lets node records and
-    // coefficients travel through scalar loads as in k_potential
-    const TcPotReq &rq = reqs[lo];
-    auto up = [](const void *p) {
-        typedef const __attribute__((address_space(1))) char *gptr;
-        const gptr q = (gptr)(tc_uniform64((unsigned long long)p));
-        return (const void *)q;
-    };
-    potential_wave<P, F32, CNT>(
-        sh, static_cast<const TcNode *>(up(rq.nodes)), static_cast<const double *>(up(rq.exps)),
-        static_cast<const float2 *>(up(rq.exps32)), static_cast<const double4 *>(up(rq.src)),
-        static_cast<const float4 *>(up(rq.src32)), static_cast<const double *>(up(rq.xt)),
-        static_cast<const double *>(up(rq.yt)), static_cast<const uint32_t *>(up(rq.tperm)),
-        const_cast<double *>(static_cast<const double *>(up(rq.out))), tc_uniform(rq.th2), tc_uniform(rq.m),
-        (g - tc_uniform(rq.wave0)) * 64, counters);
 }
 
 // ------------------------------------------------------------------ force kernels
@@ -1958,17 +1966,27 @@ void tc_direct_sums(const double *xs, const double *ys, const double *vs, int n,
     TC_HIP(hipGetLastError());
 }
 
-// Device layout of a batch (one descriptor block): TcPotReq[MAXREQ], meta {k, waves}.
+// Device layout of a batch (one descriptor block): TcPotReq[MAXREQ], meta {k, waves, work counter}.
 constexpr size_t BATCH_DESC = sizeof(TcPotReq) * TcBatchWs::MAXREQ;
 constexpr size_t BATCH_HOST = BATCH_DESC + 256;  // one block, a multiple of 256 bytes
 
 TcPotReq *batch_desc(TcBatchWs &ws, int slot) { return reinterpret_cast<TcPotReq *>(ws.dev.as<char>() + (size_t)slot * BATCH_HOST); }
 int *batch_meta(TcBatchWs &ws, int slot) { return reinterpret_cast<int *>(ws.dev.as<char>() + (size_t)slot * BATCH_HOST + BATCH_DESC); }
 
-template <int P, bool F32, bool CNT>
-void batch_kernel(TcBatchWs &ws, int slot, unsigned long long *counters, int grid, hipStream_t st)
+// persistent grid of a batch launch: the bucket's waves in workgroups of
+// four, at most as many workgroups as are resident at once
+template <int P, bool F32, bool CNT> int batch_grid(int bucket)
 {
-    k_potential_batch<P, F32, CNT><<<grid, 256, 0, st>>>(batch_desc(ws, slot), batch_meta(ws, slot), counters);
+    int device = 0;
+    TC_HIP(hipGetDevice(&device));
+    return std::min((1 << bucket) / 4, resident_blocks<k_potential_batch<P, F32, CNT>>(device));
+}
+
+template <int P, bool F32, bool CNT>
+void batch_kernel(TcBatchWs &ws, int slot, unsigned long long *counters, int bucket, hipStream_t st)
+{
+    k_potential_batch<P, F32, CNT><<<batch_grid<P, F32, CNT>(bucket), 256, 0, st>>>(batch_desc(ws, slot),
+                                                                                 batch_meta(ws, slot), counters);
     TC_HIP(hipGetLastError());
 }
 
@@ -1977,8 +1995,9 @@ void batch_kernel(TcBatchWs &ws, int slot, unsigned long long *counters, int gri
 // bucket's grid.  Built node by node rather than by stream capture, so
 // building it never restricts what other threads may call meanwhile.
 template <int P, bool F32>
-hipGraphExec_t batch_graph(TcBatchWs &ws, int slot, int grid)
+hipGraphExec_t batch_graph(TcBatchWs &ws, int slot, int bucket)
 {
+    const int grid = batch_grid<P, F32, false>(bucket);
     TcPotReq *desc = batch_desc(ws, slot);
     int *meta = batch_meta(ws, slot);
     unsigned long long *counters = nullptr;
@@ -2022,23 +2041,22 @@ void batch_launch(TcBatchWs &ws, int slot, int precision, int waves, unsigned lo
 {
     const bool f32 = precision == 1;
     const int bucket = batch_bucket(waves);
-    const int grid = (1 << bucket) / 4;
     if (counters || !graph) {
         TC_HIP(hipMemcpyAsync(batch_desc(ws, slot), static_cast<char *>(ws.host) + (size_t)slot * BATCH_HOST,
                               BATCH_HOST, hipMemcpyHostToDevice, st));
         if (f32 && counters)
-            batch_kernel<P, true, true>(ws, slot, counters, grid, st);
+            batch_kernel<P, true, true>(ws, slot, counters, bucket, st);
         else if (f32)
-            batch_kernel<P, true, false>(ws, slot, nullptr, grid, st);
+            batch_kernel<P, true, false>(ws, slot, nullptr, bucket, st);
         else if (counters)
-            batch_kernel<P, false, true>(ws, slot, counters, grid, st);
+            batch_kernel<P, false, true>(ws, slot, counters, bucket, st);
         else
-            batch_kernel<P, false, false>(ws, slot, nullptr, grid, st);
+            batch_kernel<P, false, false>(ws, slot, nullptr, bucket, st);
         return;
     }
     hipGraphExec_t &ex = ws.graph[(2 * P + (f32 ? 1 : 0)) * TcBatchWs::NBUCKET + bucket - 6][slot];
     if (!ex)
-        ex = f32 ? batch_graph<P, true>(ws, slot, grid) : batch_graph<P, false>(ws, slot, grid);
+        ex = f32 ? batch_graph<P, true>(ws, slot, bucket) : batch_graph<P, false>(ws, slot, bucket);
     TC_HIP(hipGraphLaunch(ex, st));
 }
 
@@ -2111,6 +2129,7 @@ void tc_potential_batch(const TcPotBatchItem *items, int k, int order, int preci
     int *meta = reinterpret_cast<int *>(host + BATCH_DESC);
     meta[0] = k;
     meta[1] = waves;
+    meta[2] = 0;  // work counter of the persistent grid
     switch (order) {
 #define TC_CASE(P) \
     case P: batch_launch<P>(ws, slot, precision, waves, counters, st, graph); break;

@@ -31,11 +31,13 @@
 // its own stream, then hands the traversal to a leader -- the first slot
 // that finds none running -- which launches the traversals of all waiting
 // requests of the same order and precision as one kernel (k_potential_batch,
-// through a graph built once per size bucket and replayed) on one of their
-// streams; a traversal that finds no other waiting runs unbatched.
+// a persistent grid draining the requests' target waves from one work
+// counter, through a graph built once per size bucket and replayed) on one of
+// their streams; a traversal that finds no other waiting runs unbatched.
 //
-// Kernels never communicate across requests and use no cross-request atomics,
-// so a result is bit-identical alone and in a mix, batched or not.
+// No kernel combines values across requests (the batch kernel's work counter
+// only decides which wave computes an item), so a result is bit-identical
+// alone and in a mix, batched or not.
 #include "tc_common.h"
 
 #include "../../include/treecodes2d.h"

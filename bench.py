@@ -315,6 +315,57 @@ def tune_serving(cache_path):
     return done
 
 
+def batch_resident_case(k=16, n=1 << 14, m=1 << 11, reps=20, order=10, leaf=128):
+    """k small potential requests against resident trees (one context each, FP32): one at a
+    time (each call synchronised), all issued on their own streams then synchronised, and as one
+    tc_potential_batch_resident launch of the persistent cross-request kernel.  Median wall times
+    over reps; the batched results are checked against the one-at-a-time ones."""
+    rng = np.random.default_rng(77)
+    ctxs, reqs = [], []
+    try:
+        for i in range(k):
+            c = pkg.Context(precision=1, pot_order=order, pot_leaf=leaf)
+            ctxs.append(c)
+            xs, ys, vs, _, _ = workloads.uniform_potential(n, seed=500 + i)
+            c.potential(0.5, xs, ys, vs, xs[:4], ys[:4])
+            xt, yt = rng.uniform(0, 1, m), rng.uniform(0, 1, m)
+            reqs.append((c, 0.5, c.to_device(xt), c.to_device(yt), c.empty(m)))
+        outs = [c.empty(m) for c in ctxs]
+
+        def one_by_one():
+            for (c, th, dxt, dyt, _), o in zip(reqs, outs):
+                c.potential_resident(th, dxt, dyt, out=o, device=True)
+                c.synchronize()
+
+        def overlapped():
+            for (c, th, dxt, dyt, _), o in zip(reqs, outs):
+                c.potential_resident(th, dxt, dyt, out=o, device=True)
+            for c in ctxs:
+                c.synchronize()
+
+        def batched():
+            pkg.potential_batch_resident(reqs)
+
+        res = {}
+        for name, f in (("one_by_one", one_by_one), ("own_streams", overlapped), ("batched", batched)):
+            f()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                f()
+                ts.append(time.perf_counter() - t0)
+            res[name + "_ms"] = 1e3 * float(np.median(ts))
+        one_by_one()
+        batched()
+        same = all(np.array_equal(r[4].numpy(), o.numpy()) for r, o in zip(reqs, outs))
+        res.update(requests=k, sources=n, targets_each=m, order=order, leaf=leaf, precision="f32",
+                   results_identical=bool(same))
+        return res
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def serve_stream(nreq, rate_hz, slots, budget_gib, precision=1, seed=1454, tuned=True, batch=False, **mix):
     """Replay the served mixed-size stream: Poisson arrivals, 70% potential / 30% force,
     25% of requests reuse the previous request's sources (the server keeps the tree).
@@ -680,6 +731,10 @@ def main():
             log("  burst of 3000 small requests: makespan %.3f s unbatched, %.3f s batched" % (
                 burst["unbatched"]["makespan_s"], burst["batched"]["makespan_s"]))
             serving = {"unbatched": serving, "batched": b, "burst_3000_small": burst}
+            br = batch_resident_case()
+            log("  16 small requests on resident trees: %.3f ms one by one, %.3f ms on own streams, %.3f ms "
+                "in one batched launch" % (br["one_by_one_ms"], br["own_streams_ms"], br["batched_ms"]))
+            serving["batch_resident_16"] = br
 
     res = {
         "metric": "Mtargets/s at rel err <=1e-6, FP32-bulk vs FP64 plain",

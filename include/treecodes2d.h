@@ -158,6 +158,20 @@ TC_API int tc_potential_resident(tc_context *ctx, double theta, const double *xt
 TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x0s, const double *y0s,
                                   const double *hs, int nblocks, double *xr, double *yr, int device_ptrs);
 
+/* Several potential requests in one launch of the cross-request traversal.  No
+ * counterpart in the reference (whose calls are one-shot, treecode-potential.cpp:99-105):
+ * request i evaluates the tree resident in ctxs[i] (left by its last tc_potential call)
+ * at the m[i] targets xt[i], yt[i] with theta[i], into out[i]; all arrays are device
+ * pointers.  The k <= 64 contexts must be distinct, on one GPU, with resident potential
+ * trees of one expansion order and one precision (their leaf capacities may differ).
+ * A persistent grid drains the requests' waves of 64 targets from one work counter;
+ * each result equals tc_potential_resident's for that request bit for bit.  Runs on
+ * ctxs[0]'s stream through a graph per (order, precision, size bucket) and returns
+ * with every out[i] written; no other call may use these contexts meanwhile. */
+TC_API int tc_potential_batch_resident(tc_context *const *ctxs, int k, const double *theta,
+                                       const double *const *xt, const double *const *yt, const int *m,
+                                       double *const *out);
+
 /* Times of the last call's phases (requires TC_OPT_PROFILE), counters[8] (requires
  * TC_OPT_COUNTERS): {target-source pairs evaluated directly, (target, node) multipole
  * interactions (the accepted pairs, equal in both precisions), (brick, node) multipole-to-local

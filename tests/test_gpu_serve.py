@@ -259,3 +259,58 @@ def test_peak_device_memory_within_budget(pkg, wl):
     assert st["estimate_misses"] == 0, msg
     assert st["peak_bytes"] <= budget * 2 ** 30, msg
     assert used <= budget * 2 ** 30, msg
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_resident_equals_single(pkg, wl, precision):
+    # tc_potential_batch_resident: several contexts' resident trees evaluated in one
+    # launch of the persistent cross-request kernel, each result bit-identical to the
+    # context's own tc_potential_resident; repeated (graph replay) after an eager solve
+    rng = np.random.default_rng(71)
+    leaves = (64, 128, 192, 512, 96)
+    thetas = (0.3, 0.5, 0.7, 0.5, 0.4)
+    sizes = (1, 63, 1000, 20000, 70000)
+    ctxs, reqs = [], []
+    try:
+        for i, (leaf, th, m) in enumerate(zip(leaves, thetas, sizes)):
+            c = pkg.Context(precision=precision, pot_order=10, pot_leaf=leaf)
+            ctxs.append(c)
+            n = int(rng.integers(3000, 60000))
+            if i % 2:
+                xs, ys, vs, _, _ = wl.uniform_potential(n, seed=100 + i)
+            else:
+                xs, ys, vs, _, _ = wl.mixed_plummer_potential(n, seed=100 + i)
+            c.potential(th, xs, ys, vs, xs[:5], ys[:5])  # leaves the tree resident
+            xt, yt = rng.uniform(-0.2, 1.2, m), rng.uniform(-0.2, 1.2, m)
+            reqs.append((c, th, c.to_device(xt), c.to_device(yt), c.empty(m)))
+        alone = []
+        for c, th, dxt, dyt, _ in reqs:
+            o = c.empty(len(dxt))
+            c.potential_resident(th, dxt, dyt, out=o, device=True)
+            c.synchronize()
+            alone.append(o.numpy())
+        for rep in range(2):
+            for c, th, dxt, dyt, out in reqs:
+                out.upload(np.full(len(dxt), np.nan))
+            pkg.potential_batch_resident(reqs)
+            for (c, th, dxt, dyt, out), ref in zip(reqs, alone):
+                assert np.array_equal(out.numpy(), ref)
+            ctxs[0].potential_resident(0.5, reqs[0][2], reqs[0][3], out=ctxs[0].empty(1), device=True)
+        # refused: a context twice, more than 64 requests, trees of different orders
+        with pytest.raises(pkg.TreecodeError, match="twice"):
+            pkg.potential_batch_resident([reqs[0], reqs[0]])
+        with pytest.raises(pkg.TreecodeError, match="at most 64"):
+            pkg.potential_batch_resident([reqs[0]] * 65)
+        c6 = pkg.Context(precision=precision, pot_order=12, pot_leaf=64)
+        ctxs.append(c6)
+        xs, ys, vs, _, _ = wl.uniform_potential(5000, seed=9)
+        c6.potential(0.5, xs, ys, vs, xs[:5], ys[:5])
+        with pytest.raises(pkg.TreecodeError, match="orders or precisions"):
+            pkg.potential_batch_resident([reqs[1], (c6, 0.5, reqs[1][2], reqs[1][3], reqs[1][4])])
+        c7 = pkg.Context(precision=precision, pot_order=10, pot_leaf=64)
+        ctxs.append(c7)
+        with pytest.raises(pkg.TreecodeError, match="no resident tree"):
+            pkg.potential_batch_resident([reqs[1], (c7, 0.5, reqs[1][2], reqs[1][3], reqs[1][4])])
+    finally:
+        for c in ctxs:
+            c.close()
