@@ -535,6 +535,8 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             TC_HIP(hipMemcpyAsync(bad, tc_targets_bad_flag(c->tws, m), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
             copy_out_all(c, dout, {{out, m}});
+            if (*bad & 2)
+                throw TcError(-9, "treecode_potential: the target sort did not complete (a radix pass timed out)");
             if (*bad)
                 throw TcError(-4, "treecode_potential: non-finite target coordinate");
         }
@@ -723,6 +725,8 @@ int run_force_targets(tc_context *ctx, double theta, const double *xs, const dou
             TC_HIP(hipMemcpyAsync(bad, tc_targets_bad_flag(c->ws, m), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
             copy_out_all(c, dxr, {{xr, m}, {yr, m}});
+            if (*bad & 2)
+                throw TcError(-9, "treecode_force: the target sort did not complete (a radix pass timed out)");
             if (*bad)
                 throw TcError(-4, "treecode_force: non-finite target coordinate");
         }

@@ -136,11 +136,14 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
 
 // ------------------------------------------------------------------ bbox
 // sort-sources.cpp:44-82 with upward-common.ispc:126-150's accumulators:
-// min starts at 1e13, max at 0.
+// min starts at 1e13, max at 0.  Also zeroes zero[0 .. nzero) (the one-launch
+// radix passes' counters, used after this kernel).
 __global__ __launch_bounds__(256) void k_bbox_partial(const double *x, const double *y, int n, double4 *part,
-                                                       uint32_t *bad)
+                                                       uint32_t *bad, uint32_t *zero, int nzero)
 {
     __shared__ double4 s[4];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nzero; i += gridDim.x * 256)
+        zero[i] = 0u;
     double xmi = 1e13, ymi = 1e13, xma = 0, yma = 0;
     bool nonfinite = false;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -347,12 +350,14 @@ TC_DEV unsigned long long wave_digit_peers(uint32_t d, bool ok)
     return tc_ballot(ok) & ~diff;
 }
 
-// offs, rowtot: k_rs_rowscan's per-digit tile prefixes and digit totals;
-// with offs == nullptr the block derives its starts from the tile histograms
-// `hist` itself (few tiles: the whole table is a few KiB).
-__global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                                                       uint32_t *vout, int n, int shift, const uint32_t *offs,
-                                                       const uint32_t *rowtot, const uint32_t *hist, int ntiles)
+// One tile of a radix pass: the tile's keys are ranked stably in LDS and
+// written out as contiguous digit runs.  start(tot, s_w) is called by every
+// thread of the block after the ranking (it may use the block scratch s_w for
+// collective scans); for digit threadIdx.x, of which the tile holds tot keys,
+// it returns the global position of the tile's first key of that digit.
+template <class Start>
+TC_DEV void rs_tile(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int n, int shift,
+                    int tile, Start &&start)
 {
     constexpr int NW = RS_NT / 64, WITEMS = 64 * RS_ITEMS;  // waves per tile, keys per wave
     __shared__ uint32_t s_k[RS_TILE];
@@ -363,24 +368,10 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     __shared__ uint32_t s_w[NW + 1];
 
     const int tid = threadIdx.x, lane = tc_lane(), wid = tid >> 6;
-    const int base = blockIdx.x * RS_TILE;
+    const int base = tile * RS_TILE;
     const int cnt = min(RS_TILE, n - base);
     for (int d = lane; d < RS_BINS; d += 64)
         s_wrun[wid][d] = 0;
-    if (offs) {
-        // digit tid: keys of smaller digits (scan of the digit totals) + this
-        // digit in earlier tiles
-        s_gbase[tid] = block_scan_excl<RS_NT>(rowtot[tid], s_w, nullptr) + offs[(size_t)tid * ntiles + blockIdx.x];
-    } else {
-        // digit tid: keys of smaller digits (all tiles) + this digit in earlier tiles
-        uint32_t tot = 0, pre = 0;
-        for (int t = 0; t < ntiles; ++t) {
-            const uint32_t c = hist[tid * ntiles + t];
-            pre += t < (int)blockIdx.x ? c : 0u;
-            tot += c;
-        }
-        s_gbase[tid] = block_scan_excl<RS_NT>(tot, s_w, nullptr) + pre;
-    }
     tc_wave_sync();
 
     // Each wave ranks its own contiguous 1024 keys (item j of lane l is key
@@ -411,15 +402,18 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
     }
     __syncthreads();
     {
-        // digit tid: tile offset = exclusive scan over digits of the tile
-        // counts; wave w's keys of that digit start after waves 0..w-1
+        // digit tid: its global start, then the tile offset = exclusive scan
+        // over digits of the tile counts; wave w's keys of that digit start
+        // after waves 0..w-1
         uint32_t c[NW], tot = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
             c[w] = s_wrun[w][tid];
             tot += c[w];
         }
+        const uint32_t g = start(tot, s_w);
         uint32_t off = block_scan_excl<RS_NT>(tot, s_w, nullptr);
+        s_gbase[tid] = g;
         s_tileoff[tid] = off;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
@@ -445,6 +439,123 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const
         kout[dst] = kk;
         vout[dst] = s_v[p];
     }
+}
+
+// A pass over more than RS_DIRECT_TILES tiles: offs, rowtot are
+// k_rs_rowscan's per-digit tile prefixes and digit totals.
+__global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                       uint32_t *vout, int n, int shift, const uint32_t *offs,
+                                                       const uint32_t *rowtot, int ntiles)
+{
+    rs_tile(kin, vin, kout, vout, n, shift, blockIdx.x, [&](uint32_t, uint32_t *s_w) {
+        // keys of smaller digits (scan of the digit totals) + this digit in earlier tiles
+        return block_scan_excl<RS_NT>(rowtot[threadIdx.x], s_w, nullptr) +
+               offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    });
+}
+
+// Sorts of at most RS_DIRECT_TILES tiles take each pass in one launch.  The
+// region (zeroed by k_bbox_partial before the keys) holds hist4[4][RS_BINS],
+// every pass's digit totals over all keys (k_keys_hist); for each pass the
+// tiles' published digit counts, flags[RS_DIRECT_TILES][RS_BINS]
+// (RS_PUB | count once published); and one ticket per pass.
+constexpr int LB_HIST = 4 * RS_BINS;
+constexpr int LB_PASS_FLAGS = RS_DIRECT_TILES * RS_BINS;
+constexpr int LB_WORDS = LB_HIST + 4 * LB_PASS_FLAGS + 64;
+constexpr uint32_t RS_PUB = 1u << 31;
+constexpr int RS_SPIN_MAX = 1 << 22;  // polls of one unpublished count before the pass gives up
+constexpr int RS_LB_BATCH = 16;       // published counts read per round trip
+
+// One pass of a sort of at most RS_DIRECT_TILES tiles.  Tiles are taken in
+// ticket order; a tile publishes its digit counts as soon as it has ranked its
+// keys and before it reads any other tile's, and reads only those of tiles
+// with smaller tickets, which have started before it and publish without
+// waiting: every wait ends.  Digit d of tile t starts at (keys of smaller
+// digits, from hist) + (digit d in tiles 0..t-1), exactly as with a scan of
+// the tile histograms.  A count that stays unpublished for RS_SPIN_MAX polls
+// sets bit 1 of *err (reported by the host as a failed sort) and is skipped.
+__global__ __launch_bounds__(RS_NT) void k_rs_pass_lb(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                       uint32_t *vout, int n, int shift, const uint32_t *hist,
+                                                       uint32_t *flags, uint32_t *ticket, uint32_t *err)
+{
+    __shared__ int s_tile;
+    if (threadIdx.x == 0)
+        s_tile = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    rs_tile(kin, vin, kout, vout, n, shift, tile, [&](uint32_t tot, uint32_t *s_w) {
+        const int d = threadIdx.x;
+        __hip_atomic_store(&flags[tile * RS_BINS + d], RS_PUB | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t pre = 0;
+        int t0 = 0;
+        while (t0 < tile) {
+            // the next counts, RS_LB_BATCH loads in flight at once
+            uint32_t c[RS_LB_BATCH];
+            const int nt = min(RS_LB_BATCH, tile - t0);
+#pragma unroll
+            for (int j = 0; j < RS_LB_BATCH; ++j)
+                c[j] = j < nt ? __hip_atomic_load(&flags[(t0 + j) * RS_BINS + d], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : RS_PUB;
+            int j = 0;
+            while (j < nt && (c[j] & RS_PUB)) {
+                pre += c[j] & ~RS_PUB;
+                ++j;
+            }
+            t0 += j;
+            if (j < nt) {
+                // tile t0 has not published yet: wait for it
+                uint32_t f = 0;
+                for (int spin = 0; spin < RS_SPIN_MAX; ++spin) {
+                    f = __hip_atomic_load(&flags[t0 * RS_BINS + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (f & RS_PUB)
+                        break;
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                if (!(f & RS_PUB))
+                    atomicOr(err, 2u);
+                pre += f & ~RS_PUB;
+                ++t0;
+            }
+        }
+        return block_scan_excl<RS_NT>(hist[d], s_w, nullptr) + pre;
+    });
+}
+
+// Keys as k_keys / k_keys_hilbert (RS_TILE per workgroup) for a sort of at
+// most RS_DIRECT_TILES tiles, plus the digit totals of all four passes, added
+// into hist4 (zeroed before) once per workgroup.
+template <bool HILBERT>
+__global__ __launch_bounds__(256) void k_keys_hist(const double *x, const double *y, const double *v, int n,
+                                                    const double *box, uint32_t *keys, double4 *packed,
+                                                    uint32_t *hist4)
+{
+    __shared__ uint32_t h[4 * RS_BINS];
+    for (int i = threadIdx.x; i < 4 * RS_BINS; i += 256)
+        h[i] = 0;
+    __syncthreads();
+    const double xmin = box[0], ymin = box[1], scale = box[3];
+    const int base = blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int i = base + j * 256 + threadIdx.x;
+        if (i < n) {
+            const double xi = x[i], yi = y[i];
+            const uint32_t qx = quantise(xi, xmin, scale), qy = quantise(yi, ymin, scale);
+            const uint32_t k = HILBERT ? hilbert15(qx, qy) : (tc_spread15(qx) | (tc_spread15(qy) << 1));
+            keys[i] = k;
+            if (!HILBERT && packed)
+                packed[i] = make_double4(xi, yi, v[i], 0.0);
+            atomicAdd(&h[k & 255u], 1u);
+            atomicAdd(&h[RS_BINS + ((k >> 8) & 255u)], 1u);
+            atomicAdd(&h[2 * RS_BINS + ((k >> 16) & 255u)], 1u);
+            atomicAdd(&h[3 * RS_BINS + (k >> 24)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4 * RS_BINS; i += 256)
+        if (h[i])
+            atomicAdd(&hist4[i], h[i]);
 }
 
 // ------------------------------------------------------------------ small sorts
@@ -1297,14 +1408,15 @@ void tc_upload_binomials(hipStream_t st)
     TC_HIP(hipStreamSynchronize(st));
 }
 
-void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st)
+void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st, uint32_t *zero,
+             int nzero)
 {
     const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
     ws.bbox_part.reserve(sizeof(double4) * (size_t)np + 256);
     double4 *part = ws.bbox_part.as<double4>();
     uint32_t *bad = reinterpret_cast<uint32_t *>(part + np);
     tc_fill_u32(bad, 0u, 1, st);
-    k_bbox_partial<<<np, 256, 0, st>>>(x, y, n, part, bad);
+    k_bbox_partial<<<np, 256, 0, st>>>(x, y, n, part, bad, zero, nzero);
     k_bbox_final<<<1, 64, 0, st>>>(part, np, box_d);
     TC_HIP(hipGetLastError());
 }
@@ -1334,13 +1446,20 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
         }
         return;
     }
-    tc_bbox(x, y, n, box_d, ws, st);
+    // up to RS_DIRECT_TILES tiles the radix passes take one launch each; their
+    // counters are zeroed by the bounding-box kernel
+    uint32_t *lb = nullptr;
+    if (tiles_of(n, RS_TILE) <= RS_DIRECT_TILES) {
+        ws.hist.reserve(sizeof(uint32_t) * LB_WORDS);
+        lb = ws.hist.as<uint32_t>();
+    }
+    tc_bbox(x, y, n, box_d, ws, st, lb, lb ? LB_WORDS : 0);
     double4 *packed = nullptr;
     if (src_out) {
         ws.packed.reserve(sizeof(double4) * (size_t)n);
         packed = ws.packed.as<double4>();
     }
-    tc_morton_sort(x, y, v, n, box_d, keys_out, perm_out, packed, ws, st, hilbert, ev);
+    tc_morton_sort(x, y, v, n, box_d, keys_out, perm_out, packed, ws, st, hilbert, ev, lb);
     if (src_out) {
         k_gather_src<<<tiles_of(n, 256), 256, 0, st>>>(packed, perm_out, n, src_out);
         TC_HIP(hipGetLastError());
@@ -1351,10 +1470,38 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
 
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
-                    bool hilbert, const hipEvent_t *ev)
+                    bool hilbert, const hipEvent_t *ev, uint32_t *lb)
 {
     ws.k2.reserve(sizeof(uint32_t) * (size_t)n);
     ws.v2.reserve(sizeof(uint32_t) * (size_t)n);
+    const int ntiles = tiles_of(n, RS_TILE);
+    // ping-pong out -> scratch -> out -> scratch -> out (four 8-bit passes over 30-bit keys);
+    // the first pass generates the index payload itself
+    uint32_t *kin[4] = {keys_out, ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>()};
+    uint32_t *vin[4] = {nullptr, ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>()};
+    uint32_t *kout[4] = {ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>(), keys_out};
+    uint32_t *vout[4] = {ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>(), perm_out};
+    if (lb) {
+        if (ntiles > RS_DIRECT_TILES)
+            throw TcError(-3, "tc_morton_sort: one-launch passes take at most 64 tiles");
+        // keys with all four passes' digit totals, then one launch per pass
+        uint32_t *hist4 = lb, *flags = lb + LB_HIST, *tickets = flags + 4 * LB_PASS_FLAGS;
+        uint32_t *err = const_cast<uint32_t *>(tc_bbox_bad_flag(ws, n));
+        if (hilbert)
+            k_keys_hist<true><<<ntiles, 256, 0, st>>>(x, y, nullptr, n, box_d, keys_out, nullptr, hist4);
+        else
+            k_keys_hist<false><<<ntiles, 256, 0, st>>>(x, y, v, n, box_d, keys_out, packed, hist4);
+        TC_HIP(hipGetLastError());
+        if (ev)
+            TC_HIP(hipEventRecord(ev[0], st));
+        for (int pass = 0; pass < 4; ++pass) {
+            k_rs_pass_lb<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, 8 * pass,
+                                                    hist4 + RS_BINS * pass, flags + LB_PASS_FLAGS * pass,
+                                                    tickets + pass, err);
+            TC_HIP(hipGetLastError());
+        }
+        return;
+    }
     if (hilbert)
         k_keys_hilbert<<<tiles_of(n, 256), 256, 0, st>>>(x, y, n, box_d, keys_out);
     else
@@ -1362,27 +1509,16 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
     TC_HIP(hipGetLastError());
     if (ev)
         TC_HIP(hipEventRecord(ev[0], st));
-    const int ntiles = tiles_of(n, RS_TILE);
     ws.hist.reserve(sizeof(uint32_t) * ((size_t)RS_BINS * ntiles * 2 + RS_BINS));
     uint32_t *hist = ws.hist.as<uint32_t>();
     uint32_t *offs = hist + (size_t)RS_BINS * ntiles;
     uint32_t *rowtot = offs + (size_t)RS_BINS * ntiles;
-    // ping-pong out -> scratch -> out -> scratch -> out (four 8-bit passes over 30-bit keys);
-    // the first pass generates the index payload itself
-    uint32_t *kin[4] = {keys_out, ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>()};
-    uint32_t *vin[4] = {nullptr, ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>()};
-    uint32_t *kout[4] = {ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>(), keys_out};
-    uint32_t *vout[4] = {ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>(), perm_out};
-    // up to RS_DIRECT_TILES tiles the scatter blocks read the histograms
-    // themselves and no scan is launched
-    const bool direct = ntiles <= RS_DIRECT_TILES;
     for (int pass = 0; pass < 4; ++pass) {
         const int shift = 8 * pass;
         k_rs_hist<<<ntiles, RS_NT, 0, st>>>(kin[pass], n, shift, hist, ntiles);
-        if (!direct)
-            k_rs_rowscan<<<RS_BINS, RS_NT, 0, st>>>(hist, ntiles, offs, rowtot);
-        k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift,
-                                                direct ? nullptr : offs, rowtot, hist, ntiles);
+        k_rs_rowscan<<<RS_BINS, RS_NT, 0, st>>>(hist, ntiles, offs, rowtot);
+        k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift, offs, rowtot,
+                                                ntiles);
         TC_HIP(hipGetLastError());
     }
 }
@@ -1469,6 +1605,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     int hinfo[TC_LMAX + 2];
     std::memcpy(hinfo, hp, sizeof(hinfo));
     std::memcpy(t.box, hp + sizeof(int) * INFO_BOX, sizeof(t.box));
+    if (hinfo[TC_LMAX + 1] & 2)
+        throw TcError(-9, "treecode: the source sort did not complete (a radix pass timed out)");
     if (hinfo[TC_LMAX + 1])
         throw TcError(-4, "treecode: non-finite source coordinate");
     const int T = hinfo[TC_LMAX];
