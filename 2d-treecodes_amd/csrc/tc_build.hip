@@ -136,10 +136,12 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
 
 // ------------------------------------------------------------------ bbox
 // sort-sources.cpp:44-82 with upward-common.ispc:126-150's accumulators:
-// min starts at 1e13, max at 0.  Also zeroes zero[0 .. nzero) (the one-launch
-// radix passes' counters, used after this kernel).
+// min starts at 1e13, max at 0.  Non-finite coordinates are ORed into *bad
+// (cleared before), or with pbad recorded per workgroup in pbad[blockIdx.x].
+// Also zeroes zero[0 .. nzero) (the one-launch radix passes' counters, used
+// after this kernel).
 __global__ __launch_bounds__(256) void k_bbox_partial(const double *x, const double *y, int n, double4 *part,
-                                                       uint32_t *bad, uint32_t *zero, int nzero)
+                                                       uint32_t *bad, uint32_t *pbad, uint32_t *zero, int nzero)
 {
     __shared__ double4 s[4];
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nzero; i += gridDim.x * 256)
@@ -162,8 +164,16 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double *x, const dou
         o = __shfl_xor(xma, off, 64); xma = xma < o ? o : xma;
         o = __shfl_xor(yma, off, 64); yma = yma < o ? o : yma;
     }
-    if (tc_ballot(nonfinite) && tc_lane() == 0)
-        atomicOr(bad, 1u);
+    __shared__ uint32_t s_bad;
+    if (threadIdx.x == 0)
+        s_bad = 0u;
+    __syncthreads();
+    if (tc_ballot(nonfinite) && tc_lane() == 0) {
+        if (pbad)
+            s_bad = 1u;
+        else
+            atomicOr(bad, 1u);
+    }
     const int wid = threadIdx.x >> 6;
     if (tc_lane() == 0)
         s[wid] = make_double4(xmi, xma, ymi, yma);
@@ -177,13 +187,17 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double *x, const dou
             r.w = r.w < s[w].w ? s[w].w : r.w;
         }
         part[blockIdx.x] = r;
+        if (pbad)
+            pbad[blockIdx.x] = s_bad;
     }
 }
 
-__global__ __launch_bounds__(64) void k_bbox_final(const double4 *part, int np, double *box)
+// The box from the partials (one wave): min / max over part[0 .. np), then
+// the reference's padded extent.  box[4] = {xmin, ymin, ext, 2^15 / ext}.
+TC_DEV void bbox_finish(const double4 *part, int np, double *box)
 {
     double xmi = 1e13, xma = 0, ymi = 1e13, yma = 0;
-    for (int i = threadIdx.x; i < np; i += 64) {
+    for (int i = tc_lane(); i < np; i += 64) {
         const double4 r = part[i];
         xmi = r.x < xmi ? r.x : xmi;
         xma = xma < r.y ? r.y : xma;
@@ -198,7 +212,7 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double4 *part, int np, 
         o = __shfl_xor(xma, off, 64); xma = xma < o ? o : xma;
         o = __shfl_xor(yma, off, 64); yma = yma < o ? o : yma;
     }
-    if (threadIdx.x == 0) {
+    if (tc_lane() == 0) {
         const double ext0 = xma - xmi, ext1 = yma - ymi;
         const double eps = 10000 * DBL_EPSILON;
         double ext = (ext0 < ext1 ? ext1 : ext0) * (1 + 2 * eps);
@@ -209,6 +223,11 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double4 *part, int np, 
         box[2] = ext;
         box[3] = (1.0 / ext) * (double)(1 << TC_LMAX);
     }
+}
+
+__global__ __launch_bounds__(64) void k_bbox_final(const double4 *part, int np, double *box)
+{
+    bbox_finish(part, np, box);
 }
 
 // ------------------------------------------------------------------ keys
@@ -524,17 +543,35 @@ __global__ __launch_bounds__(RS_NT) void k_rs_pass_lb(const uint32_t *kin, const
 
 // Keys as k_keys / k_keys_hilbert (RS_TILE per workgroup) for a sort of at
 // most RS_DIRECT_TILES tiles, plus the digit totals of all four passes, added
-// into hist4 (zeroed before) once per workgroup.
+// into hist4 (zeroed before) once per workgroup.  Each workgroup finishes the
+// bounding box from k_bbox_partial's np partials and per-workgroup flags
+// itself (as k_bbox_final would; workgroup 0 writes it and the non-finite flag
+// to box_out and bad_out).
 template <bool HILBERT>
 __global__ __launch_bounds__(256) void k_keys_hist(const double *x, const double *y, const double *v, int n,
-                                                    const double *box, uint32_t *keys, double4 *packed,
-                                                    uint32_t *hist4)
+                                                    const double4 *part, const uint32_t *pbad, int np,
+                                                    double *box_out, uint32_t *bad_out, uint32_t *keys,
+                                                    double4 *packed, uint32_t *hist4)
 {
     __shared__ uint32_t h[4 * RS_BINS];
+    __shared__ double s_box[4];
     for (int i = threadIdx.x; i < 4 * RS_BINS; i += 256)
         h[i] = 0;
+    if (threadIdx.x < 64)
+        bbox_finish(part, np, s_box);
     __syncthreads();
-    const double xmin = box[0], ymin = box[1], scale = box[3];
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        uint32_t b = 0;
+        for (int i = threadIdx.x; i < np; i += 64)
+            b |= pbad[i];
+        b = tc_ballot(b != 0) ? 1u : 0u;
+        if (threadIdx.x == 0) {
+            for (int q = 0; q < 4; ++q)
+                box_out[q] = s_box[q];
+            *bad_out = b;
+        }
+    }
+    const double xmin = s_box[0], ymin = s_box[1], scale = s_box[3];
     const int base = blockIdx.x * RS_TILE;
 #pragma unroll 4
     for (int j = 0; j < RS_ITEMS; ++j) {
@@ -806,6 +843,48 @@ __global__ __launch_bounds__(256) void k_int_count(const uint32_t *keys, int n, 
 // count, non-finite flag, then (at int offset INFO_BOX) the bounding box.
 constexpr int INFO_BOX = 20;
 constexpr size_t INFO_BYTES = sizeof(int) * INFO_BOX + sizeof(double) * 4;
+
+// The census scan in one workgroup (k_scan_single over the level-major
+// counts, TC_LMAX rows of nblk) with k_level_starts' record: the thread that
+// writes the first prefix of a level also writes that level's start.
+__global__ __launch_bounds__(SCAN_NT) void k_census_single(const uint32_t *in, uint32_t *out, int nblk,
+                                                            uint32_t *total, const uint32_t *bad, const double *box,
+                                                            int *info)
+{
+    __shared__ uint32_t s_w[SCAN_NT / 64 + 1];
+    const int n = TC_LMAX * nblk;
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += SCAN_CHUNK) {
+        uint32_t v[SCAN_ITEMS];
+        uint32_t sum = 0;
+        const int i0 = base + threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            v[j] = (i0 + j < n) ? in[i0 + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint32_t pre = block_scan_excl<SCAN_NT>(sum, s_w, &tot) + carry;
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            const int i = i0 + j;
+            if (i < n) {
+                out[i] = pre;
+                if (i % nblk == 0)
+                    info[i / nblk] = (int)pre;
+            }
+            pre += v[j];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        *total = carry;
+        info[TC_LMAX] = (int)carry;
+        info[TC_LMAX + 1] = (int)*bad;
+    }
+    if (threadIdx.x >= 32 && threadIdx.x < 36)
+        reinterpret_cast<double *>(info + INFO_BOX)[threadIdx.x - 32] = box[threadIdx.x - 32];
+}
 
 __global__ void k_level_starts(const uint32_t *offs, int nblk, const uint32_t *total, const uint32_t *bad,
                                const double *box, int *info)
@@ -1416,7 +1495,7 @@ void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &
     double4 *part = ws.bbox_part.as<double4>();
     uint32_t *bad = reinterpret_cast<uint32_t *>(part + np);
     tc_fill_u32(bad, 0u, 1, st);
-    k_bbox_partial<<<np, 256, 0, st>>>(x, y, n, part, bad, zero, nzero);
+    k_bbox_partial<<<np, 256, 0, st>>>(x, y, n, part, bad, nullptr, zero, nzero);
     k_bbox_final<<<1, 64, 0, st>>>(part, np, box_d);
     TC_HIP(hipGetLastError());
 }
@@ -1425,6 +1504,25 @@ static const uint32_t *tc_bbox_bad_flag(TcScratch &ws, int n)
 {
     const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
     return reinterpret_cast<const uint32_t *>(ws.bbox_part.as<double4>() + np);
+}
+
+// Bounding-box partials of a sort with one-launch passes: np partials, the
+// non-finite flag where tc_bbox_bad_flag finds it, per-workgroup flags after it.
+struct LbBox {
+    double4 *part;
+    uint32_t *bad, *pbad;
+    int np;
+};
+static LbBox lb_box(TcScratch &ws, int n)
+{
+    const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
+    ws.bbox_part.reserve(sizeof(double4) * (size_t)np + 256 + sizeof(uint32_t) * (size_t)np);
+    LbBox b;
+    b.part = ws.bbox_part.as<double4>();
+    b.bad = reinterpret_cast<uint32_t *>(b.part + np);
+    b.pbad = b.bad + 64;
+    b.np = np;
+    return b;
 }
 
 void tc_sort_points(const double *x, const double *y, const double *v, int n, double *box_d, uint32_t *keys_out,
@@ -1447,13 +1545,18 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
         return;
     }
     // up to RS_DIRECT_TILES tiles the radix passes take one launch each; their
-    // counters are zeroed by the bounding-box kernel
+    // counters are zeroed by the bounding-box kernel, whose partials the key
+    // kernel finishes
     uint32_t *lb = nullptr;
     if (tiles_of(n, RS_TILE) <= RS_DIRECT_TILES) {
         ws.hist.reserve(sizeof(uint32_t) * LB_WORDS);
         lb = ws.hist.as<uint32_t>();
+        const LbBox b = lb_box(ws, n);
+        k_bbox_partial<<<b.np, 256, 0, st>>>(x, y, n, b.part, nullptr, b.pbad, lb, LB_WORDS);
+        TC_HIP(hipGetLastError());
+    } else {
+        tc_bbox(x, y, n, box_d, ws, st);
     }
-    tc_bbox(x, y, n, box_d, ws, st, lb, lb ? LB_WORDS : 0);
     double4 *packed = nullptr;
     if (src_out) {
         ws.packed.reserve(sizeof(double4) * (size_t)n);
@@ -1484,13 +1587,17 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
     if (lb) {
         if (ntiles > RS_DIRECT_TILES)
             throw TcError(-3, "tc_morton_sort: one-launch passes take at most 64 tiles");
-        // keys with all four passes' digit totals, then one launch per pass
+        // the box and keys with all four passes' digit totals, then one launch
+        // per pass (the bounding-box partials were made by the caller)
         uint32_t *hist4 = lb, *flags = lb + LB_HIST, *tickets = flags + 4 * LB_PASS_FLAGS;
-        uint32_t *err = const_cast<uint32_t *>(tc_bbox_bad_flag(ws, n));
+        const LbBox b = lb_box(ws, n);
+        uint32_t *err = b.bad;
         if (hilbert)
-            k_keys_hist<true><<<ntiles, 256, 0, st>>>(x, y, nullptr, n, box_d, keys_out, nullptr, hist4);
+            k_keys_hist<true><<<ntiles, 256, 0, st>>>(x, y, nullptr, n, b.part, b.pbad, b.np, const_cast<double *>(box_d),
+                                                      b.bad, keys_out, nullptr, hist4);
         else
-            k_keys_hist<false><<<ntiles, 256, 0, st>>>(x, y, v, n, box_d, keys_out, packed, hist4);
+            k_keys_hist<false><<<ntiles, 256, 0, st>>>(x, y, v, n, b.part, b.pbad, b.np, const_cast<double *>(box_d),
+                                                       b.bad, keys_out, packed, hist4);
         TC_HIP(hipGetLastError());
         if (ev)
             TC_HIP(hipEventRecord(ev[0], st));
@@ -1595,8 +1702,12 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     ws.info.reserve(sizeof(int) * 32);
     int *info = ws.info.as<int>();
     k_int_count<<<nblk, 256, 0, st>>>(keys, n, leafcap, cnt, nblk);
-    tc_scan_exclusive_u32(cnt, offs, TC_LMAX * nblk, total, ws, st);
-    k_level_starts<<<1, 64, 0, st>>>(offs, nblk, total, tc_bbox_bad_flag(ws, n), box, info);
+    if (TC_LMAX * nblk <= 4 * SCAN_CHUNK) {
+        k_census_single<<<1, SCAN_NT, 0, st>>>(cnt, offs, nblk, total, tc_bbox_bad_flag(ws, n), box, info);
+    } else {
+        tc_scan_exclusive_u32(cnt, offs, TC_LMAX * nblk, total, ws, st);
+        k_level_starts<<<1, 64, 0, st>>>(offs, nblk, total, tc_bbox_bad_flag(ws, n), box, info);
+    }
     TC_HIP(hipGetLastError());
     static_assert(INFO_BYTES <= sizeof(int) * 32 && INFO_BYTES <= TcScratch::PINNED_BYTES, "census record size");
     char *hp = static_cast<char *>(ws.host_pinned());

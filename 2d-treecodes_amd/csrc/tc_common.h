@@ -351,7 +351,9 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
                     uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert,
                     const hipEvent_t *ev = nullptr);
 // lb: for n of at most 64 tiles of 4096, the counters of the one-launch radix
-// passes, zeroed beforehand (tc_bbox's zero region); null: multi-launch passes.
+// passes, zeroed beforehand by the bounding-box partial kernel, whose partials
+// the key kernel then finishes into box_d (as tc_sort_points does); null:
+// tc_bbox has written box_d and the passes take several launches.
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
                     bool hilbert = false, const hipEvent_t *ev = nullptr, uint32_t *lb = nullptr);

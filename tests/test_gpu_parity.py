@@ -389,15 +389,18 @@ def test_empty_and_error_paths(pkg, wl, ctx64):
         assert np.array_equal(ctx64.potential(0.5, xs, ys, vs, xt, yt), before), n
     xs[10] = 0.5
     ctx64.potential(0.5, xs, ys, vs, xt, yt)
-    # non-finite targets are reported too (flagged on the device by the target sort)
-    bad = yt.copy()
-    bad[777] = np.inf
-    with pytest.raises(pkg.TreecodeError):
-        ctx64.potential(0.5, xs, ys, vs, xt, bad)
-    with pytest.raises(pkg.TreecodeError):
-        ctx64.force_targets(0.5, xs, ys, vs, xt, bad)
-    good = ctx64.potential(0.5, xs, ys, vs, xt, yt)
-    assert np.all(np.isfinite(good))
+    # non-finite targets are reported too (flagged on the device by the target sort:
+    # the one-workgroup sort for 1000 targets, the one-launch passes for 40,000)
+    for n in (1000, 40_000):
+        xs, ys, vs, xt, yt = wl.uniform_potential(n, seed=16)
+        bad = yt.copy()
+        bad[777] = np.inf
+        with pytest.raises(pkg.TreecodeError, match="non-finite"):
+            ctx64.potential(0.5, xs, ys, vs, xt, bad)
+        with pytest.raises(pkg.TreecodeError, match="non-finite"):
+            ctx64.force_targets(0.5, xs, ys, vs, xt, bad)
+        good = ctx64.potential(0.5, xs, ys, vs, xt, yt)
+        assert np.all(np.isfinite(good))
 
 
 def test_failed_build_leaves_no_resident_tree(pkg, wl):
