@@ -341,7 +341,10 @@ struct ExtTree {
     const std::function<void(const TcPotBatchItem &, hipStream_t)> *eval = nullptr;
 };
 
-void build(tc_context *c, const double *xs, const double *ys, const double *vs, int n, int kind, const ExtTree &ext)
+// at_census (optional): work to queue once the build's census readback is
+// queued (see tc_build_tree).
+void build(tc_context *c, const double *xs, const double *ys, const double *vs, int n, int kind, const ExtTree &ext,
+           const std::function<void()> *at_census = nullptr)
 {
     if (ext.tree) {
         const int order = kind == TC_KIND_POTENTIAL ? c->pot_order : c->force_order;
@@ -349,7 +352,7 @@ void build(tc_context *c, const double *xs, const double *ys, const double *vs, 
         ext.tree->nnodes = 0;
         ext.tree->has_fp32 = false;
         tc_build_tree(*ext.tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws,
-                      c->stream);
+                      c->stream, nullptr, at_census);
         if (ext.ready)
             TC_HIP(hipEventRecord(ext.ready, c->stream));
         return;
@@ -365,7 +368,7 @@ void build(tc_context *c, const double *xs, const double *ys, const double *vs, 
     c->tree.has_fp32 = false;
     c->bev_valid = false;
     tc_build_tree(c->tree, xs, ys, vs, n, leaf, order, c->precision == 1, kind == TC_KIND_FORCE, c->ws, c->stream,
-                  c->profile ? c->bev : nullptr);
+                  c->profile ? c->bev : nullptr, at_census);
     c->bev_valid = c->profile;
     c->tree_kind = kind;
     c->tree_precision = c->precision;
@@ -484,21 +487,28 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
                     // the context stream
                     (void)hipEventRecord(c->join_ev, c->tstream);
                 });
-            } else {
+            }
+            // otherwise the target sort is queued once the source build's
+            // census is, so it runs while the host waits for the census
+            // readback (a gap in which a small solve's GPU had nothing to do)
+            bool tsorted = false;
+            const std::function<void()> tsort = [&] {
+                tsorted = true;
                 try {
                     tc_sort_targets(dxt, dyt, m, c->tws, c->tstream);
-                    TC_HIP(hipEventRecord(c->join_ev, c->tstream));
                 } catch (...) {
                     // whatever part of the sort was queued stays ordered before
-                    // later work on the context stream
+                    // later work on the context stream (the caller waits on join_ev)
                     (void)hipEventRecord(c->join_ev, c->tstream);
-                    (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
                     throw;
                 }
-            }
+                TC_HIP(hipEventRecord(c->join_ev, c->tstream));
+            };
             record(c, 0);
             try {
-                build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL, ext);
+                build(c, dxs, dys, dvs, n, TC_KIND_POTENTIAL, ext, overlap ? nullptr : &tsort);
+                if (!overlap && !tsorted)
+                    tsort();
             } catch (...) {
                 // later work on the context stream (the next call's copies
                 // into the io buffer) must not overtake the sort reading it

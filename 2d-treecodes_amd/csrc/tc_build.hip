@@ -1667,7 +1667,8 @@ static void upward_launch(TcTree &t, const uint32_t *child, const int2 *se, cons
 }
 
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
-                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev)
+                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev,
+                   const std::function<void()> *at_census)
 {
     if (n <= 0)
         throw TcError(-3, "tc_build_tree: no sources");
@@ -1712,6 +1713,8 @@ void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v,
     static_assert(INFO_BYTES <= sizeof(int) * 32 && INFO_BYTES <= TcScratch::PINNED_BYTES, "census record size");
     char *hp = static_cast<char *>(ws.host_pinned());
     TC_HIP(hipMemcpyAsync(hp, info, INFO_BYTES, hipMemcpyDeviceToHost, st));
+    if (at_census)
+        (*at_census)();  // work queued elsewhere fills the GPU while the host waits
     TC_HIP(hipStreamSynchronize(st));
     int hinfo[TC_LMAX + 2];
     std::memcpy(hinfo, hp, sizeof(hinfo));

@@ -361,8 +361,11 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
 // k b_k (force trees) instead of b_k (potential trees).
 // ev (optional, four events): recorded after the Morton keys, the sorted
 // records, the topology and the upward pass (the build's phase boundaries).
+// at_census (optional): called once the census readback is queued, before the
+// host waits for it (the caller can queue independent work on another stream).
 void tc_build_tree(TcTree &t, const double *x, const double *y, const double *v, int n, int leafcap, int order,
-                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev = nullptr);
+                   bool fp32, bool deriv32, TcScratch &ws, hipStream_t st, const hipEvent_t *ev = nullptr,
+                   const std::function<void()> *at_census = nullptr);
 void tc_sort_targets(const double *xt, const double *yt, int m, TcScratch &ws, hipStream_t st);
 // Several target sets of at most TC_SORT_SMALL_MAX targets each sorted in one
 // launch: tc_sort_targets_desc reserves ws's target buffers for m targets (as
