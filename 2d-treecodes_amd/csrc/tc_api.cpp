@@ -87,6 +87,10 @@ struct tc_context {
     TcScratch tws;
     hipStream_t tstream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // ends of the block ranges of a host-array force solve (their outputs are
+    // copied back while the next range is evaluated)
+    static constexpr int FORCE_RANGES = 4;
+    hipEvent_t range_ev[FORCE_RANGES] = {};
     DevBuf io, cnt;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // build phase boundaries: after the keys, the sorted records, the topology, the upward pass
@@ -198,17 +202,17 @@ void copy_in(TcStager &sg, hipStream_t st, const double *h, double *d, long long
     }
 }
 
-// Device -> host array; returns with the data in h (synchronises the stream).
-void copy_out(tc_context *c, const double *d, double *h, long long n)
+// Device -> host array through the staging buffers of sg on stream st;
+// returns with the data in h (synchronises st).
+void copy_out(TcStager &sg, hipStream_t st, const double *d, double *h, long long n)
 {
     const size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 0);
     if (bytes < STAGE_MIN) {
         if (bytes)
-            TC_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
-        TC_HIP(hipStreamSynchronize(c->stream));
+            TC_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
         return;
     }
-    TcStager &sg = c->stg;
     sg.ready();
     size_t pend_o[2] = {0, 0}, pend_len[2] = {0, 0};
     int b = 0;
@@ -223,16 +227,18 @@ void copy_out(tc_context *c, const double *d, double *h, long long n)
         const size_t len = std::min(STAGE_BYTES, bytes - o);
         drain(b);
         sg.wait(b);
-        TC_HIP(hipMemcpyAsync(sg.buf[b], (const char *)d + o, len, hipMemcpyDeviceToHost, c->stream));
-        TC_HIP(hipEventRecord(sg.ev[b], c->stream));
+        TC_HIP(hipMemcpyAsync(sg.buf[b], (const char *)d + o, len, hipMemcpyDeviceToHost, st));
+        TC_HIP(hipEventRecord(sg.ev[b], st));
         sg.busy[b] = true;
         pend_o[b] = o;
         pend_len[b] = len;
     }
     drain(b);
     drain(b ^ 1);
-    TC_HIP(hipStreamSynchronize(c->stream));
+    TC_HIP(hipStreamSynchronize(st));
 }
+
+void copy_out(tc_context *c, const double *d, double *h, long long n) { copy_out(c->stg, c->stream, d, h, n); }
 
 struct HostIn {
     const double *h;
@@ -681,14 +687,46 @@ int run_force(tc_context *ctx, double theta, const double *xs, const double *ys,
         if (rebuild)
             build(c, dxs, dys, dvs, n, TC_KIND_FORCE, ext);
         record(c, 1);
-        tc_eval_force(ext.tree ? *ext.tree : c->tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws,
-                      c->force_chunks, c->force_inline,
-                      c->stream,
-                      c->profile ? c->ev[2] : nullptr, c->profile ? c->ev[3] : nullptr);
+        const TcTree &tree = ext.tree ? *ext.tree : c->tree;
+        // Host outputs of at least FORCE_SPLIT_BLOCKS blocks: the blocks are
+        // evaluated in up to FORCE_RANGES ranges of at least half that (each a
+        // complete evaluation of its blocks, whose outputs are contiguous), and
+        // a range's forces are copied back on the second stream, through its
+        // own staging buffers, while the next range is evaluated.  Each range
+        // gets the chunk pool the whole call would have.
+        constexpr int FORCE_SPLIT_BLOCKS = 2048;
+        const int nranges = device_ptrs || nb < FORCE_SPLIT_BLOCKS
+                                ? 1
+                                : std::min<int>(tc_context::FORCE_RANGES, nb / (FORCE_SPLIT_BLOCKS / 2));
+        if (nranges <= 1) {
+            tc_eval_force(tree, theta, dx0, dy0, dh, nb, dxr, dyr, c->precision, cnt, c->ws, c->force_chunks,
+                          c->force_inline, c->stream, c->profile ? c->ev[2] : nullptr,
+                          c->profile ? c->ev[3] : nullptr);
+            c->ev_valid = c->profile;
+            c->eval_has_tsort = false;
+            if (!device_ptrs)
+                copy_out_all(c, dxr, {{xr, (long long)nt}, {yr, (long long)nt}});
+            return;
+        }
+        const int pool = c->force_chunks > 0 ? c->force_chunks : std::max(4096, std::min(nb * 16, 1 << 20) / 4);
+        const int per = (nb + nranges - 1) / nranges;
+        for (int k = 0, b0 = 0; b0 < nb; ++k, b0 += per) {
+            const int nbk = std::min(per, nb - b0);
+            const size_t o = (size_t)b0 * 1024;
+            tc_eval_force(tree, theta, dx0 + b0, dy0 + b0, dh + b0, nbk, dxr + o, dyr + o, c->precision, cnt, c->ws,
+                          pool, c->force_inline, c->stream, c->profile && k == 0 ? c->ev[2] : nullptr,
+                          c->profile && b0 + nbk == nb ? c->ev[3] : nullptr);
+            TC_HIP(hipEventRecord(c->range_ev[k], c->stream));
+        }
         c->ev_valid = c->profile;
         c->eval_has_tsort = false;
-        if (!device_ptrs)
-            copy_out_all(c, dxr, {{xr, (long long)nt}, {yr, (long long)nt}});
+        for (int k = 0, b0 = 0; b0 < nb; ++k, b0 += per) {
+            const long long len = (long long)std::min(per, nb - b0) * 1024;
+            const size_t o = (size_t)b0 * 1024;
+            TC_HIP(hipStreamWaitEvent(c->tstream, c->range_ev[k], 0));
+            copy_out(c->tstg, c->tstream, dxr + o, xr + o, len);
+            copy_out(c->tstg, c->tstream, dyr + o, yr + o, len);
+        }
     });
 }
 
@@ -901,6 +939,8 @@ TC_API tc_context *tc_context_create(int device)
             TC_HIP(hipEventCreate(&e));
         TC_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
         TC_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+        for (auto &e : c->range_ev)
+            TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         ensure_binomials(device, c->stream);
 #ifndef TREECODES2D_FLOAT_DROPIN
         // set-up of tc_potential_noalloc: the device address of the static
@@ -948,6 +988,9 @@ TC_API void tc_context_destroy(tc_context *c)
     c->stg.release();
     c->tstg.release();
     for (hipEvent_t e : {c->fork_ev, c->join_ev})
+        if (e)
+            (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->range_ev)
         if (e)
             (void)hipEventDestroy(e);
     if (c->tstream)

@@ -672,3 +672,21 @@ def test_profile_phases_and_cycles(pkg, wl):
     ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
     assert ctx.profile()["cycles"]["total"] == 0
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_force_host_ranges_equal_device(pkg, wl, precision):
+    # a host-array force solve of 4096 blocks is evaluated in block ranges whose
+    # outputs are copied back while the next range runs; the result equals the
+    # one-launch device-pointer solve bit for bit
+    xs, ys, vs, x0s, y0s, hs = wl.gaussian4_force(1 << 18)  # 64 x 64 = 4096 blocks
+    assert len(x0s) == 4096
+    ctx = pkg.Context(precision=precision, force_order=8, force_leaf=64)
+    hx, hy = ctx.force(0.5, xs, ys, vs, x0s, y0s, hs)
+    d = {k: ctx.to_device(v) for k, v in dict(xs=xs, ys=ys, vs=vs, x0s=x0s, y0s=y0s, hs=hs).items()}
+    xr, yr = ctx.empty(len(x0s) * 1024), ctx.empty(len(x0s) * 1024)
+    ctx.force(0.5, d["xs"], d["ys"], d["vs"], d["x0s"], d["y0s"], d["hs"], xr=xr, yr=yr, device=True)
+    ctx.synchronize()
+    assert np.array_equal(hx, xr.numpy()) and np.array_equal(hy, yr.numpy())
+    assert np.all(np.isfinite(hx)) and np.any(hx != 0)
+    ctx.close()
