@@ -703,7 +703,9 @@ def main():
             if (cfg, theta) == (args.config, args.theta):
                 continue
             key = "%s_theta%.1f" % (cfg, theta)
-            legs[key] = measure(cfg, theta, args, 0.0 if args.no_cpu else budget)
+            # the force leg also times the drop-in call with host arrays (how an MRAG solver calls it)
+            legs[key] = measure(cfg, theta, args, 0.0 if args.no_cpu else budget,
+                                host=cfg == "c5" and not args.no_host)
             log("  leg %s: %.1f Mtargets/s (%.3f ms), FP32/FP64 %.2fx" % (
                 key, legs[key]["value"], legs[key]["ms_per_step"], legs[key].get("speedup_fp32_vs_fp64", 0)))
     serving = None
