@@ -167,7 +167,8 @@ TC_API int tc_force_mrag_resident(tc_context *ctx, double theta, const double *x
  * A persistent grid drains the requests' waves of 64 targets from one work counter;
  * each result equals tc_potential_resident's for that request bit for bit.  Runs on
  * ctxs[0]'s stream through a graph per (order, precision, size bucket) and returns
- * with every out[i] written; no other call may use these contexts meanwhile. */
+ * with every out[i] written; no other call may use these contexts meanwhile.  The
+ * contexts' interaction counters and phase timers are not updated by this call. */
 TC_API int tc_potential_batch_resident(tc_context *const *ctxs, int k, const double *theta,
                                        const double *const *xt, const double *const *yt, const int *m,
                                        double *const *out);
