@@ -115,6 +115,8 @@ struct tc_context {
     DevBuf na_cnt;
 };
 
+thread_local std::vector<TcDeferredFree> *tc_defer_free = nullptr;
+
 namespace {
 
 thread_local std::string g_last_error;

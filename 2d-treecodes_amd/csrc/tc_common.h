@@ -19,6 +19,7 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #define TC_LMAX 15
 #define TC_MAXORDER 32
@@ -219,6 +220,17 @@ struct TcArena {
     }
 };
 
+// Set by a thread that synchronises every stream its work used before the
+// listed memory is freed (the request server's slot threads, per request):
+// DevBuf::reserve then hands a grown buffer's old memory (pointer, bytes) to
+// this list instead of freeing it, because hipFree waits for the whole device
+// and would stall the thread behind other threads' kernels.  Null: freed at once.
+struct TcDeferredFree {
+    void *ptr;
+    size_t bytes;
+};
+extern thread_local std::vector<TcDeferredFree> *tc_defer_free;
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
@@ -236,8 +248,12 @@ struct DevBuf {
             cap = b;
             return;
         }
-        if (ptr)
-            TC_HIP(hipFree(ptr));
+        if (ptr) {
+            if (tc_defer_free)
+                tc_defer_free->push_back({ptr, cap});
+            else
+                TC_HIP(hipFree(ptr));
+        }
         ptr = nullptr;
         cap = 0;
         b = (b + 4095) & ~size_t(4095);

@@ -13,9 +13,15 @@
 // whichever slot built it (the builder's stream records an event after the
 // build; readers wait on it on the GPU).  Trees are read-only once built.
 // At most 4 trees per slot stay cached: beyond, the least recently used idle
-// tree's buffers are kept for the next build instead of being freed (hipFree
-// waits for the whole device), so a steady stream allocates and frees
-// nothing once its buffers have grown.
+// tree's buffers are kept for the next build instead of being freed (one
+// spare tree per slot; a build takes the smallest spare that fits its
+// estimate, and a full spare pool keeps the larger trees), so a steady stream
+// allocates little once its buffers have grown.  What is freed in steady
+// state -- trees beyond the spares, the old memory of grown buffers -- is
+// freed by a reaper thread, not by the slot threads: hipFree waits for the
+// whole device and would stall a request behind other slots' kernels.  A
+// slot hands its request's old buffers over only after synchronising the
+// request's streams; until freed, their bytes count as held.
 //
 // HBM admission counts everything the server holds: the cached and spare
 // trees and each slot's grow-only buffers, plus an upper estimate of what each
@@ -173,6 +179,14 @@ struct tc_server {
     bool leader = false;
     TcBatchWs bws;
     long long batches = 0, batched = 0, merged = 0;
+    // frees off the request path (their bytes are in `freeing` until done)
+    std::mutex rmu;
+    std::condition_variable cv_reap;
+    std::vector<TcTree *> reap_trees;
+    std::vector<TcDeferredFree> reap_bufs;
+    size_t reap_bytes = 0;  // bytes of the queued trees and buffers
+    bool reap_stop = false;
+    std::thread reaper;
 };
 
 namespace {
@@ -299,9 +313,29 @@ bool make_room(tc_server *s, size_t need, std::vector<TcTree *> &victims, size_t
     return fits();
 }
 
+// Keeps the larger of a tree about to leave the cache and the smallest spare
+// when the spare pool is full: the other goes to victims.  Caller holds s->mu.
+void spare_or_victim(tc_server *s, TcTree *t, size_t b, std::vector<TcTree *> &victims, size_t &mine)
+{
+    if (s->spare.size() < s->ctx.size()) {
+        s->spare.push_back({t, b});
+        return;
+    }
+    auto small = std::min_element(s->spare.begin(), s->spare.end(),
+                                  [](const SpareTree &x, const SpareTree &y) { return x.bytes < y.bytes; });
+    if (small != s->spare.end() && small->bytes < b) {
+        std::swap(small->tree, t);
+        std::swap(small->bytes, b);
+    }
+    victims.push_back(t);
+    s->freeing += b;
+    mine += b;
+}
+
 // Keeps at most s->max_trees cached trees: the least recently used idle ones
-// beyond go to the spare buffers (at most one per slot; the rest are freed by
-// the caller after releasing the lock).  Caller holds s->mu.
+// beyond go to the spare buffers (at most one per slot, the larger trees
+// kept; the rest go to victims, which the caller hands to the reaper).
+// Caller holds s->mu.
 void enforce_cap(tc_server *s, std::vector<TcTree *> &victims, size_t &mine)
 {
     size_t built = 0;
@@ -315,31 +349,70 @@ void enforce_cap(tc_server *s, std::vector<TcTree *> &victims, size_t &mine)
         TcTree *t = drop_entry(s, victim);
         s->evictions++;
         --built;
-        if (s->spare.size() < s->ctx.size()) {
-            s->spare.push_back({t, b});
-        } else {
-            victims.push_back(t);
-            s->freeing += b;
-            mine += b;
-        }
+        spare_or_victim(s, t, b, victims, mine);
     }
 }
 
-// Frees trees evicted by the cache limit (s->mu released meanwhile; no
-// reservation counts on their bytes), then stops counting them.
-void free_victims(tc_server *s, std::unique_lock<std::mutex> &lk, std::vector<TcTree *> &victims, size_t &mine)
+// Hands trees evicted by the cache limit (no reservation counts on their
+// bytes, which stay in s->freeing) and a request's old buffers to the reaper.
+// Caller holds s->mu.
+void reap(tc_server *s, std::vector<TcTree *> &victims, size_t &mine, std::vector<TcDeferredFree> *bufs = nullptr)
 {
-    if (victims.empty())
+    size_t bb = 0;
+    if (bufs)
+        for (const TcDeferredFree &f : *bufs)
+            bb += f.bytes;
+    if (victims.empty() && !bb)
         return;
-    lk.unlock();
-    for (TcTree *t : victims)
-        free_tree(s, t);
-    lk.lock();
-    s->freeing -= mine;
+    s->freeing += bb;
+    {
+        std::lock_guard<std::mutex> rl(s->rmu);
+        s->reap_trees.insert(s->reap_trees.end(), victims.begin(), victims.end());
+        if (bufs)
+            s->reap_bufs.insert(s->reap_bufs.end(), bufs->begin(), bufs->end());
+        s->reap_bytes += mine + bb;
+    }
+    s->cv_reap.notify_one();
     victims.clear();
     mine = 0;
-    s->cv_work.notify_all();
+    if (bufs)
+        bufs->clear();
 }
+
+void free_victims(tc_server *s, std::unique_lock<std::mutex> &, std::vector<TcTree *> &victims, size_t &mine)
+{
+    reap(s, victims, mine);
+}
+
+// The reaper thread: frees what reap() queued, then stops counting its bytes.
+void reaper(tc_server *s)
+{
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> rl(s->rmu);
+    while (true) {
+        s->cv_reap.wait(rl, [&] { return s->reap_stop || !s->reap_trees.empty() || !s->reap_bufs.empty(); });
+        if (s->reap_trees.empty() && s->reap_bufs.empty() && s->reap_stop)
+            return;
+        std::vector<TcTree *> trees;
+        std::vector<TcDeferredFree> bufs;
+        trees.swap(s->reap_trees);
+        bufs.swap(s->reap_bufs);
+        const size_t bytes = s->reap_bytes;
+        s->reap_bytes = 0;
+        rl.unlock();
+        for (TcTree *t : trees)
+            free_tree(s, t);
+        for (const TcDeferredFree &f : bufs)
+            (void)hipFree(f.ptr);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->freeing -= bytes;
+            s->cv_work.notify_all();
+        }
+        rl.lock();
+    }
+}
+
 
 // Batch leader: takes all waiting traversals of the same order and precision
 // as the first one (up to TcBatchWs::MAXREQ).  One alone is launched as the
@@ -499,13 +572,22 @@ void worker(tc_server *s, int slot)
             hit->precision = pick->r.precision;
             hit->order = req_order(pick->r);
             hit->leaf = req_leaf(pick->r);
-            // the largest spare buffers, else new ones
+            // the smallest spare buffers that hold the tree's estimate, else
+            // the largest spare ones, else new ones
             if (!s->spare.empty()) {
-                auto big = std::max_element(s->spare.begin(), s->spare.end(),
-                                            [](const SpareTree &a, const SpareTree &b) { return a.bytes < b.bytes; });
-                hit->tree = big->tree;
-                hit->bytes = big->bytes;
-                s->spare.erase(big);
+                const size_t est = tree_estimate(pick->r);
+                auto fit = s->spare.end(), big = s->spare.begin();
+                for (auto it = s->spare.begin(); it != s->spare.end(); ++it) {
+                    if (it->bytes >= est && (fit == s->spare.end() || it->bytes < fit->bytes))
+                        fit = it;
+                    if (it->bytes > big->bytes)
+                        big = it;
+                }
+                if (fit == s->spare.end())
+                    fit = big;
+                hit->tree = fit->tree;
+                hit->bytes = fit->bytes;
+                s->spare.erase(fit);
             } else {
                 hit->tree = new TcTree();
             }
@@ -536,15 +618,21 @@ void worker(tc_server *s, int slot)
         mine = 0;
         s->peak = std::max(s->peak, held(s) + s->reserved);
         lk.unlock();
+        // grown buffers' old memory is freed by the reaper after this request
+        std::vector<TcDeferredFree> old_bufs;
         if (!hit->ready) {
             pick->status = -2;
             pick->err = "treecode server: event creation failed";
         } else {
+            tc_defer_free = &old_bufs;
             run_job(s, slot, pick, build);
+            tc_defer_free = nullptr;
         }
         const size_t cbytes = tc_srv_context_bytes(s->ctx[slot]);
-        if (build && pick->status != 0)
-            (void)tc_synchronize(s->ctx[slot]);  // a failed build's kernels end before its buffers are reused
+        // a failed build's kernels end before its buffers are reused, and no
+        // kernel of this request may still read a buffer handed to the reaper
+        if ((build && pick->status != 0) || !old_bufs.empty())
+            (void)tc_synchronize(s->ctx[slot]);
         const size_t tbytes = build ? tc_tree_bytes(*hit->tree) : 0;
         lk.lock();
         // the estimates admission relies on must bound what the request took
@@ -567,13 +655,7 @@ void worker(tc_server *s, int slot)
                 enforce_cap(s, victims, mine);
             } else {
                 TcTree *t = drop_entry(s, hit);
-                s->spare.push_back({t, tbytes});
-                if (s->spare.size() > s->ctx.size()) {
-                    victims.push_back(s->spare.front().tree);
-                    s->freeing += s->spare.front().bytes;
-                    mine += s->spare.front().bytes;
-                    s->spare.erase(s->spare.begin());
-                }
+                spare_or_victim(s, t, tbytes, victims, mine);
             }
         }
         s->peak = std::max(s->peak, held(s) + s->reserved);
@@ -589,7 +671,7 @@ void worker(tc_server *s, int slot)
         pick->done = true;
         s->cv_done.notify_all();
         s->cv_work.notify_all();
-        free_victims(s, lk, victims, mine);
+        reap(s, victims, mine, &old_bufs);
     }
 }
 
@@ -634,6 +716,7 @@ TC_API tc_server *tc_server_create_ex(int device, int nslots, double hbm_budget_
         delete s;
         return nullptr;
     }
+    s->reaper = std::thread(reaper, s);
     for (int i = 0; i < nslots; ++i)
         s->workers.emplace_back(worker, s, i);
     return s;
@@ -655,6 +738,14 @@ TC_API void tc_server_destroy(tc_server *s)
     s->cv_work.notify_all();
     for (auto &t : s->workers)
         t.join();
+    // the workers have handed everything over: the reaper frees it and ends
+    {
+        std::lock_guard<std::mutex> rl(s->rmu);
+        s->reap_stop = true;
+    }
+    s->cv_reap.notify_one();
+    if (s->reaper.joinable())
+        s->reaper.join();
     for (auto &kv : s->jobs)
         delete kv.second;
     (void)hipSetDevice(s->device);
