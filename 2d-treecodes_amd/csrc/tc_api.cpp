@@ -453,11 +453,12 @@ int run_potential(tc_context *ctx, double theta, const double *xs, const double 
             TC_HIP(hipStreamWaitEvent(c->stream, ext.ready, 0));
         const double *dxs = xs, *dys = ys, *dvs = vs, *dxt = xt, *dyt = yt;
         double *dout = out;
-        // host targets of at least two staging buffers are uploaded by a helper
-        // thread, through their own staging buffers on the second stream, while
-        // the source tree builds (the build waits on the host only for its
-        // census): the upload then overlaps the build instead of preceding it
-        const bool overlap = !device_ptrs && rebuild && 2 * arr_bytes(m) >= 2 * STAGE_BYTES;
+        // host target arrays of 4 MiB or more (512 Ki targets) are uploaded by a
+        // helper thread, through their own staging buffers on the second
+        // stream, while the source tree builds (the build waits on the host only
+        // for its census): the upload then overlaps the build instead of
+        // preceding it
+        const bool overlap = !device_ptrs && rebuild && arr_bytes(m) >= STAGE_BYTES / 4;
         if (!device_ptrs) {
             const size_t ns = rebuild ? (size_t)n : 0;
             c->io.reserve(sizeof(double) * (3 * ns + 3 * (size_t)m));

@@ -218,12 +218,12 @@ def test_potential_xcd_chunk_tiling(pkg, wl, precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("n", [(1 << 19) + 3, 1 << 21])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_host_targets_uploaded_beside_build(pkg, wl, precision):
-    # host target arrays of 2^21 points (two staging buffers and more) are uploaded
-    # by a helper thread on the second stream while the sources build: the results
+def test_host_targets_uploaded_beside_build(pkg, wl, precision, n):
+    # host target arrays of 4 MiB and more (2^19 points and more) are uploaded by
+    # a helper thread on the second stream while the sources build: the results
     # equal the device-pointer solve's, and a non-finite target is still refused
-    n = 1 << 21
     xs, ys, vs, xt, yt = wl.mixed_plummer_potential(n, seed=35)
     ctx = pkg.Context(precision=precision, pot_order=10, pot_leaf=192)
     host = ctx.potential(0.5, xs, ys, vs, xt, yt)
