@@ -9,6 +9,7 @@
 #include "tc_common.h"
 
 #include <cfloat>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -18,8 +19,7 @@ constexpr int RS_NT = 256;     // radix sort block size (4 waves)
 constexpr int RS_ITEMS = 16;   // keys per thread per tile
 constexpr int RS_TILE = RS_NT * RS_ITEMS;
 constexpr int RS_BINS = 256;   // 8-bit digits: passes at bits 0, 8, 16, 24 (30-bit keys)
-constexpr int RS_DIRECT_TILES = 64;  // up to this many tiles, no separate scan per pass
-constexpr int RS_ROW_ITEMS = 16;     // tile counts per thread in the per-digit row scan
+constexpr int KH_GRID = 1024;  // most workgroups of the key kernel (each adds its digit totals once)
 constexpr int SCAN_NT = 1024;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_CHUNK = SCAN_NT * SCAN_ITEMS;
@@ -225,11 +225,6 @@ TC_DEV void bbox_finish(const double4 *part, int np, double *box)
     }
 }
 
-__global__ __launch_bounds__(64) void k_bbox_final(const double4 *part, int np, double *box)
-{
-    bbox_finish(part, np, box);
-}
-
 // ------------------------------------------------------------------ keys
 TC_DEV uint32_t quantise(double v, double vmin, double scale)
 {
@@ -239,23 +234,6 @@ TC_DEV uint32_t quantise(double v, double vmin, double scale)
     if (q > (double)((1 << TC_LMAX) - 1))
         q = (double)((1 << TC_LMAX) - 1);
     return (uint32_t)q;
-}
-
-// sort-sources.cpp:87-110 (quantise + interleave, x in even bits).  With
-// `packed`, also writes the source record {x, y, v, 0} in input order, so the
-// gather after the sort reads one 32-byte record per source.
-__global__ __launch_bounds__(256) void k_keys(const double *x, const double *y, const double *v, int n,
-                                               const double *box, uint32_t *keys, double4 *packed)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n)
-        return;
-    const double xmin = box[0], ymin = box[1], scale = box[3];
-    const double xi = x[i], yi = y[i];
-    const uint32_t qx = quantise(xi, xmin, scale), qy = quantise(yi, ymin, scale);
-    keys[i] = tc_spread15(qx) | (tc_spread15(qy) << 1);
-    if (packed)
-        packed[i] = make_double4(xi, yi, v[i], 0.0);
 }
 
 // Hilbert index of the quantised point (the standard quadrant-rotation
@@ -284,73 +262,13 @@ TC_DEV uint32_t hilbert15(uint32_t x, uint32_t y)
     return d;
 }
 
-__global__ __launch_bounds__(256) void k_keys_hilbert(const double *x, const double *y, int n, const double *box,
-                                                       uint32_t *keys)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n)
-        return;
-    const double xmin = box[0], ymin = box[1], scale = box[3];
-    keys[i] = hilbert15(quantise(x[i], xmin, scale), quantise(y[i], ymin, scale));
-}
-
 // ------------------------------------------------------------------ radix sort
-// Stable LSD radix sort of (key, index) pairs, 8-bit digits.  Per pass:
-// k_rs_hist (tile histograms, digit-major), k_rs_rowscan (per digit, the
-// exclusive prefix of its counts over the tiles and the digit's total), and
-// k_rs_scatter, which places digit d of tile t at (keys of smaller digits) +
-// (digit d in earlier tiles), ranks the tile stably in LDS with wavefront
-// ballots and writes digit runs contiguously.
-__global__ __launch_bounds__(RS_NT) void k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *hist, int ntiles)
-{
-    __shared__ uint32_t h[RS_BINS];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int base = blockIdx.x * RS_TILE;
-#pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int i = base + j * RS_NT + threadIdx.x;
-        if (i < n)
-            atomicAdd(&h[(keys[i] >> shift) & (RS_BINS - 1)], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Block d scans row d of the digit-major tile histograms: offs[d][t] = count
-// of digit d in tiles 0..t-1, rowtot[d] = its count in all tiles.  One launch
-// of RS_BINS independent blocks (a scan of the whole table in digit-major
-// order would need three launches and a serial middle step).
-__global__ __launch_bounds__(RS_NT) void k_rs_rowscan(const uint32_t *hist, int ntiles, uint32_t *offs,
-                                                       uint32_t *rowtot)
-{
-    __shared__ uint32_t s_w[RS_NT / 64 + 1];
-    const uint32_t *row = hist + (size_t)blockIdx.x * ntiles;
-    uint32_t *orow = offs + (size_t)blockIdx.x * ntiles;
-    uint32_t carry = 0;
-    for (int base = 0; base < ntiles; base += RS_NT * RS_ROW_ITEMS) {
-        uint32_t v[RS_ROW_ITEMS];
-        uint32_t sum = 0;
-        const int i0 = base + threadIdx.x * RS_ROW_ITEMS;
-#pragma unroll
-        for (int j = 0; j < RS_ROW_ITEMS; ++j) {
-            v[j] = i0 + j < ntiles ? row[i0 + j] : 0u;
-            sum += v[j];
-        }
-        uint32_t tot;
-        uint32_t pre = block_scan_excl<RS_NT>(sum, s_w, &tot) + carry;
-#pragma unroll
-        for (int j = 0; j < RS_ROW_ITEMS; ++j) {
-            if (i0 + j < ntiles)
-                orow[i0 + j] = pre;
-            pre += v[j];
-        }
-        carry += tot;
-    }
-    if (threadIdx.x == 0)
-        rowtot[blockIdx.x] = carry;
-}
-
+// Stable LSD radix sort of (key, index) pairs, 8-bit digits, one launch per
+// pass (k_rs_pass_lb): each tile of RS_TILE keys is ranked stably in LDS with
+// wavefront ballots, digit d of tile t is placed at (keys of smaller digits,
+// from the digit totals the key kernel adds up) + (digit d in earlier tiles,
+// by look-back over the counts the earlier tiles publish), and the tile's digit
+// runs are written contiguously.
 // Lanes of the wave holding the same 8-bit digit d as this lane (among lanes
 // with ok set): a lane differs from this one where, for some bit, its ballot
 // bit differs from this lane's own bit.  The per-lane 0 / ~0 of each bit is a
@@ -460,42 +378,34 @@ TC_DEV void rs_tile(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, ui
     }
 }
 
-// A pass over more than RS_DIRECT_TILES tiles: offs, rowtot are
-// k_rs_rowscan's per-digit tile prefixes and digit totals.
-__global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                                                       uint32_t *vout, int n, int shift, const uint32_t *offs,
-                                                       const uint32_t *rowtot, int ntiles)
-{
-    rs_tile(kin, vin, kout, vout, n, shift, blockIdx.x, [&](uint32_t, uint32_t *s_w) {
-        // keys of smaller digits (scan of the digit totals) + this digit in earlier tiles
-        return block_scan_excl<RS_NT>(rowtot[threadIdx.x], s_w, nullptr) +
-               offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
-    });
-}
-
-// Sorts of at most RS_DIRECT_TILES tiles take each pass in one launch.  The
-// region (zeroed by k_bbox_partial before the keys) holds hist4[4][RS_BINS],
-// every pass's digit totals over all keys (k_keys_hist); for each pass the
-// tiles' published digit counts, flags[RS_DIRECT_TILES][RS_BINS]
-// (RS_PUB | count once published); and one ticket per pass.
+// Above SS_N points each pass is one launch.  The region (zeroed by
+// k_bbox_partial before the keys) holds hist4[4][RS_BINS], every pass's digit
+// totals over all keys (k_keys_hist); one ticket per pass (64 words); and for
+// each pass the tiles' published digit counts, flags[ntiles][RS_BINS] as 64-bit
+// words: status in the high half (LB_AGG: the tile's own count, LB_INC: its
+// count plus those of all earlier tiles), the count in the low half.
 constexpr int LB_HIST = 4 * RS_BINS;
-constexpr int LB_PASS_FLAGS = RS_DIRECT_TILES * RS_BINS;
-constexpr int LB_WORDS = LB_HIST + 4 * LB_PASS_FLAGS + 64;
-constexpr uint32_t RS_PUB = 1u << 31;
+constexpr int LB_TICKETS = 64;
+constexpr unsigned long long LB_AGG = 1ull << 32, LB_INC = 2ull << 32;
 constexpr int RS_SPIN_MAX = 1 << 22;  // polls of one unpublished count before the pass gives up
-constexpr int RS_LB_BATCH = 16;       // published counts read per round trip
+constexpr int RS_LB_BATCH = 4;        // earlier tiles' counts read per round trip
 
-// One pass of a sort of at most RS_DIRECT_TILES tiles.  Tiles are taken in
-// ticket order; a tile publishes its digit counts as soon as it has ranked its
-// keys and before it reads any other tile's, and reads only those of tiles
-// with smaller tickets, which have started before it and publish without
-// waiting: every wait ends.  Digit d of tile t starts at (keys of smaller
-// digits, from hist) + (digit d in tiles 0..t-1), exactly as with a scan of
-// the tile histograms.  A count that stays unpublished for RS_SPIN_MAX polls
-// sets bit 1 of *err (reported by the host as a failed sort) and is skipped.
+// 32-bit words of the region for a sort of ntiles tiles
+static size_t lb_words(int ntiles) { return (size_t)LB_HIST + LB_TICKETS + (size_t)4 * 2 * RS_BINS * ntiles; }
+
+// One pass of the sort (decoupled look-back).  Tiles are taken in ticket
+// order; a tile publishes its digit counts (LB_AGG) as soon as it has ranked
+// its keys, then walks back over the tiles before it, adding their counts,
+// until it meets one that has published LB_INC, and publishes its own LB_INC.
+// A tile only ever waits for an LB_AGG, which a tile with a smaller ticket
+// (started earlier) publishes without waiting: every wait ends.  Digit d of
+// tile t starts at (keys of smaller digits, from hist) + (digit d in tiles
+// 0..t-1), exactly as with a scan of the tile histograms.  A count that stays
+// unpublished for RS_SPIN_MAX polls sets bit 1 of *err (reported by the host
+// as a failed sort) and ends the walk.
 __global__ __launch_bounds__(RS_NT) void k_rs_pass_lb(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                        uint32_t *vout, int n, int shift, const uint32_t *hist,
-                                                       uint32_t *flags, uint32_t *ticket, uint32_t *err)
+                                                       unsigned long long *flags, uint32_t *ticket, uint32_t *err)
 {
     __shared__ int s_tile;
     if (threadIdx.x == 0)
@@ -504,49 +414,55 @@ __global__ __launch_bounds__(RS_NT) void k_rs_pass_lb(const uint32_t *kin, const
     const int tile = s_tile;
     rs_tile(kin, vin, kout, vout, n, shift, tile, [&](uint32_t tot, uint32_t *s_w) {
         const int d = threadIdx.x;
-        __hip_atomic_store(&flags[tile * RS_BINS + d], RS_PUB | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long *fd = flags + d;  // fd[t * RS_BINS]: digit d of tile t
+        __hip_atomic_store(&fd[(size_t)tile * RS_BINS], (tile ? LB_AGG : LB_INC) | tot, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
         uint32_t pre = 0;
-        int t0 = 0;
-        while (t0 < tile) {
-            // the next counts, RS_LB_BATCH loads in flight at once
-            uint32_t c[RS_LB_BATCH];
-            const int nt = min(RS_LB_BATCH, tile - t0);
+        bool done = tile == 0;
+        int t = tile;  // tiles t .. tile-1 are summed
+        while (!done) {
+            // the counts of tiles t-1, t-2, ..., RS_LB_BATCH loads in flight at once
+            const int nt = min(RS_LB_BATCH, t);
+            unsigned long long c[RS_LB_BATCH];
 #pragma unroll
             for (int j = 0; j < RS_LB_BATCH; ++j)
-                c[j] = j < nt ? __hip_atomic_load(&flags[(t0 + j) * RS_BINS + d], __ATOMIC_RELAXED,
+                c[j] = j < nt ? __hip_atomic_load(&fd[(size_t)(t - 1 - j) * RS_BINS], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT)
-                              : RS_PUB;
-            int j = 0;
-            while (j < nt && (c[j] & RS_PUB)) {
-                pre += c[j] & ~RS_PUB;
-                ++j;
-            }
-            t0 += j;
-            if (j < nt) {
-                // tile t0 has not published yet: wait for it
-                uint32_t f = 0;
-                for (int spin = 0; spin < RS_SPIN_MAX; ++spin) {
-                    f = __hip_atomic_load(&flags[t0 * RS_BINS + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (f & RS_PUB)
-                        break;
-                    __builtin_amdgcn_s_sleep(2);
+                              : 0ull;
+#pragma unroll
+            for (int j = 0; j < RS_LB_BATCH; ++j) {
+                if (!done && j < nt) {
+                    unsigned long long w = c[j];
+                    // tile t-1-j has not published yet: wait for it
+                    for (int spin = 0; !(w >> 32) && spin < RS_SPIN_MAX; ++spin) {
+                        __builtin_amdgcn_s_sleep(2);
+                        w = __hip_atomic_load(&fd[(size_t)(t - 1 - j) * RS_BINS], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    if (!(w >> 32))
+                        atomicOr(err, 2u);
+                    pre += (uint32_t)w;
+                    done = (w & LB_INC) || !(w >> 32);
                 }
-                if (!(f & RS_PUB))
-                    atomicOr(err, 2u);
-                pre += f & ~RS_PUB;
-                ++t0;
             }
+            t -= nt;
         }
+        if (tile)
+            __hip_atomic_store(&fd[(size_t)tile * RS_BINS], LB_INC | (pre + tot), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         return block_scan_excl<RS_NT>(hist[d], s_w, nullptr) + pre;
     });
 }
 
-// Keys as k_keys / k_keys_hilbert (RS_TILE per workgroup) for a sort of at
-// most RS_DIRECT_TILES tiles, plus the digit totals of all four passes, added
-// into hist4 (zeroed before) once per workgroup.  Each workgroup finishes the
-// bounding box from k_bbox_partial's np partials and per-workgroup flags
-// itself (as k_bbox_final would; workgroup 0 writes it and the non-finite flag
-// to box_out and bad_out).
+// Keys of the points: sources by sort-sources.cpp:87-110 (quantise +
+// interleave, x in even bits), with `packed` also the source record
+// {x, y, v, 0} in input order, so the gather after the sort reads one 32-byte
+// record per source; targets (HILBERT) by hilbert15.  Tiles of RS_TILE points
+// are taken in turn by at most KH_GRID workgroups, and each workgroup adds the
+// digit totals of all four passes into hist4 (zeroed before) once.  Each
+// workgroup finishes the bounding box from k_bbox_partial's np partials and
+// per-workgroup flags itself (workgroup 0 writes it and the non-finite flag to
+// box_out and bad_out).
 template <bool HILBERT>
 __global__ __launch_bounds__(256) void k_keys_hist(const double *x, const double *y, const double *v, int n,
                                                     const double4 *part, const uint32_t *pbad, int np,
@@ -572,21 +488,22 @@ __global__ __launch_bounds__(256) void k_keys_hist(const double *x, const double
         }
     }
     const double xmin = s_box[0], ymin = s_box[1], scale = s_box[3];
-    const int base = blockIdx.x * RS_TILE;
+    for (long long base = (long long)blockIdx.x * RS_TILE; base < n; base += (long long)gridDim.x * RS_TILE) {
 #pragma unroll 4
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int i = base + j * 256 + threadIdx.x;
-        if (i < n) {
-            const double xi = x[i], yi = y[i];
-            const uint32_t qx = quantise(xi, xmin, scale), qy = quantise(yi, ymin, scale);
-            const uint32_t k = HILBERT ? hilbert15(qx, qy) : (tc_spread15(qx) | (tc_spread15(qy) << 1));
-            keys[i] = k;
-            if (!HILBERT && packed)
-                packed[i] = make_double4(xi, yi, v[i], 0.0);
-            atomicAdd(&h[k & 255u], 1u);
-            atomicAdd(&h[RS_BINS + ((k >> 8) & 255u)], 1u);
-            atomicAdd(&h[2 * RS_BINS + ((k >> 16) & 255u)], 1u);
-            atomicAdd(&h[3 * RS_BINS + (k >> 24)], 1u);
+        for (int j = 0; j < RS_ITEMS; ++j) {
+            const long long i = base + j * 256 + threadIdx.x;
+            if (i < n) {
+                const double xi = x[i], yi = y[i];
+                const uint32_t qx = quantise(xi, xmin, scale), qy = quantise(yi, ymin, scale);
+                const uint32_t k = HILBERT ? hilbert15(qx, qy) : (tc_spread15(qx) | (tc_spread15(qy) << 1));
+                keys[i] = k;
+                if (!HILBERT && packed)
+                    packed[i] = make_double4(xi, yi, v[i], 0.0);
+                atomicAdd(&h[k & 255u], 1u);
+                atomicAdd(&h[RS_BINS + ((k >> 8) & 255u)], 1u);
+                atomicAdd(&h[2 * RS_BINS + ((k >> 16) & 255u)], 1u);
+                atomicAdd(&h[3 * RS_BINS + (k >> 24)], 1u);
+            }
         }
     }
     __syncthreads();
@@ -596,12 +513,11 @@ __global__ __launch_bounds__(256) void k_keys_hist(const double *x, const double
 }
 
 // ------------------------------------------------------------------ small sorts
-// Up to SS_N points in one workgroup: bounding box (k_bbox_partial /
-// k_bbox_final semantics), keys (Morton as k_keys or Hilbert as
-// k_keys_hilbert), the same stable 4-pass LSD radix sort as above with the
-// keys in registers and each pass's scatter in LDS, and for sources the gather
-// of the sorted records.  One launch instead of about sixteen, which is what a
-// small solve is made of.
+// Up to SS_N points in one workgroup: bounding box (k_bbox_partial +
+// bbox_finish semantics), keys (Morton or Hilbert as k_keys_hist), the same
+// stable 4-pass LSD radix sort as above with the keys in registers and each
+// pass's scatter in LDS, and for sources the gather of the sorted records.
+// One launch instead of seven, which is what a small solve is made of.
 constexpr int SS_NT = 1024, SS_ITEMS = 16, SS_N = SS_NT * SS_ITEMS;
 static_assert(SS_N == TC_SORT_SMALL_MAX, "one-workgroup sort size");
 
@@ -701,7 +617,7 @@ TC_DEV void sort_small(const double *__restrict__ x, const double *__restrict__ 
         for (int d = lane; d < RS_BINS; d += 64)
             s_wrun[wid][d] = 0;
         tc_wave_sync();
-        // wave-local stable ranks, as in k_rs_scatter
+        // wave-local stable ranks, as in rs_tile
 #pragma unroll
         for (int j = 0; j < SS_ITEMS; ++j) {
             const bool ok = wid * WITEMS + j * 64 + lane < n;
@@ -1487,19 +1403,6 @@ void tc_upload_binomials(hipStream_t st)
     TC_HIP(hipStreamSynchronize(st));
 }
 
-void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st, uint32_t *zero,
-             int nzero)
-{
-    const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
-    ws.bbox_part.reserve(sizeof(double4) * (size_t)np + 256);
-    double4 *part = ws.bbox_part.as<double4>();
-    uint32_t *bad = reinterpret_cast<uint32_t *>(part + np);
-    tc_fill_u32(bad, 0u, 1, st);
-    k_bbox_partial<<<np, 256, 0, st>>>(x, y, n, part, bad, nullptr, zero, nzero);
-    k_bbox_final<<<1, 64, 0, st>>>(part, np, box_d);
-    TC_HIP(hipGetLastError());
-}
-
 static const uint32_t *tc_bbox_bad_flag(TcScratch &ws, int n)
 {
     const int np = std::max(1, std::min(1024, tiles_of(n, 256 * 8)));
@@ -1544,19 +1447,16 @@ void tc_sort_points(const double *x, const double *y, const double *v, int n, do
         }
         return;
     }
-    // up to RS_DIRECT_TILES tiles the radix passes take one launch each; their
-    // counters are zeroed by the bounding-box kernel, whose partials the key
-    // kernel finishes
-    uint32_t *lb = nullptr;
-    if (tiles_of(n, RS_TILE) <= RS_DIRECT_TILES) {
-        ws.hist.reserve(sizeof(uint32_t) * LB_WORDS);
-        lb = ws.hist.as<uint32_t>();
-        const LbBox b = lb_box(ws, n);
-        k_bbox_partial<<<b.np, 256, 0, st>>>(x, y, n, b.part, nullptr, b.pbad, lb, LB_WORDS);
-        TC_HIP(hipGetLastError());
-    } else {
-        tc_bbox(x, y, n, box_d, ws, st);
-    }
+    // the radix passes take one launch each; their counters are zeroed by the
+    // bounding-box kernel, whose partials the key kernel finishes
+    const size_t words = lb_words(tiles_of(n, RS_TILE));
+    if (words > (size_t)INT_MAX)
+        throw TcError(-3, "tc_sort_points: too many points");
+    ws.hist.reserve(sizeof(uint32_t) * words);
+    uint32_t *lb = ws.hist.as<uint32_t>();
+    const LbBox b = lb_box(ws, n);
+    k_bbox_partial<<<b.np, 256, 0, st>>>(x, y, n, b.part, nullptr, b.pbad, lb, (int)words);
+    TC_HIP(hipGetLastError());
     double4 *packed = nullptr;
     if (src_out) {
         ws.packed.reserve(sizeof(double4) * (size_t)n);
@@ -1584,48 +1484,26 @@ void tc_morton_sort(const double *x, const double *y, const double *v, int n, co
     uint32_t *vin[4] = {nullptr, ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>()};
     uint32_t *kout[4] = {ws.k2.as<uint32_t>(), keys_out, ws.k2.as<uint32_t>(), keys_out};
     uint32_t *vout[4] = {ws.v2.as<uint32_t>(), perm_out, ws.v2.as<uint32_t>(), perm_out};
-    if (lb) {
-        if (ntiles > RS_DIRECT_TILES)
-            throw TcError(-3, "tc_morton_sort: one-launch passes take at most 64 tiles");
-        // the box and keys with all four passes' digit totals, then one launch
-        // per pass (the bounding-box partials were made by the caller)
-        uint32_t *hist4 = lb, *flags = lb + LB_HIST, *tickets = flags + 4 * LB_PASS_FLAGS;
-        const LbBox b = lb_box(ws, n);
-        uint32_t *err = b.bad;
-        if (hilbert)
-            k_keys_hist<true><<<ntiles, 256, 0, st>>>(x, y, nullptr, n, b.part, b.pbad, b.np, const_cast<double *>(box_d),
-                                                      b.bad, keys_out, nullptr, hist4);
-        else
-            k_keys_hist<false><<<ntiles, 256, 0, st>>>(x, y, v, n, b.part, b.pbad, b.np, const_cast<double *>(box_d),
-                                                       b.bad, keys_out, packed, hist4);
-        TC_HIP(hipGetLastError());
-        if (ev)
-            TC_HIP(hipEventRecord(ev[0], st));
-        for (int pass = 0; pass < 4; ++pass) {
-            k_rs_pass_lb<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, 8 * pass,
-                                                    hist4 + RS_BINS * pass, flags + LB_PASS_FLAGS * pass,
-                                                    tickets + pass, err);
-            TC_HIP(hipGetLastError());
-        }
-        return;
-    }
+    // the box and keys with all four passes' digit totals, then one launch per
+    // pass (the bounding-box partials and the zeroed region lb were made by the caller)
+    uint32_t *hist4 = lb, *tickets = lb + LB_HIST;
+    unsigned long long *flags = reinterpret_cast<unsigned long long *>(tickets + LB_TICKETS);
+    const LbBox b = lb_box(ws, n);
+    uint32_t *err = b.bad;
+    const int grid = std::min(ntiles, KH_GRID);
     if (hilbert)
-        k_keys_hilbert<<<tiles_of(n, 256), 256, 0, st>>>(x, y, n, box_d, keys_out);
+        k_keys_hist<true><<<grid, 256, 0, st>>>(x, y, nullptr, n, b.part, b.pbad, b.np, const_cast<double *>(box_d),
+                                                b.bad, keys_out, nullptr, hist4);
     else
-        k_keys<<<tiles_of(n, 256), 256, 0, st>>>(x, y, v, n, box_d, keys_out, packed);
+        k_keys_hist<false><<<grid, 256, 0, st>>>(x, y, v, n, b.part, b.pbad, b.np, const_cast<double *>(box_d),
+                                                 b.bad, keys_out, packed, hist4);
     TC_HIP(hipGetLastError());
     if (ev)
         TC_HIP(hipEventRecord(ev[0], st));
-    ws.hist.reserve(sizeof(uint32_t) * ((size_t)RS_BINS * ntiles * 2 + RS_BINS));
-    uint32_t *hist = ws.hist.as<uint32_t>();
-    uint32_t *offs = hist + (size_t)RS_BINS * ntiles;
-    uint32_t *rowtot = offs + (size_t)RS_BINS * ntiles;
     for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 8 * pass;
-        k_rs_hist<<<ntiles, RS_NT, 0, st>>>(kin[pass], n, shift, hist, ntiles);
-        k_rs_rowscan<<<RS_BINS, RS_NT, 0, st>>>(hist, ntiles, offs, rowtot);
-        k_rs_scatter<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, shift, offs, rowtot,
-                                                ntiles);
+        k_rs_pass_lb<<<ntiles, RS_NT, 0, st>>>(kin[pass], vin[pass], kout[pass], vout[pass], n, 8 * pass,
+                                                hist4 + RS_BINS * pass, flags + (size_t)RS_BINS * ntiles * pass,
+                                                tickets + pass, err);
         TC_HIP(hipGetLastError());
     }
 }

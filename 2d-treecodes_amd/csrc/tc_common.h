@@ -356,23 +356,20 @@ size_t tc_noalloc_scratch_bytes();
 void *tc_noalloc_scratch_lookup(int device);
 
 // Internal entry points (tc_build.hip, tc_eval.hip)
-// zero, nzero: words the bounding-box kernel also clears (counters of the sort that follows)
-void tc_bbox(const double *x, const double *y, int n, double *box_d, TcScratch &ws, hipStream_t st,
-             uint32_t *zero = nullptr, int nzero = 0);
 // Bounding box, keys (Morton, or Hilbert for targets) and stable sort of n
 // points; with src_out also the sorted records {x, y, v, 0}.  One workgroup
-// for n <= 16384, else the multi-kernel path.
+// for n <= 16384, else a bounding-box partial kernel and tc_morton_sort.
 // ev (optional, two events): recorded after the keys and after the sorted records.
 void tc_sort_points(const double *x, const double *y, const double *v, int n, double *box_d, uint32_t *keys_out,
                     uint32_t *perm_out, double4 *src_out, TcScratch &ws, hipStream_t st, bool hilbert,
                     const hipEvent_t *ev = nullptr);
-// lb: for n of at most 64 tiles of 4096, the counters of the one-launch radix
-// passes, zeroed beforehand by the bounding-box partial kernel, whose partials
-// the key kernel then finishes into box_d (as tc_sort_points does); null:
-// tc_bbox has written box_d and the passes take several launches.
+// Keys and the four one-launch radix passes.  lb: the passes' digit totals,
+// tickets and published tile counts, zeroed beforehand by the bounding-box
+// partial kernel, whose partials the key kernel then finishes into box_d (as
+// tc_sort_points does).
 void tc_morton_sort(const double *x, const double *y, const double *v, int n, const double *box_d,
                     uint32_t *keys_out, uint32_t *perm_out, double4 *packed, TcScratch &ws, hipStream_t st,
-                    bool hilbert = false, const hipEvent_t *ev = nullptr, uint32_t *lb = nullptr);
+                    bool hilbert, const hipEvent_t *ev, uint32_t *lb);
 // fp32: also build the FP32 mirrors; deriv32: the coefficient mirror holds
 // k b_k (force trees) instead of b_k (potential trees).
 // ev (optional, four events): recorded after the Morton keys, the sorted
