@@ -658,7 +658,8 @@ def main():
                     help="replay the stream only without cross-request batching")
     ap.add_argument("--tune", action="store_true",
                     help="re-run the autotuner for this workload (grid of order x leaf) before timing")
-    ap.add_argument("--err-target", type=float, default=5e-7, help="autotuner error target (default 5e-7)")
+    ap.add_argument("--err-target", type=float, default=8e-7,
+                    help="autotuner error target (default 8e-7, a 20%% margin under the metric's 1e-6)")
     ap.add_argument("--tune-cache", default="", metavar="PATH",
                     help="autotuner cache to write (default 2d-treecodes_amd/tuned.json)")
     ap.add_argument("--tune-tiling", action="store_true",

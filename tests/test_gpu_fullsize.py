@@ -129,8 +129,9 @@ def _sample(n, k, seed):
     return np.sort(np.random.default_rng(seed).choice(n, k, replace=False))
 
 
-# reference order / leaf (treecode-potential.cpp:99, Makefile:13) and the tuned FP32 / FP64 ones
-@pytest.mark.parametrize("order,leaf", [(12, 512), (10, 192), (10, 64)])
+# reference order / leaf (treecode-potential.cpp:99, Makefile:13), the tuned FP32 / FP64 ones at
+# theta 0.5 (9 / 128, 9 / 64) and 0.7 (14 / 192), and 10 / 192
+@pytest.mark.parametrize("order,leaf", [(12, 512), (9, 128), (9, 64), (14, 192), (10, 192)])
 def test_potential_2e24_fp64_vs_oracle(pkg, oracle, c3, order, leaf):
     xs, ys, vs, xt, yt = c3
     ctx = pkg.Context(precision=pkg.FP64, pot_order=order, pot_leaf=leaf, counters=True)
