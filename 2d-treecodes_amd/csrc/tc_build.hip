@@ -16,7 +16,7 @@
 namespace {
 
 constexpr int RS_NT = 256;     // radix sort block size (4 waves)
-constexpr int RS_ITEMS = 16;   // keys per thread per tile
+constexpr int RS_ITEMS = 20;   // keys per thread per tile
 constexpr int RS_TILE = RS_NT * RS_ITEMS;
 constexpr int RS_BINS = 256;   // 8-bit digits: passes at bits 0, 8, 16, 24 (30-bit keys)
 constexpr int KH_GRID = 1024;  // most workgroups of the key kernel (each adds its digit totals once)
@@ -311,22 +311,25 @@ TC_DEV void rs_tile(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, ui
         s_wrun[wid][d] = 0;
     tc_wave_sync();
 
-    // Each wave ranks its own contiguous 1024 keys (item j of lane l is key
-    // base + wid*1024 + 64 j + l), so the order (wave, j, lane) is the input
+    // Each wave ranks its own contiguous WITEMS keys (item j of lane l is key
+    // base + wid*WITEMS + 64 j + l), so the order (wave, j, lane) is the input
     // order and the ranks are stable without block barriers: peers with the
     // same digit are found with 8 ballots, the wave's running count per digit
     // lives in s_wrun[wid] (all lanes read it before the leader updates it).
+    // Bounds are tested on the offset within the tile, so base + offset is
+    // only formed below n (no int overflow for n near INT_MAX).
     uint32_t k[RS_ITEMS], v[RS_ITEMS], dr[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        const int i = base + wid * WITEMS + j * 64 + lane;
-        const bool ok = i < n;
+        const int o = wid * WITEMS + j * 64 + lane;
+        const bool ok = o < cnt;
+        const int i = base + (ok ? o : 0);
         k[j] = ok ? kin[i] : 0u;
         v[j] = ok ? (vin ? vin[i] : (uint32_t)i) : 0u;
     }
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        const bool ok = base + wid * WITEMS + j * 64 + lane < n;
+        const bool ok = wid * WITEMS + j * 64 + lane < cnt;
         const uint32_t d = (k[j] >> shift) & (RS_BINS - 1);
         const unsigned long long peers = wave_digit_peers(d, ok);
         const uint32_t lower = __popcll(peers & tc_lanemask_lt());
@@ -361,7 +364,7 @@ TC_DEV void rs_tile(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, ui
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        if (base + wid * WITEMS + j * 64 + lane < n) {
+        if (wid * WITEMS + j * 64 + lane < cnt) {
             const uint32_t pos = s_wrun[wid][dr[j] & 255u] + (dr[j] >> 8);
             s_k[pos] = k[j];
             s_v[pos] = v[j];

@@ -29,11 +29,11 @@ def _device_sort(pkg, ctx, x, y):
     return box.numpy(), keys.numpy(), perm.numpy()
 
 
-# up to 16384 points the sort runs in one workgroup, above it in tiles of 4096
-# whose passes look back over the counts earlier tiles publish; 1101 tiles are
-# more than the GPU holds at once, so the last tiles start after the first finish
-@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 16384, 16385, 100003, 262144, 262145, 1000003,
-                               1100 * 4096 + 9])
+# up to 16384 points the sort runs in one workgroup, above it in tiles of 5120
+# whose passes look back over the counts earlier tiles publish; 881 tiles are
+# more than the GPU holds at once (768), so the last tiles start after the first finish
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 16384, 16385, 20480, 20481, 100003, 262144, 262145,
+                               1000003, 1100 * 4096 + 9])
 def test_morton_sort_bit_exact(pkg, oracle, wl, ctx64, n):
     st = wl.Stream(100 + n)
     x, y = st.uniform(n), st.uniform(n)
@@ -63,7 +63,7 @@ def test_morton_sort_duplicates_stable(pkg, oracle, wl, ctx64):
 def test_morton_sort_two_keys_stable(pkg, oracle, wl, ctx64, n):
     # two distinct keys over many tiles: each 1024-key run of the sort holds
     # hundreds of equal digits, so ranks within a run reach their maximum
-    # (48 and 81 tiles, all resident at once, so every tile looks back)
+    # (39 and 65 tiles, all resident at once, so every tile looks back)
     st = wl.Stream(6)
     pick = st.uniform(n) < 0.5
     x = np.where(pick, 0.1, 0.9)
